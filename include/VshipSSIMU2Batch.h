@@ -1,0 +1,118 @@
+/*
+ * Additive SSIMULACRA2 entry points of the MI355X build: batched frame pairs
+ * of uneven resolution already resident in device memory, and a continuous-
+ * batching request server. These symbols have no counterpart in the
+ * reference's src/VshipAPI.h; the reference entry points are unchanged
+ * (VshipAPI.h).
+ */
+#ifndef VSHIP_SSIMU2_BATCH_HEADER
+#define VSHIP_SSIMU2_BATCH_HEADER
+
+#include "VshipAPI.h"
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* Sample formats. RGB formats are the reference's InputMemType
+ * (src/util/preprocessor.hpp:94): planar R, G, B, gamma-encoded (BT.709
+ * transfer), u8/u16 full range or f16/f32 in [0,1]. The YUV 4:2:0 formats
+ * are limited range; chroma is upsampled with Catmull-Rom cubic weights
+ * (left-sited horizontally, centre-sited vertically, edge samples clamped)
+ * and converted with the given matrix before the same pipeline runs. */
+typedef enum {
+    Vship_PF_RGB_U8 = 0,
+    Vship_PF_RGB_U16 = 1,
+    Vship_PF_RGB_F16 = 2,
+    Vship_PF_RGB_F32 = 3,
+    Vship_PF_YUV420_U8 = 4,
+    Vship_PF_YUV420_U16 = 5, /* 9..16-bit samples in little-endian 16-bit words */
+} Vship_PixelFormat;
+
+typedef enum {
+    Vship_Matrix_BT709 = 0,
+    Vship_Matrix_BT601 = 1,
+    Vship_Matrix_BT2020NCL = 2,
+} Vship_Matrix;
+
+/* One frame pair. Plane pointers are device pointers; strides in bytes. */
+typedef struct {
+    const uint8_t* ref[3];
+    const uint8_t* dist[3];
+    int64_t ref_stride[3];
+    int64_t dist_stride[3];
+    int width;
+    int height;
+    int format; /* Vship_PixelFormat */
+    int bits;   /* sample depth for Vship_PF_YUV420_U16 (9..16); ignored otherwise */
+    int matrix; /* Vship_Matrix for the YUV formats */
+    int reserved;
+} Vship_SSIMU2Pair;
+
+typedef struct {
+    int id;
+} Vship_SSIMU2BatchHandler;
+
+typedef struct {
+    int max_pairs;         /* pairs per batch */
+    int max_width;         /* largest frame: sizes the default arena */
+    int max_height;
+    int fp16_planes;       /* 1: store the XYB pyramid in FP16 (FP32 math, FP64 sums) */
+    uint64_t arena_bytes;  /* 0: max_pairs * Vship_SSIMU2PairFootprint(max_width, max_height, fp16_planes) */
+    int map_tile_height;   /* 0: tuned default; else 32 or 64 */
+    int persistent_grid;   /* 0: one workgroup per tile; else fixed grid of this many workgroups */
+    int use_graph;         /* 1: capture the batch launch into a hipGraph once and replay it */
+    int profile;           /* 1: time each kernel with HIP events (eager launches only) */
+} Vship_SSIMU2BatchConfig;
+
+/* Device bytes one in-flight pair needs in the XYB arena (both images, six scales). */
+EXPORTPREPROCESS uint64_t Vship_SSIMU2PairFootprint(int width, int height, int fp16_planes);
+
+EXPORTPREPROCESS Vship_Exception Vship_SSIMU2BatchInit(Vship_SSIMU2BatchHandler* handler,
+                                                       const Vship_SSIMU2BatchConfig* config);
+EXPORTPREPROCESS Vship_Exception Vship_SSIMU2BatchFree(Vship_SSIMU2BatchHandler handler);
+
+/* Enqueue a batch on the handler's stream (asynchronous). */
+EXPORTPREPROCESS Vship_Exception Vship_SSIMU2BatchSubmit(Vship_SSIMU2BatchHandler handler,
+                                                         const Vship_SSIMU2Pair* pairs, int npairs);
+/* Wait for the last submitted batch; scores[npairs] and, if not NULL,
+ * features[npairs*108] (reference feature order) receive the results. */
+EXPORTPREPROCESS Vship_Exception Vship_SSIMU2BatchWait(Vship_SSIMU2BatchHandler handler, double* scores,
+                                                       double* features);
+/* Submit + Wait. */
+EXPORTPREPROCESS Vship_Exception Vship_SSIMU2BatchCompute(Vship_SSIMU2BatchHandler handler,
+                                                          const Vship_SSIMU2Pair* pairs, int npairs, double* scores,
+                                                          double* features);
+
+typedef struct {
+    float prep_ms;      /* convert + pyramid + XYB kernel */
+    float map_ms;       /* blur + SSIM/edge maps, both precision classes */
+    float map_fp32_ms;  /* map kernel of the segments with FP32 moment accumulation */
+    float map_fp64_ms;  /* map kernel of the small segments (FP64 moment accumulation) */
+    float finalize_ms;  /* FP64 norms + score kernel */
+    float total_ms;     /* whole batch, first copy to last copy */
+    int npairs;
+    int prep_tiles;
+    int map_tiles;
+    int map_tiles_fp32;
+    int map_tiles_fp64;
+    int map_tile_height;
+    uint64_t map_bytes;      /* algorithmic bytes the map kernels read (both images, all scales) */
+    uint64_t map_bytes_fp32; /* the part read by the FP32 map kernel */
+    uint64_t prep_bytes;     /* algorithmic bytes the prep kernel reads and writes */
+} Vship_SSIMU2BatchStats;
+
+/* Per-kernel HIP-event timings of the last waited batch (profile = 1). */
+EXPORTPREPROCESS Vship_Exception Vship_SSIMU2BatchGetStats(Vship_SSIMU2BatchHandler handler,
+                                                           Vship_SSIMU2BatchStats* stats);
+
+/* Device memory helpers so callers without a GPU framework can stage inputs. */
+EXPORTPREPROCESS Vship_Exception Vship_DeviceMalloc(void** ptr, uint64_t size);
+EXPORTPREPROCESS Vship_Exception Vship_DeviceFree(void* ptr);
+EXPORTPREPROCESS Vship_Exception Vship_MemcpyHtoD(void* dst, const void* src, uint64_t size);
+EXPORTPREPROCESS Vship_Exception Vship_MemcpyDtoH(void* dst, const void* src, uint64_t size);
+
+#ifdef __cplusplus
+} /* extern "C" */
+#endif
+#endif /* VSHIP_SSIMU2_BATCH_HEADER */

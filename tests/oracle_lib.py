@@ -194,3 +194,42 @@ def get_threads():
 
 SO_RGB_U8, SO_RGB_U16, SO_RGB_F16, SO_RGB_F32 = 0, 1, 2, 3
 SO_MAT_BT709, SO_MAT_BT601, SO_MAT_BT2020NCL = 0, 1, 2
+
+
+# ---------------------------------------------------------------------------
+# the reference's own HIP implementation, built from its sources by
+# `make -C oracle ref` (GPU only; absent where the sources were not mounted)
+# ---------------------------------------------------------------------------
+_ref = None
+
+
+def ref_available():
+    return os.path.exists(REF_PATH)
+
+
+def ref_lib():
+    global _ref
+    if _ref is None:
+        R = ctypes.CDLL(REF_PATH)
+        R.vref_ssimu2.restype = ctypes.c_int
+        R.vref_ssimu2.argtypes = [ctypes.c_int, P3, P3, I64, I64, ctypes.c_int, ctypes.c_int, D]
+        R.vref_ssimu2_features.restype = ctypes.c_int
+        R.vref_ssimu2_features.argtypes = [ctypes.c_int, P3, P3, I64, I64, ctypes.c_int, ctypes.c_int, F]
+        _ref = R
+    return _ref
+
+
+def ref_score_rgb(fmt, img1, img2):
+    """Score and features of the reference's HIP code on planar RGB (same formats as score_rgb)."""
+    p1 = [np.ascontiguousarray(p) for p in img1]
+    p2 = [np.ascontiguousarray(p) for p in img2]
+    h, w = p1[0].shape
+    s = ctypes.c_double()
+    R = ref_lib()
+    rc = R.vref_ssimu2(fmt, _planes(p1), _planes(p2), p1[0].strides[0], p2[0].strides[0], w, h, ctypes.byref(s))
+    assert rc == 0, rc
+    f = np.zeros(108, np.float32)
+    rc = R.vref_ssimu2_features(fmt, _planes(p1), _planes(p2), p1[0].strides[0], p2[0].strides[0], w, h,
+                                f.ctypes.data_as(F))
+    assert rc == 0, rc
+    return s.value, f.astype(np.float64)

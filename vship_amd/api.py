@@ -1,0 +1,369 @@
+"""ctypes binding of libvship.so (see the package docstring)."""
+import ctypes
+import os
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+
+
+def lib_path():
+    return os.environ.get("VSHIP_LIB", os.path.join(_HERE, "libvship.so"))
+
+
+# Vship_Exception values, src/VshipAPI.h:35-58 of the reference
+ERRORS = [
+    "NoError", "OutOfVRAM", "OutOfRAM", "DifferingInputType", "NonRGBSInput", "DeviceCountError",
+    "NoDeviceDetected", "BadDeviceArgument", "BadDeviceCode", "BadHandler", "BadPointer", "BadErrorType",
+]
+
+# Vship_PixelFormat / Vship_Matrix (include/VshipSSIMU2Batch.h)
+PF_RGB_U8, PF_RGB_U16, PF_RGB_F16, PF_RGB_F32, PF_YUV420_U8, PF_YUV420_U16 = range(6)
+MATRIX_BT709, MATRIX_BT601, MATRIX_BT2020NCL = range(3)
+
+
+class VshipNotBuilt(RuntimeError):
+    pass
+
+
+class VshipError(RuntimeError):
+    def __init__(self, code, where=""):
+        self.code = int(code)
+        self.name = ERRORS[self.code] if 0 <= self.code < len(ERRORS) else f"code {self.code}"
+        super().__init__(f"{self.name}: {error_message(self.code)} {where}".strip())
+
+
+class Version(ctypes.Structure):
+    _fields_ = [("major", ctypes.c_int), ("minor", ctypes.c_int), ("minorMinor", ctypes.c_int),
+                ("backend", ctypes.c_int)]
+
+
+class DeviceInfo(ctypes.Structure):
+    _fields_ = [("name", ctypes.c_char * 256), ("VRAMSize", ctypes.c_uint64), ("integrated", ctypes.c_int),
+                ("MultiProcessorCount", ctypes.c_int), ("WarpSize", ctypes.c_int)]
+
+
+class Handler(ctypes.Structure):
+    _fields_ = [("id", ctypes.c_int)]
+
+
+class Pair(ctypes.Structure):
+    _fields_ = [("ref", ctypes.c_void_p * 3), ("dist", ctypes.c_void_p * 3),
+                ("ref_stride", ctypes.c_int64 * 3), ("dist_stride", ctypes.c_int64 * 3),
+                ("width", ctypes.c_int), ("height", ctypes.c_int), ("format", ctypes.c_int),
+                ("bits", ctypes.c_int), ("matrix", ctypes.c_int), ("reserved", ctypes.c_int)]
+
+
+class BatchConfig(ctypes.Structure):
+    _fields_ = [("max_pairs", ctypes.c_int), ("max_width", ctypes.c_int), ("max_height", ctypes.c_int),
+                ("fp16_planes", ctypes.c_int), ("arena_bytes", ctypes.c_uint64),
+                ("map_tile_height", ctypes.c_int), ("persistent_grid", ctypes.c_int),
+                ("use_graph", ctypes.c_int), ("profile", ctypes.c_int)]
+
+
+class BatchStats(ctypes.Structure):
+    _fields_ = [("prep_ms", ctypes.c_float), ("map_ms", ctypes.c_float), ("map_fp32_ms", ctypes.c_float),
+                ("map_fp64_ms", ctypes.c_float), ("finalize_ms", ctypes.c_float), ("total_ms", ctypes.c_float),
+                ("npairs", ctypes.c_int), ("prep_tiles", ctypes.c_int), ("map_tiles", ctypes.c_int),
+                ("map_tiles_fp32", ctypes.c_int), ("map_tiles_fp64", ctypes.c_int),
+                ("map_tile_height", ctypes.c_int), ("map_bytes", ctypes.c_uint64),
+                ("map_bytes_fp32", ctypes.c_uint64), ("prep_bytes", ctypes.c_uint64)]
+
+
+_lib = None
+P3 = ctypes.c_void_p * 3
+
+
+def lib():
+    """The loaded native library; raises VshipNotBuilt when it is missing."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    path = lib_path()
+    if not os.path.exists(path):
+        raise VshipNotBuilt(f"{path} is not built: run `python -m vship_amd.build`")
+    L = ctypes.CDLL(path)
+    E = ctypes.c_int
+    L.Vship_GetVersion.restype = Version
+    L.Vship_GetDeviceCount.restype = E
+    L.Vship_GetDeviceCount.argtypes = [ctypes.POINTER(ctypes.c_int)]
+    L.Vship_SetDevice.restype = E
+    L.Vship_SetDevice.argtypes = [ctypes.c_int]
+    L.Vship_GetDeviceInfo.restype = E
+    L.Vship_GetDeviceInfo.argtypes = [ctypes.POINTER(DeviceInfo), ctypes.c_int]
+    L.Vship_GPUFullCheck.restype = E
+    L.Vship_GPUFullCheck.argtypes = [ctypes.c_int]
+    L.Vship_GetErrorMessage.restype = ctypes.c_int
+    L.Vship_GetErrorMessage.argtypes = [E, ctypes.c_char_p, ctypes.c_int]
+    L.Vship_PinnedMalloc.restype = E
+    L.Vship_PinnedMalloc.argtypes = [ctypes.POINTER(ctypes.c_void_p), ctypes.c_uint64]
+    L.Vship_PinnedFree.restype = E
+    L.Vship_PinnedFree.argtypes = [ctypes.c_void_p]
+    L.Vship_SSIMU2Init.restype = E
+    L.Vship_SSIMU2Init.argtypes = [ctypes.POINTER(Handler), ctypes.c_int, ctypes.c_int]
+    L.Vship_SSIMU2Free.restype = E
+    L.Vship_SSIMU2Free.argtypes = [Handler]
+    for name in ("Vship_ComputeSSIMU2Float", "Vship_ComputeSSIMU2Uint16"):
+        fn = getattr(L, name)
+        fn.restype = E
+        fn.argtypes = [Handler, ctypes.POINTER(ctypes.c_double), P3, P3, ctypes.c_int64]
+    for name in ("Vship_ComputeSSIMU2Floatv2", "Vship_ComputeSSIMU2Uint16v2"):
+        fn = getattr(L, name)
+        fn.restype = E
+        fn.argtypes = [Handler, ctypes.POINTER(ctypes.c_double), P3, P3, ctypes.c_int64, ctypes.c_int64]
+    L.Vship_SSIMU2PairFootprint.restype = ctypes.c_uint64
+    L.Vship_SSIMU2PairFootprint.argtypes = [ctypes.c_int, ctypes.c_int, ctypes.c_int]
+    L.Vship_SSIMU2BatchInit.restype = E
+    L.Vship_SSIMU2BatchInit.argtypes = [ctypes.POINTER(Handler), ctypes.POINTER(BatchConfig)]
+    L.Vship_SSIMU2BatchFree.restype = E
+    L.Vship_SSIMU2BatchFree.argtypes = [Handler]
+    L.Vship_SSIMU2BatchSubmit.restype = E
+    L.Vship_SSIMU2BatchSubmit.argtypes = [Handler, ctypes.POINTER(Pair), ctypes.c_int]
+    L.Vship_SSIMU2BatchWait.restype = E
+    L.Vship_SSIMU2BatchWait.argtypes = [Handler, ctypes.c_void_p, ctypes.c_void_p]
+    L.Vship_SSIMU2BatchCompute.restype = E
+    L.Vship_SSIMU2BatchCompute.argtypes = [Handler, ctypes.POINTER(Pair), ctypes.c_int, ctypes.c_void_p,
+                                           ctypes.c_void_p]
+    L.Vship_SSIMU2BatchGetStats.restype = E
+    L.Vship_SSIMU2BatchGetStats.argtypes = [Handler, ctypes.POINTER(BatchStats)]
+    L.Vship_DeviceMalloc.restype = E
+    L.Vship_DeviceMalloc.argtypes = [ctypes.POINTER(ctypes.c_void_p), ctypes.c_uint64]
+    L.Vship_DeviceFree.restype = E
+    L.Vship_DeviceFree.argtypes = [ctypes.c_void_p]
+    L.Vship_MemcpyHtoD.restype = E
+    L.Vship_MemcpyHtoD.argtypes = [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_uint64]
+    L.Vship_MemcpyDtoH.restype = E
+    L.Vship_MemcpyDtoH.argtypes = [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_uint64]
+    _lib = L
+    return L
+
+
+def _check(code, where=""):
+    if code != 0:
+        raise VshipError(code, where)
+
+
+def error_message(code):
+    L = lib()
+    n = L.Vship_GetErrorMessage(int(code), None, 0)
+    buf = ctypes.create_string_buffer(n)
+    L.Vship_GetErrorMessage(int(code), buf, n)
+    return buf.value.decode()
+
+
+def version():
+    v = lib().Vship_GetVersion()
+    return (v.major, v.minor, v.minorMinor, v.backend)
+
+
+def device_count():
+    n = ctypes.c_int()
+    _check(lib().Vship_GetDeviceCount(ctypes.byref(n)), "Vship_GetDeviceCount")
+    return n.value
+
+
+def set_device(i):
+    _check(lib().Vship_SetDevice(int(i)), "Vship_SetDevice")
+
+
+def device_info(i=0):
+    d = DeviceInfo()
+    _check(lib().Vship_GetDeviceInfo(ctypes.byref(d), int(i)), "Vship_GetDeviceInfo")
+    return {"name": d.name.decode(errors="replace"), "VRAMSize": d.VRAMSize, "integrated": d.integrated,
+            "MultiProcessorCount": d.MultiProcessorCount, "WarpSize": d.WarpSize}
+
+
+def gpu_full_check(i=0):
+    _check(lib().Vship_GPUFullCheck(int(i)), "Vship_GPUFullCheck")
+
+
+def pair_footprint(w, h, fp16=False):
+    return int(lib().Vship_SSIMU2PairFootprint(int(w), int(h), int(bool(fp16))))
+
+
+class DeviceBuffer:
+    """A device allocation made through Vship_DeviceMalloc."""
+
+    def __init__(self, nbytes):
+        self.nbytes = int(nbytes)
+        p = ctypes.c_void_p()
+        _check(lib().Vship_DeviceMalloc(ctypes.byref(p), max(1, self.nbytes)), "Vship_DeviceMalloc")
+        self.ptr = p.value
+
+    def upload(self, arr, offset=0):
+        a = np.ascontiguousarray(arr)
+        if offset + a.nbytes > self.nbytes:
+            raise ValueError("upload past the end of the buffer")
+        _check(lib().Vship_MemcpyHtoD(self.ptr + offset, a.ctypes.data, a.nbytes), "Vship_MemcpyHtoD")
+
+    def download(self, nbytes=None, offset=0):
+        n = self.nbytes - offset if nbytes is None else nbytes
+        out = np.empty(n, np.uint8)
+        _check(lib().Vship_MemcpyDtoH(out.ctypes.data, self.ptr + offset, n), "Vship_MemcpyDtoH")
+        return out
+
+    def free(self):
+        if self.ptr:
+            lib().Vship_DeviceFree(self.ptr)
+            self.ptr = None
+
+    def __del__(self):
+        try:
+            self.free()
+        except Exception:
+            pass
+
+
+class PinnedBuffer:
+    """Page-locked host memory through Vship_PinnedMalloc, viewed as numpy."""
+
+    def __init__(self, nbytes):
+        p = ctypes.c_void_p()
+        _check(lib().Vship_PinnedMalloc(ctypes.byref(p), int(nbytes)), "Vship_PinnedMalloc")
+        self.ptr = p.value
+        self.nbytes = int(nbytes)
+        self.array = np.ctypeslib.as_array((ctypes.c_uint8 * self.nbytes).from_address(self.ptr))
+
+    def free(self):
+        if self.ptr:
+            self.array = None
+            _check(lib().Vship_PinnedFree(self.ptr), "Vship_PinnedFree")
+            self.ptr = None
+
+    def __del__(self):
+        try:
+            self.free()
+        except Exception:
+            pass
+
+
+def _plane_ptrs(planes):
+    return P3(*[p.ctypes.data for p in planes])
+
+
+class SSIMU2Handler:
+    """Per-frame SSIMULACRA2 handle of fixed size: the reference's
+    Vship_SSIMU2Init / Vship_ComputeSSIMU2{Float,Uint16}[v2] / Vship_SSIMU2Free
+    (src/VshipAPI.h:87-111, src/VshipLib.cpp:162-250)."""
+
+    def __init__(self, width, height):
+        self.width, self.height = int(width), int(height)
+        self._h = Handler()
+        code = lib().Vship_SSIMU2Init(ctypes.byref(self._h), self.width, self.height)
+        if code != 0:
+            lib().Vship_SSIMU2Free(self._h)
+            raise VshipError(code, "Vship_SSIMU2Init")
+
+    @property
+    def id(self):
+        return self._h.id
+
+    def _compute(self, fn, ref, dist, stride, stride2):
+        score = ctypes.c_double()
+        _check(fn(self._h, ctypes.byref(score), _plane_ptrs(ref), _plane_ptrs(dist), stride, stride2), fn.__name__)
+        return score.value
+
+    def compute_float(self, ref, dist):
+        """ref/dist: three 2-D float32 planes (R, G, B), gamma-encoded in [0, 1]."""
+        ref = [np.asarray(p, np.float32) for p in ref]
+        dist = [np.asarray(p, np.float32) for p in dist]
+        return self._compute(lib().Vship_ComputeSSIMU2Floatv2, ref, dist, ref[0].strides[0], dist[0].strides[0])
+
+    def compute_uint16(self, ref, dist):
+        """ref/dist: three 2-D uint16 planes, full range."""
+        ref = [np.asarray(p, np.uint16) for p in ref]
+        dist = [np.asarray(p, np.uint16) for p in dist]
+        return self._compute(lib().Vship_ComputeSSIMU2Uint16v2, ref, dist, ref[0].strides[0], dist[0].strides[0])
+
+    def close(self):
+        if self._h is not None:
+            _check(lib().Vship_SSIMU2Free(self._h), "Vship_SSIMU2Free")
+            self._h = None
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
+
+class DevicePairs:
+    """Host frame pairs copied into one device allocation, with the
+    Vship_SSIMU2Pair descriptors that point into it."""
+
+    def __init__(self, pairs, fmt, bits=8, matrix=MATRIX_BT709):
+        """pairs: list of (ref_planes, dist_planes), each three 2-D numpy arrays."""
+        offs, total = [], 0
+        for ref, dist in pairs:
+            o = []
+            for p in list(ref) + list(dist):
+                o.append(total)
+                total += (np.ascontiguousarray(p).nbytes + 255) // 256 * 256
+            offs.append(o)
+        self.buffer = DeviceBuffer(total)
+        self.descs = (Pair * len(pairs))()
+        for i, (ref, dist) in enumerate(pairs):
+            planes = [np.ascontiguousarray(p) for p in list(ref) + list(dist)]
+            for j, p in enumerate(planes):
+                self.buffer.upload(p, offs[i][j])
+            d = self.descs[i]
+            for j in range(3):
+                d.ref[j] = self.buffer.ptr + offs[i][j]
+                d.dist[j] = self.buffer.ptr + offs[i][3 + j]
+                d.ref_stride[j] = planes[j].strides[0]
+                d.dist_stride[j] = planes[3 + j].strides[0]
+            d.height, d.width = planes[0].shape
+            d.format = fmt
+            d.bits = bits
+            d.matrix = matrix
+        self.n = len(pairs)
+        self.nbytes = total
+
+    def free(self):
+        self.buffer.free()
+
+
+class SSIMU2Batch:
+    """Batched SSIMULACRA2 over device-resident pairs (Vship_SSIMU2Batch*)."""
+
+    def __init__(self, max_pairs, max_width, max_height, fp16=False, arena_bytes=0, map_tile_height=0,
+                 persistent_grid=0, use_graph=False, profile=False):
+        cfg = BatchConfig(max_pairs=max_pairs, max_width=max_width, max_height=max_height,
+                          fp16_planes=int(bool(fp16)), arena_bytes=int(arena_bytes),
+                          map_tile_height=int(map_tile_height), persistent_grid=int(persistent_grid),
+                          use_graph=int(bool(use_graph)), profile=int(bool(profile)))
+        self.max_pairs = max_pairs
+        self._h = Handler()
+        _check(lib().Vship_SSIMU2BatchInit(ctypes.byref(self._h), ctypes.byref(cfg)), "Vship_SSIMU2BatchInit")
+
+    def submit(self, descs, n=None):
+        n = len(descs) if n is None else n
+        _check(lib().Vship_SSIMU2BatchSubmit(self._h, descs, n), "Vship_SSIMU2BatchSubmit")
+        self._n = n
+
+    def wait(self, features=False):
+        scores = np.zeros(self._n)
+        feats = np.zeros((self._n, 108)) if features else None
+        _check(lib().Vship_SSIMU2BatchWait(self._h, scores.ctypes.data, feats.ctypes.data if features else None),
+               "Vship_SSIMU2BatchWait")
+        return (scores, feats) if features else scores
+
+    def compute(self, descs, n=None, features=False):
+        self.submit(descs, n)
+        return self.wait(features)
+
+    def stats(self):
+        s = BatchStats()
+        _check(lib().Vship_SSIMU2BatchGetStats(self._h, ctypes.byref(s)), "Vship_SSIMU2BatchGetStats")
+        return {k: getattr(s, k) for k, _ in BatchStats._fields_}
+
+    def close(self):
+        if self._h is not None:
+            _check(lib().Vship_SSIMU2BatchFree(self._h), "Vship_SSIMU2BatchFree")
+            self._h = None
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()

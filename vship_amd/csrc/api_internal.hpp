@@ -1,0 +1,33 @@
+// Internals shared by the C ABI (vship_api.hip) and the VapourSynth filter.
+#pragma once
+#include <atomic>
+#include <cstdint>
+#include <memory>
+
+#include "engine.hpp"
+
+namespace vship {
+
+const char* error_message(int code);
+int device_count_or_throw();
+bool gpu_kernel_check();
+void gpu_full_check(int gpu_id);
+int default_map_th(int w, int h);
+
+// One per-frame engine of fixed size, as a Vship_SSIMU2Handler refers to
+// (src/ssimu2/main.hpp:153-191 of the reference): host RGB planes are copied
+// row-packed into a device staging area and scored as a batch of one.
+struct FrameHandle {
+    int width, height, device;
+    std::unique_ptr<Engine> engine;
+    uint8_t* staging = nullptr;
+    size_t staging_bytes = 0;
+    std::atomic<bool> busy{false};
+
+    FrameHandle(int w, int h, int device);
+    ~FrameHandle();
+    // fmt: IN_RGB_U8 / IN_RGB_U16 / IN_RGB_F16 / IN_RGB_F32; host planes
+    double compute(int fmt, const uint8_t* const p1[3], const uint8_t* const p2[3], int64_t s1, int64_t s2);
+};
+
+}  // namespace vship

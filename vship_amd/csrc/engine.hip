@@ -1,0 +1,375 @@
+// Batch engine (host side). See engine.hpp.
+#include "engine.hpp"
+
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+
+#include "ssimu2_kernels.hpp"
+
+namespace vship {
+
+namespace {
+
+int64_t round_up(int64_t v, int64_t a) { return (v + a - 1) / a * a; }
+
+int input_bytes_per_sample(int fmt) {
+    switch (fmt) {
+    case IN_RGB_U8:
+    case IN_YUV420_U8: return 1;
+    case IN_RGB_F32: return 4;
+    default: return 2;
+    }
+}
+
+// Normalised 17-tap Gaussian, sigma 1.5, radius 8 (src/ssimu2/gaussianblur.hpp:7-18,
+// src/util/preprocessor.hpp:89-92). Normalising to sum 1 does not change the
+// renormalised blur and makes interior positions need no division.
+Taps make_taps() {
+    double g[17], sum = 0.0;
+    for (int i = 0; i < 17; i++) {
+        const double k = i - 8;
+        g[i] = std::exp(-k * k / (2.0 * 1.5 * 1.5));
+        sum += g[i];
+    }
+    Taps t;
+    for (int i = 0; i < 17; i++) t.g[i] = (float)(g[i] / sum);
+    return t;
+}
+
+void matrix_coefs(int matrix, float& kr, float& kb) {
+    switch (matrix) {
+    case 1: kr = 0.299f; kb = 0.114f; break;       // BT.601
+    case 2: kr = 0.2627f; kb = 0.0593f; break;     // BT.2020 non-constant luminance
+    default: kr = 0.2126f; kb = 0.0722f; break;    // BT.709
+    }
+}
+
+// Every plane must lie inside one device allocation with rows that hold the
+// image: a bad pointer or stride becomes an error here instead of a GPU fault.
+void validate_planes(const PairInput& p) {
+    const int bps = input_bytes_per_sample(p.fmt);
+    const bool yuv = p.fmt >= IN_YUV420_U8;
+    for (int im = 0; im < 2; im++)
+        for (int pl = 0; pl < 3; pl++) {
+            const int pw = (yuv && pl > 0) ? (p.w + 1) / 2 : p.w;
+            const int ph = (yuv && pl > 0) ? (p.h + 1) / 2 : p.h;
+            const uint8_t* ptr = p.src[im][pl];
+            const int64_t row = (int64_t)pw * bps;
+            if (!ptr || p.stride[im][pl] < row) VSHIP_THROW(E_BadPointer);
+            hipPointerAttribute_t attr;
+            if (hipPointerGetAttributes(&attr, ptr) != hipSuccess) VSHIP_THROW(E_BadPointer);
+            if (attr.type != hipMemoryTypeDevice && attr.type != hipMemoryTypeManaged) VSHIP_THROW(E_BadPointer);
+            hipDeviceptr_t base = nullptr;
+            size_t size = 0;
+            if (hipMemGetAddressRange(&base, &size, (hipDeviceptr_t)ptr) != hipSuccess) VSHIP_THROW(E_BadPointer);
+            const uint8_t* b = reinterpret_cast<const uint8_t*>(base);
+            const uint8_t* end = ptr + (int64_t)(ph - 1) * p.stride[im][pl] + row;
+            if (ptr < b || end > b + size) VSHIP_THROW(E_BadPointer);
+        }
+}
+
+}  // namespace
+
+PairLayout pair_layout(int w, int h) {
+    PairLayout L;
+    int64_t off = 0;
+    int ww = w, hh = h;
+    for (int s = 0; s < kScales; s++) {
+        L.sw[s] = ww;
+        L.sh[s] = hh;
+        L.pitch[s] = (int)round_up(ww, 64);
+        L.plane_elems[s] = (int64_t)L.pitch[s] * hh;
+        L.scale_off[s] = off;
+        off += 3 * L.plane_elems[s];
+        ww = (ww - 1) / 2 + 1;
+        hh = (hh - 1) / 2 + 1;
+    }
+    L.img_elems = off;
+    return L;
+}
+
+int64_t pair_elems(int w, int h) { return 2 * pair_layout(w, h).img_elems; }
+
+int prep_tiles(int w, int h) { return 2 * ((w + kPrepTile - 1) / kPrepTile) * ((h + kPrepTile - 1) / kPrepTile); }
+
+int map_tiles(int w, int h, int th) {
+    int n = 0, ww = w, hh = h;
+    for (int s = 0; s < kScales; s++) {
+        n += ((ww + kMapTW - 1) / kMapTW) * ((hh + th - 1) / th);
+        ww = (ww - 1) / 2 + 1;
+        hh = (hh - 1) / 2 + 1;
+    }
+    return n;
+}
+
+uint64_t map_algorithmic_bytes(int w, int h, bool fp16) {
+    uint64_t t = 0;
+    int ww = w, hh = h;
+    for (int s = 0; s < kScales; s++) {
+        t += (uint64_t)ww * hh;
+        ww = (ww - 1) / 2 + 1;
+        hh = (hh - 1) / 2 + 1;
+    }
+    return t * 2 /*images*/ * 3 /*planes*/ * (fp16 ? 2 : 4);
+}
+
+uint64_t prep_algorithmic_bytes(const PairInput& p, bool fp16) {
+    const uint64_t bps = input_bytes_per_sample(p.fmt);
+    uint64_t in;
+    if (p.fmt >= IN_YUV420_U8)
+        in = bps * ((uint64_t)p.w * p.h + 2ull * ((p.w + 1) / 2) * ((p.h + 1) / 2));
+    else
+        in = bps * 3ull * p.w * p.h;
+    return 2 * in + map_algorithmic_bytes(p.w, p.h, fp16);  // read inputs, write every XYB plane once
+}
+
+Engine::Engine(const EngineOptions& opt) : opt_(opt) {
+    if (opt_.max_pairs < 1) VSHIP_THROW(E_BadPointer);
+    if (opt_.map_th != 32 && opt_.map_th != 64) opt_.map_th = 64;
+    HIP_OR_THROW(hipSetDevice(opt_.device), E_BadDeviceArgument);
+    HIP_OR_THROW(hipDeviceGetAttribute(&num_cus_, hipDeviceAttributeMultiprocessorCount, opt_.device),
+                 E_BadDeviceCode);
+    taps_ = make_taps();
+    try {
+        HIP_OR_THROW(hipStreamCreateWithFlags(&stream_, hipStreamNonBlocking), E_OutOfVRAM);
+        const size_t nseg = (size_t)kScales * opt_.max_pairs;
+        off_pairs_ = round_up(sizeof(BatchHdr), 64);
+        off_prep_ = off_pairs_ + round_up(sizeof(PairDev) * opt_.max_pairs, 64);
+        off_segstart_ = off_prep_ + round_up(sizeof(int32_t) * (opt_.max_pairs + 1), 64);
+        off_segcount_ = off_segstart_ + round_up(sizeof(int32_t) * nseg, 64);
+        off_order_ = off_segcount_ + round_up(sizeof(int32_t) * nseg, 64);
+        off_ostart_ = off_order_ + round_up(sizeof(int32_t) * nseg, 64);
+        desc_bytes_ = off_ostart_ + round_up(sizeof(int32_t) * (nseg + 1), 64);
+        HIP_OR_THROW(hipHostMalloc(reinterpret_cast<void**>(&desc_h_), desc_bytes_, hipHostMallocDefault),
+                     E_OutOfRAM);
+        std::memset(desc_h_, 0, desc_bytes_);
+        HIP_OR_THROW(hipMalloc(reinterpret_cast<void**>(&desc_d_), desc_bytes_), E_OutOfVRAM);
+        const int es = opt_.fp16 ? 2 : 4;
+        arena_elems_ = opt_.arena_bytes / es;
+        if (arena_elems_ <= 0) VSHIP_THROW(E_OutOfVRAM);
+        HIP_OR_THROW(hipMalloc(&arena_d_, (size_t)arena_elems_ * es), E_OutOfVRAM);
+        max_map_tiles_ = opt_.max_map_tiles;
+        if (max_map_tiles_ <= 0) VSHIP_THROW(E_OutOfVRAM);
+        HIP_OR_THROW(hipMalloc(reinterpret_cast<void**>(&partials_d_), sizeof(double) * kFeat * max_map_tiles_),
+                     E_OutOfVRAM);
+        const size_t out_bytes = sizeof(double) * 109 * opt_.max_pairs;
+        HIP_OR_THROW(hipMalloc(reinterpret_cast<void**>(&out_d_), out_bytes), E_OutOfVRAM);
+        HIP_OR_THROW(hipHostMalloc(reinterpret_cast<void**>(&out_h_), out_bytes, hipHostMallocDefault), E_OutOfRAM);
+        for (auto& e : ev_) HIP_OR_THROW(hipEventCreate(&e), E_OutOfRAM);
+    } catch (...) {
+        release();
+        throw;
+    }
+}
+
+Engine::~Engine() { release(); }
+
+void Engine::release() {
+    if (stream_) (void)hipStreamSynchronize(stream_);
+    if (graph_exec_) (void)hipGraphExecDestroy(graph_exec_);
+    if (graph_) (void)hipGraphDestroy(graph_);
+    graph_exec_ = nullptr;
+    graph_ = nullptr;
+    for (auto& e : ev_)
+        if (e) {
+            (void)hipEventDestroy(e);
+            e = nullptr;
+        }
+    if (out_h_) (void)hipHostFree(out_h_);
+    if (out_d_) (void)hipFree(out_d_);
+    if (partials_d_) (void)hipFree(partials_d_);
+    if (arena_d_) (void)hipFree(arena_d_);
+    if (desc_d_) (void)hipFree(desc_d_);
+    if (desc_h_) (void)hipHostFree(desc_h_);
+    if (stream_) (void)hipStreamDestroy(stream_);
+    out_h_ = nullptr;
+    out_d_ = nullptr;
+    partials_d_ = nullptr;
+    arena_d_ = nullptr;
+    desc_d_ = nullptr;
+    desc_h_ = nullptr;
+    stream_ = nullptr;
+}
+
+void Engine::plan(const PairInput* in, int n) {
+    if (n < 1 || n > opt_.max_pairs || !in) VSHIP_THROW(E_BadPointer);
+    BatchHdr* hdr = reinterpret_cast<BatchHdr*>(desc_h_);
+    PairDev* pd = reinterpret_cast<PairDev*>(desc_h_ + off_pairs_);
+    int32_t* prep_start = reinterpret_cast<int32_t*>(desc_h_ + off_prep_);
+    int32_t* seg_start = reinterpret_cast<int32_t*>(desc_h_ + off_segstart_);
+    int32_t* seg_count = reinterpret_cast<int32_t*>(desc_h_ + off_segcount_);
+    int32_t* order_seg = reinterpret_cast<int32_t*>(desc_h_ + off_order_);
+    int32_t* order_start = reinterpret_cast<int32_t*>(desc_h_ + off_ostart_);
+    int64_t arena_off = 0;
+    int64_t ptiles = 0;
+    uint64_t mbytes = 0, pbytes = 0;
+    for (int i = 0; i < n; i++) {
+        const PairInput& p = in[i];
+        if (p.w < 1 || p.h < 1 || p.fmt < IN_RGB_U8 || p.fmt > IN_YUV420_U16) VSHIP_THROW(E_BadPointer);
+        if (p.fmt == IN_YUV420_U16 && (p.bits < 9 || p.bits > 16)) VSHIP_THROW(E_BadPointer);
+        validate_planes(p);
+        const PairLayout L = pair_layout(p.w, p.h);
+        PairDev& d = pd[i];
+        std::memset(&d, 0, sizeof(d));
+        for (int im = 0; im < 2; im++)
+            for (int pl = 0; pl < 3; pl++) {
+                d.src[im][pl] = p.src[im][pl];
+                d.stride[im][pl] = p.stride[im][pl];
+            }
+        d.fmt = p.fmt;
+        d.bits = p.fmt == IN_YUV420_U8 ? 8 : (p.fmt == IN_YUV420_U16 ? p.bits : 0);
+        matrix_coefs(p.matrix, d.kr, d.kb);
+        d.w = p.w;
+        d.h = p.h;
+        d.arena_off = arena_off;
+        d.img_elems = L.img_elems;
+        for (int s = 0; s < kScales; s++) {
+            d.scale_off[s] = L.scale_off[s];
+            d.plane_elems[s] = L.plane_elems[s];
+            d.pitch[s] = L.pitch[s];
+            d.sw[s] = L.sw[s];
+            d.sh[s] = L.sh[s];
+        }
+        arena_off += 2 * L.img_elems;
+        if (arena_off > arena_elems_) VSHIP_THROW(E_OutOfVRAM);
+        prep_start[i] = (int32_t)ptiles;
+        ptiles += prep_tiles(p.w, p.h);
+        mbytes += map_algorithmic_bytes(p.w, p.h, opt_.fp16);
+        pbytes += prep_algorithmic_bytes(p, opt_.fp16);
+    }
+    // map segments: class 0 (FP32 moments) numbered first, then class 1 (FP64)
+    int64_t mtiles = 0;
+    int k = 0;
+    cls_bytes_[0] = cls_bytes_[1] = 0;
+    int nseg_cls[2] = {0, 0};
+    int64_t tiles_cls[2] = {0, 0};
+    for (int cls = 0; cls < 2; cls++) {
+        for (int i = 0; i < n; i++)
+            for (int s = 0; s < kScales; s++) {
+                const int sw = pd[i].sw[s], sh = pd[i].sh[s];
+                const int c = (int64_t)sw * sh < kSmallSegment ? 1 : 0;
+                if (c != cls) continue;
+                const int th = c ? kSmallTH : opt_.map_th;
+                const int64_t cnt = (int64_t)((sw + kMapTW - 1) / kMapTW) * ((sh + th - 1) / th);
+                const int seg = i * kScales + s;
+                cls_bytes_[cls] += (uint64_t)sw * sh * 2 * 3 * (opt_.fp16 ? 2 : 4);
+                seg_start[seg] = (int32_t)mtiles;
+                seg_count[seg] = (int32_t)cnt;
+                order_seg[k] = seg;
+                order_start[k] = (int32_t)mtiles;
+                k++;
+                mtiles += cnt;
+                nseg_cls[cls]++;
+                tiles_cls[cls] += cnt;
+            }
+    }
+    order_start[k] = (int32_t)mtiles;
+    if (mtiles > max_map_tiles_ || mtiles > INT32_MAX || ptiles > INT32_MAX) VSHIP_THROW(E_OutOfVRAM);
+    prep_start[n] = (int32_t)ptiles;
+    hdr->npairs = n;
+    hdr->prep_tiles = (int32_t)ptiles;
+    hdr->map_tiles = (int32_t)mtiles;
+    hdr->map_th = opt_.map_th;
+    for (int c = 0; c < 2; c++) {
+        hdr->nseg[c] = nseg_cls[c];
+        hdr->tiles[c] = (int32_t)tiles_cls[c];
+        ntiles_cls_[c] = (int)tiles_cls[c];
+    }
+    npairs_ = n;
+    ntiles_prep_ = (int)ptiles;
+    ntiles_map_ = (int)mtiles;
+    map_bytes_ = mbytes;
+    prep_bytes_ = pbytes;
+}
+
+void Engine::enqueue_body() {
+    const bool graph = opt_.use_graph;
+    const BatchHdr* hdr = reinterpret_cast<const BatchHdr*>(desc_d_);
+    const PairDev* pd = reinterpret_cast<const PairDev*>(desc_d_ + off_pairs_);
+    const int32_t* prep_start = reinterpret_cast<const int32_t*>(desc_d_ + off_prep_);
+    const int32_t* seg_start = reinterpret_cast<const int32_t*>(desc_d_ + off_segstart_);
+    const int32_t* seg_count = reinterpret_cast<const int32_t*>(desc_d_ + off_segcount_);
+    const int32_t* order_seg = reinterpret_cast<const int32_t*>(desc_d_ + off_order_);
+    const int32_t* order_start = reinterpret_cast<const int32_t*>(desc_d_ + off_ostart_);
+    const bool prof = opt_.profile && !graph;
+    const int fixed = opt_.persistent_grid > 0 ? opt_.persistent_grid : 4 * num_cus_;
+    auto grid_for = [&](int work, int cap_in_graph) {
+        if (graph) return cap_in_graph;
+        if (opt_.persistent_grid > 0) return std::max(1, std::min(work, fixed));
+        return std::max(1, work);
+    };
+    const int g_prep = grid_for(ntiles_prep_, fixed);
+    const int g_map0 = grid_for(ntiles_cls_[0], fixed);
+    const int g_map1 = grid_for(ntiles_cls_[1], std::max(1, fixed / 4));
+    const int g_fin = grid_for(npairs_, std::min(opt_.max_pairs, fixed));
+    const size_t copy_bytes = graph ? desc_bytes_ : desc_bytes_;
+    if (prof) HIP_OR_THROW(hipEventRecord(ev_[0], stream_), E_BadDeviceCode);
+    HIP_OR_THROW(hipMemcpyAsync(desc_d_, desc_h_, copy_bytes, hipMemcpyHostToDevice, stream_), E_BadDeviceCode);
+    if (prof) HIP_OR_THROW(hipEventRecord(ev_[1], stream_), E_BadDeviceCode);
+    HIP_OR_THROW(launch_prep(opt_.fp16, g_prep, pd, prep_start, hdr, arena_d_, stream_), E_BadDeviceCode);
+    if (prof) HIP_OR_THROW(hipEventRecord(ev_[2], stream_), E_BadDeviceCode);
+    HIP_OR_THROW(launch_map(opt_.fp16, 0, opt_.map_th, g_map0, pd, order_seg, order_start, hdr, arena_d_,
+                            partials_d_, taps_, stream_),
+                 E_BadDeviceCode);
+    if (prof) HIP_OR_THROW(hipEventRecord(ev_[3], stream_), E_BadDeviceCode);
+    HIP_OR_THROW(launch_map(opt_.fp16, 1, kSmallTH, g_map1, pd, order_seg, order_start, hdr, arena_d_,
+                            partials_d_, taps_, stream_),
+                 E_BadDeviceCode);
+    if (prof) HIP_OR_THROW(hipEventRecord(ev_[4], stream_), E_BadDeviceCode);
+    double* scores_d = out_d_;
+    double* feats_d = out_d_ + opt_.max_pairs;
+    HIP_OR_THROW(launch_finalize(g_fin, pd, seg_start, seg_count, hdr, partials_d_, scores_d, feats_d, stream_),
+                 E_BadDeviceCode);
+    if (prof) HIP_OR_THROW(hipEventRecord(ev_[5], stream_), E_BadDeviceCode);
+    const int np = graph ? opt_.max_pairs : npairs_;
+    HIP_OR_THROW(hipMemcpyAsync(out_h_, scores_d, sizeof(double) * np, hipMemcpyDeviceToHost, stream_),
+                 E_BadDeviceCode);
+    HIP_OR_THROW(hipMemcpyAsync(out_h_ + opt_.max_pairs, feats_d, sizeof(double) * 108 * np,
+                                hipMemcpyDeviceToHost, stream_),
+                 E_BadDeviceCode);
+    if (prof) HIP_OR_THROW(hipEventRecord(ev_[6], stream_), E_BadDeviceCode);
+}
+
+void Engine::submit(const PairInput* pairs, int n) {
+    if (pending_) VSHIP_THROW(E_BadHandler);
+    HIP_OR_THROW(hipSetDevice(opt_.device), E_BadDeviceArgument);
+    plan(pairs, n);
+    if (opt_.use_graph) {
+        if (!graph_exec_) {
+            HIP_OR_THROW(hipStreamBeginCapture(stream_, hipStreamCaptureModeThreadLocal), E_BadDeviceCode);
+            try {
+                enqueue_body();
+            } catch (...) {
+                hipGraph_t g = nullptr;
+                (void)hipStreamEndCapture(stream_, &g);
+                if (g) (void)hipGraphDestroy(g);
+                throw;
+            }
+            HIP_OR_THROW(hipStreamEndCapture(stream_, &graph_), E_BadDeviceCode);
+            HIP_OR_THROW(hipGraphInstantiate(&graph_exec_, graph_, nullptr, nullptr, 0), E_BadDeviceCode);
+        }
+        HIP_OR_THROW(hipGraphLaunch(graph_exec_, stream_), E_BadDeviceCode);
+    } else {
+        enqueue_body();
+    }
+    pending_ = true;
+}
+
+void Engine::wait(double* scores, double* features) {
+    if (!pending_) VSHIP_THROW(E_BadHandler);
+    pending_ = false;
+    HIP_OR_THROW(hipStreamSynchronize(stream_), E_BadDeviceCode);
+    if (scores) std::memcpy(scores, out_h_, sizeof(double) * npairs_);
+    if (features) std::memcpy(features, out_h_ + opt_.max_pairs, sizeof(double) * 108 * npairs_);
+    if (opt_.profile && !opt_.use_graph) {
+        (void)hipEventElapsedTime(&timings_.prep_ms, ev_[1], ev_[2]);
+        (void)hipEventElapsedTime(&timings_.map32_ms, ev_[2], ev_[3]);
+        (void)hipEventElapsedTime(&timings_.map64_ms, ev_[3], ev_[4]);
+        (void)hipEventElapsedTime(&timings_.map_ms, ev_[2], ev_[4]);
+        (void)hipEventElapsedTime(&timings_.finalize_ms, ev_[4], ev_[5]);
+        (void)hipEventElapsedTime(&timings_.total_ms, ev_[0], ev_[6]);
+    }
+}
+
+}  // namespace vship

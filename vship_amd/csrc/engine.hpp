@@ -1,0 +1,144 @@
+// Batch engine: plans a batch of frame pairs into device descriptors, owns the
+// XYB arena, the partial-sum buffer and one HIP stream, and launches the three
+// kernels (optionally through one captured hipGraph).
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <stdexcept>
+#include <string>
+#include <vector>
+
+#include "ssimu2_dev.hpp"
+
+namespace vship {
+
+// Error codes with the numeric values of the public header
+// (src/VshipAPI.h:35-58 of the reference).
+enum ErrCode : int {
+    E_NoError = 0,
+    E_OutOfVRAM = 1,
+    E_OutOfRAM = 2,
+    E_DifferingInputType = 3,
+    E_NonRGBSInput = 4,
+    E_DeviceCountError = 5,
+    E_NoDeviceDetected = 6,
+    E_BadDeviceArgument = 7,
+    E_BadDeviceCode = 8,
+    E_BadHandler = 9,
+    E_BadPointer = 10,
+    E_BadErrorType = 11,
+};
+
+struct VshipErr : std::exception {
+    ErrCode code;
+    std::string where;
+    VshipErr(ErrCode c, const char* file, int line) : code(c), where(std::string(file) + ":" + std::to_string(line)) {}
+    const char* what() const noexcept override { return where.c_str(); }
+};
+#define VSHIP_THROW(code) throw ::vship::VshipErr((code), __FILE__, __LINE__)
+
+// Any HIP failure inside the engine surfaces as this code (the reference maps
+// failed device allocations to OutOfVRAM, src/ssimu2/main.hpp:116-119).
+#define HIP_OR_THROW(expr, code)                 \
+    do {                                         \
+        hipError_t e_ = (expr);                  \
+        if (e_ != hipSuccess) VSHIP_THROW(code); \
+    } while (0)
+
+struct PairInput {
+    const uint8_t* src[2][3];  // device pointers [image][plane]
+    int64_t stride[2][3];
+    int w, h;
+    int fmt;     // InFmt
+    int bits;
+    int matrix;  // 0 BT.709, 1 BT.601, 2 BT.2020 NCL
+};
+
+struct EngineOptions {
+    int device = 0;
+    int max_pairs = 1;
+    int64_t arena_bytes = 0;   // XYB arena capacity
+    int64_t max_map_tiles = 0;  // 0: derived from the arena for the smallest tile count per element
+    bool fp16 = false;
+    int map_th = 64;
+    int persistent_grid = 0;   // 0: grid = tile count
+    bool use_graph = false;
+    bool profile = false;
+};
+
+struct Timings {
+    float prep_ms = 0, map_ms = 0, map32_ms = 0, map64_ms = 0, finalize_ms = 0, total_ms = 0;
+};
+
+// Geometry of one pair in the arena.
+struct PairLayout {
+    int sw[kScales], sh[kScales], pitch[kScales];
+    int64_t scale_off[kScales], plane_elems[kScales];
+    int64_t img_elems;
+};
+
+PairLayout pair_layout(int w, int h);
+int64_t pair_elems(int w, int h);  // both images
+int prep_tiles(int w, int h);
+int map_tiles(int w, int h, int th);
+uint64_t map_algorithmic_bytes(int w, int h, bool fp16);
+uint64_t prep_algorithmic_bytes(const PairInput& p, bool fp16);
+
+class Engine {
+public:
+    explicit Engine(const EngineOptions& opt);
+    ~Engine();
+    Engine(const Engine&) = delete;
+    Engine& operator=(const Engine&) = delete;
+
+    // Plan and enqueue one batch (asynchronous).
+    void submit(const PairInput* pairs, int n);
+    // Wait for the last submitted batch and copy results out.
+    void wait(double* scores, double* features);
+
+    const EngineOptions& options() const { return opt_; }
+    hipStream_t stream() const { return stream_; }
+    const Timings& timings() const { return timings_; }
+    int last_npairs() const { return npairs_; }
+    int last_prep_tiles() const { return ntiles_prep_; }
+    int last_map_tiles() const { return ntiles_map_; }
+    int last_map_tiles_class(int c) const { return ntiles_cls_[c]; }
+    uint64_t last_map_bytes() const { return map_bytes_; }
+    uint64_t last_map_bytes_class(int c) const { return cls_bytes_[c]; }
+    uint64_t last_prep_bytes() const { return prep_bytes_; }
+    int num_cus() const { return num_cus_; }
+
+private:
+    void plan(const PairInput* pairs, int n);
+    void enqueue_body();
+    void release();
+
+    EngineOptions opt_;
+    hipStream_t stream_ = nullptr;
+    int num_cus_ = 0;
+    Taps taps_{};
+
+    // descriptor block: [BatchHdr][PairDev x max_pairs][prep_start x (max_pairs+1)]
+    //   [seg_start x 6 max_pairs][seg_count x 6 max_pairs][order_seg x 6 max_pairs][order_start x (6 max_pairs+1)]
+    size_t desc_bytes_ = 0;
+    size_t off_pairs_ = 0, off_prep_ = 0, off_segstart_ = 0, off_segcount_ = 0, off_order_ = 0, off_ostart_ = 0;
+    uint8_t* desc_h_ = nullptr;  // pinned
+    uint8_t* desc_d_ = nullptr;
+    void* arena_d_ = nullptr;
+    int64_t arena_elems_ = 0;
+    double* partials_d_ = nullptr;
+    int64_t max_map_tiles_ = 0;
+    double* out_d_ = nullptr;   // scores [max_pairs] then features [max_pairs*108]
+    double* out_h_ = nullptr;   // pinned
+    hipEvent_t ev_[7] = {};
+    hipGraph_t graph_ = nullptr;
+    hipGraphExec_t graph_exec_ = nullptr;
+
+    int npairs_ = 0, ntiles_prep_ = 0, ntiles_map_ = 0, ntiles_cls_[2] = {0, 0};
+    uint64_t map_bytes_ = 0, prep_bytes_ = 0, cls_bytes_[2] = {0, 0};
+    bool pending_ = false;
+    Timings timings_;
+};
+
+}  // namespace vship

@@ -1,0 +1,686 @@
+// SSIMULACRA2 kernels for gfx950 (CDNA4).
+//
+//   prep_kernel      fused input unpack (planar RGB u8/u16/f16/f32 or YUV 4:2:0)
+//                    -> 2.4-power linearisation -> six-level 2x2 pyramid kept in
+//                    LDS -> positive XYB written once per scale.
+//                    Reference stages: src/util/float3operations.hpp:197-218,
+//                    src/ssimu2/makeXYB.hpp:4-100, src/ssimu2/downsample.hpp:3-31,
+//                    src/ssimu2/main.hpp:15-70.
+//   map_kernel       per 32xTH tile of one scale: renormalised 17-tap Gaussian
+//                    (row pass from global/L1 into LDS, column pass from LDS with
+//                    register blocking) of 4 moment planes per channel, SSIM,
+//                    artifact and detail-loss maps and their 1-/4-norm partial
+//                    sums. Reference: src/ssimu2/gaussianblur.hpp:24-92,
+//                    src/ssimu2/score.hpp:95-196.
+//   finalize_kernel  per pair: FP64 sums of the tile partials in a fixed order,
+//                    norms, 108-feature weighting and the output polynomial.
+//                    Reference: src/ssimu2/score.hpp:198-401, src/ssimu2/main.hpp:82-98.
+//
+// All three are persistent: a fixed grid walks the batch's tile list, which
+// lives in device memory, so one captured graph serves every batch shape.
+#include <hip/hip_fp16.h>
+#include <hip/hip_runtime.h>
+
+#include "ssimu2_dev.hpp"
+#include "ssimu2_kernels.hpp"
+
+namespace vship {
+
+// ---------------------------------------------------------------------------
+// small device helpers
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ float ldf(const float* p) { return *p; }
+__device__ __forceinline__ float ldf(const __half* p) { return __half2float(*p); }
+__device__ __forceinline__ void stf(float* p, float v) { *p = v; }
+__device__ __forceinline__ void stf(__half* p, float v) { *p = __float2half_rn(v); }
+
+// sign(v) |v|^2.4 as 2^(2.4 log2|v|) on the hardware exp2/log2 units
+// (src/ssimu2/makeXYB.hpp:52-64).
+__device__ __forceinline__ float linearize(float v) {
+    const float a = fabsf(v);
+    const float r = a > 0.f ? __builtin_amdgcn_exp2f(2.4f * __builtin_amdgcn_logf(a)) : 0.f;
+    return copysignf(r, v);
+}
+
+// cube root of x >= 0: 2^(log2(x)/3) refined by one Newton step on y^3 = x.
+__device__ __forceinline__ float cbrt_pos(float x) {
+    if (!(x > 0.f)) return 0.f;
+    float y = __builtin_amdgcn_exp2f(__builtin_amdgcn_logf(x) * (1.0f / 3.0f));
+    y = y - (y * y * y - x) * __builtin_amdgcn_rcpf(3.0f * y * y);
+    return y;
+}
+
+// Linear RGB -> positive XYB, src/ssimu2/makeXYB.hpp:4-50.
+__device__ __forceinline__ void to_xyb(float r, float g, float b, float& X, float& Y, float& B) {
+    const float bias = 0.0037930734f;
+    const float abs_bias = -0.1559542025327239f;
+    float l = fmaf(0.30f, r, fmaf(0.622f, g, fmaf(0.078f, b, bias)));
+    float m = fmaf(0.23f, r, fmaf(0.692f, g, fmaf(0.078f, b, bias)));
+    float s = fmaf(0.24342269f, r, fmaf(0.20476745f, g, fmaf(0.55180986f, b, bias)));
+    l = cbrt_pos(l) + abs_bias;
+    m = cbrt_pos(m) + abs_bias;
+    s = cbrt_pos(s) + abs_bias;
+    const float x = 0.5f * (l - m);
+    const float y = 0.5f * (l + m);
+    B = (s - y) + 0.55f;
+    X = x * 14.0f + 0.42f;
+    Y = y + 0.01f;
+}
+
+// Multiply and add that the compiler may not fuse into an FMA: the map
+// kernel relies on separately rounded products for its exactness on
+// identical inputs.
+__device__ __forceinline__ float mul_nc(float a, float b) {
+#pragma clang fp contract(off)
+    return a * b;
+}
+__device__ __forceinline__ float add_nc(float a, float b) {
+#pragma clang fp contract(off)
+    return a + b;
+}
+__device__ __forceinline__ float sub_nc(float a, float b) {
+#pragma clang fp contract(off)
+    return a - b;
+}
+
+__device__ __forceinline__ float sample_u(const uint8_t* plane, int64_t stride, int x, int y, bool wide) {
+    if (wide) return (float)reinterpret_cast<const uint16_t*>(plane + (int64_t)y * stride)[x];
+    return (float)plane[(int64_t)y * stride + x];
+}
+
+// ---------------------------------------------------------------------------
+// prep kernel
+// ---------------------------------------------------------------------------
+struct PrepLds {
+    float lin0[3][64][65];
+    union {
+        struct {
+            float ch[2][36][37];  // staged chroma (clamped indices)
+            float hh[2][36][65];  // chroma upsampled horizontally to luma columns
+        } c;
+        struct {
+            float l1[3][32][33];
+            float l2[3][16][17];
+            float l3[3][8][9];
+            float l4[3][4][5];
+        } p;
+    } u;
+};
+
+template <typename ST>
+__device__ __forceinline__ void store_xyb(ST* arena, const PairDev& P, int img, int s, int x, int y, float r,
+                                          float g, float b) {
+    float X, Y, B;
+    to_xyb(r, g, b, X, Y, B);
+    ST* base = arena + P.arena_off + img * P.img_elems + P.scale_off[s] + (int64_t)y * P.pitch[s] + x;
+    const int64_t pe = P.plane_elems[s];
+    stf(base, X);
+    stf(base + pe, Y);
+    stf(base + 2 * pe, B);
+}
+
+// One pyramid level in LDS: child(cx, cy) = mean of the 2x2 parents with the
+// parent index clamped to the last valid row/column
+// (src/ssimu2/downsample.hpp:3-19, summation order kept).
+template <int TS, int PP, int CP, typename ST>
+__device__ __forceinline__ void pyr_level(const float* par, float* chi, ST* arena, const PairDev& P, int img, int s,
+                                          int tx, int ty, int npx, int npy) {
+    const int x0 = tx * TS, y0 = ty * TS;
+    const int nvx = min(TS, P.sw[s] - x0), nvy = min(TS, P.sh[s] - y0);
+    for (int idx = threadIdx.x; idx < TS * TS; idx += blockDim.x) {
+        const int cy = idx / TS, cx = idx - cy * TS;
+        if (cx >= nvx || cy >= nvy) continue;
+        const int px0 = 2 * cx, py0 = 2 * cy;
+        const int px1 = min(2 * cx + 1, npx - 1), py1 = min(2 * cy + 1, npy - 1);
+        float v[3];
+#pragma unroll
+        for (int c = 0; c < 3; c++) {
+            const float* pc = par + c * (2 * TS) * PP;
+            float a = pc[py0 * PP + px0];
+            a += pc[py1 * PP + px0];
+            a += pc[py0 * PP + px1];
+            a += pc[py1 * PP + px1];
+            v[c] = a * 0.25f;
+            if (chi) chi[(c * TS + cy) * CP + cx] = v[c];
+        }
+        store_xyb(arena, P, img, s, x0 + cx, y0 + cy, v[0], v[1], v[2]);
+    }
+}
+
+template <typename ST>
+__global__ __launch_bounds__(256) void prep_kernel(const PairDev* __restrict__ pairs,
+                                                   const int32_t* __restrict__ prep_start,
+                                                   const BatchHdr* __restrict__ hdr, ST* __restrict__ arena) {
+    __shared__ PrepLds L;
+    const int npairs = hdr->npairs;
+    const int ntiles = hdr->prep_tiles;
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    for (int t = blockIdx.x; t < ntiles; t += gridDim.x) {
+        int lo = 0, hi = npairs - 1;
+        while (lo < hi) {
+            const int mid = (lo + hi + 1) >> 1;
+            if (prep_start[mid] <= t) lo = mid;
+            else hi = mid - 1;
+        }
+        const PairDev& P = pairs[lo];
+        const int W = P.w, H = P.h;
+        const int ntx = (W + kPrepTile - 1) / kPrepTile, nty = (H + kPrepTile - 1) / kPrepTile;
+        int local = t - prep_start[lo];
+        const int img = local / (ntx * nty);
+        local -= img * ntx * nty;
+        const int ty = local / ntx, tx = local - ty * ntx;
+        const int x0 = tx * kPrepTile, y0 = ty * kPrepTile;
+        const uint8_t* const* src = P.src[img];
+        const int64_t* stride = P.stride[img];
+        const int fmt = P.fmt;
+        const bool yuv = fmt >= IN_YUV420_U8;
+        const bool wide = (fmt == IN_YUV420_U16);
+
+        if (yuv) {
+            // stage the 36x36 chroma neighbourhood (indices clamped to the plane)
+            const int cw = (W + 1) >> 1, chh = (H + 1) >> 1;
+            const int cx0 = x0 / 2 - 1, cy0 = y0 / 2 - 2;
+            for (int idx = tid; idx < 2 * 36 * 36; idx += 256) {
+                const int pl = idx / 1296, rem = idx - pl * 1296;
+                const int r = rem / 36, c = rem - r * 36;
+                const int gy = min(max(cy0 + r, 0), chh - 1), gx = min(max(cx0 + c, 0), cw - 1);
+                L.u.c.ch[pl][r][c] = sample_u(src[pl + 1], stride[pl + 1], gx, gy, wide);
+            }
+            __syncthreads();
+            // horizontal Catmull-Rom, left-sited: even luma x = sample, odd = (-1, 9, 9, -1)/16
+            for (int idx = tid; idx < 2 * 36 * 64; idx += 256) {
+                const int pl = idx / 2304, rem = idx - pl * 2304;
+                const int r = rem >> 6, j = rem & 63;
+                const float* row = L.u.c.ch[pl][r];
+                float v;
+                if ((j & 1) == 0) {
+                    v = row[j / 2 + 1];
+                } else {
+                    const int k = (j - 1) / 2;
+                    v = (-row[k] + 9.0f * row[k + 1] + 9.0f * row[k + 2] - row[k + 3]) * (1.0f / 16.0f);
+                }
+                L.u.c.hh[pl][r][j] = v;
+            }
+            __syncthreads();
+        }
+
+        // scale 0: unpack, linearise, XYB, keep linear RGB in LDS
+        const float sc = yuv ? (float)(1 << (P.bits - 8)) : 1.0f;
+        const float kr = P.kr, kb = P.kb, kg = 1.0f - P.kr - P.kb;
+        for (int j = 0; j < 16; j++) {
+            const int r = wave * 16 + j, c = lane;
+            const int x = x0 + c, y = y0 + r;
+            float rgb[3] = {0.f, 0.f, 0.f};
+            if (x < W && y < H) {
+                if (!yuv) {
+#pragma unroll
+                    for (int p = 0; p < 3; p++) {
+                        const uint8_t* row = src[p] + (int64_t)y * stride[p];
+                        float v;
+                        switch (fmt) {
+                        case IN_RGB_U8: v = (float)row[x] / 255.0f; break;
+                        case IN_RGB_U16: v = (float)reinterpret_cast<const uint16_t*>(row)[x] / 65535.0f; break;
+                        case IN_RGB_F16: v = __half2float(reinterpret_cast<const __half*>(row)[x]); break;
+                        default: v = reinterpret_cast<const float*>(row)[x]; break;
+                        }
+                        rgb[p] = v;
+                    }
+                } else {
+                    // vertical Catmull-Rom, centre-sited
+                    const bool odd = (r & 1) != 0;
+                    const int rb = odd ? (r - 1) / 2 + 1 : r / 2;
+                    const float w0 = odd ? -9.0f : -3.0f, w1 = odd ? 111.0f : 29.0f;
+                    const float w2 = odd ? 29.0f : 111.0f, w3 = odd ? -3.0f : -9.0f;
+                    float cbcr[2];
+#pragma unroll
+                    for (int pl = 0; pl < 2; pl++) {
+                        const float v = w0 * L.u.c.hh[pl][rb][c] + w1 * L.u.c.hh[pl][rb + 1][c] +
+                                        w2 * L.u.c.hh[pl][rb + 2][c] + w3 * L.u.c.hh[pl][rb + 3][c];
+                        cbcr[pl] = (v * (1.0f / 128.0f) - 128.0f * sc) / (224.0f * sc);
+                    }
+                    const float Yv = (sample_u(src[0], stride[0], x, y, wide) - 16.0f * sc) / (219.0f * sc);
+                    const float R = Yv + 2.0f * (1.0f - kr) * cbcr[1];
+                    const float B = Yv + 2.0f * (1.0f - kb) * cbcr[0];
+                    const float G = (Yv - kr * R - kb * B) / kg;
+                    rgb[0] = R;
+                    rgb[1] = G;
+                    rgb[2] = B;
+                }
+#pragma unroll
+                for (int p = 0; p < 3; p++) rgb[p] = linearize(rgb[p]);
+                store_xyb(arena, P, img, 0, x, y, rgb[0], rgb[1], rgb[2]);
+            }
+#pragma unroll
+            for (int p = 0; p < 3; p++) L.lin0[p][r][c] = rgb[p];
+        }
+        __syncthreads();
+
+        // scales 1..5 from LDS
+        const int nv0x = min(64, W - x0), nv0y = min(64, H - y0);
+        pyr_level<32, 65, 33>(&L.lin0[0][0][0], &L.u.p.l1[0][0][0], arena, P, img, 1, tx, ty, nv0x, nv0y);
+        __syncthreads();
+        const int nv1x = min(32, P.sw[1] - tx * 32), nv1y = min(32, P.sh[1] - ty * 32);
+        pyr_level<16, 33, 17>(&L.u.p.l1[0][0][0], &L.u.p.l2[0][0][0], arena, P, img, 2, tx, ty, nv1x, nv1y);
+        __syncthreads();
+        const int nv2x = min(16, P.sw[2] - tx * 16), nv2y = min(16, P.sh[2] - ty * 16);
+        pyr_level<8, 17, 9>(&L.u.p.l2[0][0][0], &L.u.p.l3[0][0][0], arena, P, img, 3, tx, ty, nv2x, nv2y);
+        __syncthreads();
+        const int nv3x = min(8, P.sw[3] - tx * 8), nv3y = min(8, P.sh[3] - ty * 8);
+        pyr_level<4, 9, 5>(&L.u.p.l3[0][0][0], &L.u.p.l4[0][0][0], arena, P, img, 4, tx, ty, nv3x, nv3y);
+        __syncthreads();
+        const int nv4x = min(4, P.sw[4] - tx * 4), nv4y = min(4, P.sh[4] - ty * 4);
+        pyr_level<2, 5, 1>(&L.u.p.l4[0][0][0], (float*)nullptr, arena, P, img, 5, tx, ty, nv4x, nv4y);
+        __syncthreads();
+    }
+}
+
+// ---------------------------------------------------------------------------
+// map kernel
+// ---------------------------------------------------------------------------
+template <typename ST>
+struct Row4 {
+    static __device__ __forceinline__ float4 load(const ST* p);
+};
+template <>
+__device__ __forceinline__ float4 Row4<float>::load(const float* p) {
+    return *reinterpret_cast<const float4*>(p);
+}
+template <>
+__device__ __forceinline__ float4 Row4<__half>::load(const __half* p) {
+    const uint2 u = *reinterpret_cast<const uint2*>(p);
+    const __half2 a = *reinterpret_cast<const __half2*>(&u.x);
+    const __half2 b = *reinterpret_cast<const __half2*>(&u.y);
+    const float2 fa = __half22float2(a), fb = __half22float2(b);
+    return make_float4(fa.x, fa.y, fb.x, fb.y);
+}
+
+template <int TH, typename AT>
+struct MapCfg {
+    static constexpr int TW = kMapTW;
+    static constexpr int HR = TH + 16;  // rows of the row-pass result
+    static constexpr int R = TH / 8;    // output rows per thread in the column pass
+    // LDS row pitch (elements): the two half-waves of the column pass land on
+    // disjoint banks
+    static constexpr int LDW = sizeof(AT) == 4 ? ((R == 8) ? 36 : 40) : 34;
+};
+
+// Arithmetic of the moment accumulation type: FP32 with the hardware
+// reciprocal, or FP64 with IEEE division.
+template <typename AT>
+struct Arith;
+template <>
+struct Arith<float> {
+    static __device__ __forceinline__ float rcp(float x) { return __builtin_amdgcn_rcpf(x); }
+};
+template <>
+struct Arith<double> {
+    static __device__ __forceinline__ double rcp(double x) { return 1.0 / x; }
+};
+
+__device__ __forceinline__ double mul_nc(double a, double b) {
+#pragma clang fp contract(off)
+    return a * b;
+}
+__device__ __forceinline__ double add_nc(double a, double b) {
+#pragma clang fp contract(off)
+    return a + b;
+}
+__device__ __forceinline__ double sub_nc(double a, double b) {
+#pragma clang fp contract(off)
+    return a - b;
+}
+
+// Sum of the in-bounds normalised taps at position p of an axis of length n,
+// from the cumulative tap table (renormalisation of src/ssimu2/gaussianblur.hpp:58-60).
+template <typename AT>
+__device__ __forceinline__ AT tap_total(const AT* cum, int p, int n) {
+    const int lo = max(0, 8 - p);
+    const int hi = max(min(16, n - 1 - p + 8), lo - 1);
+    return cum[hi + 1] - cum[lo];
+}
+
+// One map tile class: AT is the accumulation type of the moment blur.
+template <int TH, typename ST, typename AT>
+__global__ __launch_bounds__(256) void map_kernel(const PairDev* __restrict__ pairs,
+                                                  const int32_t* __restrict__ order_seg,
+                                                  const int32_t* __restrict__ order_start,
+                                                  const BatchHdr* __restrict__ hdr, const ST* __restrict__ arena,
+                                                  double* __restrict__ partials, const Taps tp) {
+    using C = MapCfg<TH, AT>;
+    constexpr int TW = C::TW, HR = C::HR, R = C::R, LDW = C::LDW;
+    constexpr int cls = sizeof(AT) == 4 ? 0 : 1;
+    __shared__ AT hb[4][HR][LDW];   // row-pass results: a, b, a^2 + b^2, ab
+    __shared__ AT cen[2][TH][LDW];  // centred samples of the output pixels
+    __shared__ double red[4][kFeat];
+    __shared__ AT cum[18];
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    if (tid == 0) {
+        AT a = 0;
+        cum[0] = 0;
+        for (int k = 0; k < 17; k++) {
+            a += (AT)tp.g[k];
+            cum[k + 1] = a;
+        }
+    }
+    __syncthreads();
+    const int seg_lo = cls == 0 ? 0 : hdr->nseg[0];
+    const int seg_hi = seg_lo + hdr->nseg[cls] - 1;
+    const int t_begin = cls == 0 ? 0 : hdr->tiles[0];
+    const int t_end = t_begin + hdr->tiles[cls];
+    AT gr[17];
+#pragma unroll
+    for (int k = 0; k < 17; k++) gr[k] = (AT)tp.g[k];
+
+    for (int t = t_begin + blockIdx.x; t < t_end; t += gridDim.x) {
+        int lo = seg_lo, hi = seg_hi;
+        while (lo < hi) {
+            const int mid = (lo + hi + 1) >> 1;
+            if (order_start[mid] <= t) lo = mid;
+            else hi = mid - 1;
+        }
+        const int seg = order_seg[lo];
+        const int pair = seg / kScales, s = seg - pair * kScales;
+        const PairDev& P = pairs[pair];
+        const int W = P.sw[s], H = P.sh[s];
+        const int64_t pitch = P.pitch[s];
+        const int ntx = (W + TW - 1) / TW;
+        const int local = t - order_start[lo];
+        const int ty = local / ntx, tx = local - ty * ntx;
+        const int x0 = tx * TW, y0 = ty * TH;
+        const ST* base1 = arena + P.arena_off + P.scale_off[s];
+        const ST* base2 = base1 + P.img_elems;
+        const int64_t pe = P.plane_elems[s];
+        const bool xin = (x0 >= 8) && (x0 + TW + 8 <= W);
+
+        // column-pass coordinates: 32 columns x 8 groups of R rows
+        const int vj = tid & 31, vg = tid >> 5;
+        const int vx = x0 + vj;
+        const int i0 = vg * R;
+        const AT invh = Arith<AT>::rcp(tap_total(cum, vx, W));
+
+        // row-pass coordinates: 8 threads per row, 4 adjacent outputs each
+        const int hr0 = tid >> 3, j0 = (tid & 7) * 4;
+        const int xs = x0 + j0 - 8;
+
+#pragma unroll 1
+        for (int c = 0; c < 3; c++) {
+            const ST* p1 = base1 + c * pe;
+            const ST* p2 = base2 + c * pe;
+            // per-tile centring constant: the moments are taken of (value - cc),
+            // exact for a renormalised blur, which keeps E[x^2] - E[x]^2 from
+            // cancelling away the FP32 precision
+            const int cxp = min(x0 + TW / 2, W - 1), cyp = min(y0 + TH / 2, H - 1);
+            const AT cc = (AT)ldf(p1 + (int64_t)cyp * pitch + cxp);
+
+            // ---- row pass: rows y0-8 .. y0+TH+7, columns x0 .. x0+31
+#pragma unroll 1
+            for (int r = hr0; r < HR; r += 32) {
+                const int y = y0 - 8 + r;
+                AT ha[4] = {0, 0, 0, 0}, hbv[4] = {0, 0, 0, 0};
+                AT hp[4] = {0, 0, 0, 0}, hq[4] = {0, 0, 0, 0};
+                AT ca[4] = {0, 0, 0, 0}, cb[4] = {0, 0, 0, 0};
+                if (y >= 0 && y < H) {
+                    const ST* r1 = p1 + (int64_t)y * pitch;
+                    const ST* r2 = p2 + (int64_t)y * pitch;
+                    AT av[20], bv[20];
+                    if (xin) {
+#pragma unroll
+                        for (int q = 0; q < 5; q++) {
+                            const float4 u = Row4<ST>::load(r1 + xs + 4 * q);
+                            const float4 w = Row4<ST>::load(r2 + xs + 4 * q);
+                            av[4 * q + 0] = (AT)u.x - cc; av[4 * q + 1] = (AT)u.y - cc;
+                            av[4 * q + 2] = (AT)u.z - cc; av[4 * q + 3] = (AT)u.w - cc;
+                            bv[4 * q + 0] = (AT)w.x - cc; bv[4 * q + 1] = (AT)w.y - cc;
+                            bv[4 * q + 2] = (AT)w.z - cc; bv[4 * q + 3] = (AT)w.w - cc;
+                        }
+                    } else {
+#pragma unroll
+                        for (int k = 0; k < 20; k++) {
+                            const int x = xs + k;
+                            const bool in = (x >= 0) && (x < W);
+                            const int xc = min(max(x, 0), W - 1);
+                            const AT u = (AT)ldf(r1 + xc), w = (AT)ldf(r2 + xc);
+                            av[k] = in ? u - cc : (AT)0;
+                            bv[k] = in ? w - cc : (AT)0;
+                        }
+                    }
+#pragma unroll
+                    for (int o = 0; o < 4; o++) {
+                        ca[o] = av[8 + o];
+                        cb[o] = bv[8 + o];
+                    }
+#pragma unroll
+                    for (int k = 0; k < 20; k++) {
+                        const AT a = av[k], b = bv[k];
+                        // products rounded separately so that identical inputs give
+                        // a^2 + b^2 == 2ab bit for bit
+                        const AT pp = add_nc(mul_nc(a, a), mul_nc(b, b)), qq = mul_nc(a, b);
+#pragma unroll
+                        for (int o = 0; o < 4; o++) {
+                            const int kk = k - o;
+                            if (kk >= 0 && kk <= 16) {
+                                ha[o] = fma(gr[kk], a, ha[o]);
+                                hbv[o] = fma(gr[kk], b, hbv[o]);
+                                hp[o] = fma(gr[kk], pp, hp[o]);
+                                hq[o] = fma(gr[kk], qq, hq[o]);
+                            }
+                        }
+                    }
+                }
+#pragma unroll
+                for (int o = 0; o < 4; o++) {
+                    hb[0][r][j0 + o] = ha[o];
+                    hb[1][r][j0 + o] = hbv[o];
+                    hb[2][r][j0 + o] = hp[o];
+                    hb[3][r][j0 + o] = hq[o];
+                }
+                if (r >= 8 && r < TH + 8) {
+#pragma unroll
+                    for (int o = 0; o < 4; o++) {
+                        cen[0][r - 8][j0 + o] = ca[o];
+                        cen[1][r - 8][j0 + o] = cb[o];
+                    }
+                }
+            }
+            __syncthreads();
+
+            // ---- column pass, streaming down the R output rows of column vj;
+            // output row m is complete after input row m + 16, and its maps
+            // (src/ssimu2/score.hpp:138-172) are formed right there.
+            AT v0[R], v1[R], v2[R], v3[R];
+#pragma unroll
+            for (int m = 0; m < R; m++) v0[m] = v1[m] = v2[m] = v3[m] = 0;
+            AT acc[6] = {0, 0, 0, 0, 0, 0};
+#pragma unroll
+            for (int k = 0; k < R + 16; k++) {
+                const AT h0 = hb[0][i0 + k][vj], h1 = hb[1][i0 + k][vj];
+                const AT h2 = hb[2][i0 + k][vj], h3 = hb[3][i0 + k][vj];
+#pragma unroll
+                for (int m = 0; m < R; m++) {
+                    const int kk = k - m;
+                    if (kk >= 0 && kk <= 16) {
+                        v0[m] = fma(gr[kk], h0, v0[m]);
+                        v1[m] = fma(gr[kk], h1, v1[m]);
+                        v2[m] = fma(gr[kk], h2, v2[m]);
+                        v3[m] = fma(gr[kk], h3, v3[m]);
+                    }
+                }
+                if (k >= 16) {
+                    const int m = k - 16;
+                    const int y = y0 + i0 + m;
+                    if (vx < W && y < H) {
+                        // Written so that identical inputs give exactly 0: the third
+                        // moment plane is then exactly twice the fourth (FMA chains
+                        // scale exactly by 2), so den_s == num_s bit for bit, and both
+                        // ratios are formed as (difference) * 1/denominator.
+                        const AT iv = invh * Arith<AT>::rcp(tap_total(cum, y, H));
+                        const AT mu1 = v0[m] * iv, mu2 = v1[m] * iv;
+                        const AT S = mul_nc((AT)0.5, mul_nc(v2[m], iv)), Q = mul_nc(v3[m], iv);
+                        const AT m11 = mul_nc(mu1, mu1), m22 = mul_nc(mu2, mu2);
+                        const AT m12 = mul_nc(mu1, mu2);
+                        const AT md = mu1 - mu2;
+                        const AT num_m = fma(-md, md, (AT)1);
+                        const AT e1 = sub_nc(S, mul_nc((AT)0.5, add_nc(m11, m22)));  // (sigma11 + sigma22) / 2
+                        const AT e2 = sub_nc(Q, m12);                                 // sigma12
+                        const AT num_s = fma((AT)2, e2, (AT)0.0009);
+                        const AT den_s = fma((AT)2, e1, (AT)0.0009);
+                        AT d = fma(-num_m, num_s, den_s) * Arith<AT>::rcp(den_s);
+                        d = d > 0 ? d : (AT)0;
+                        const AT a = cen[0][i0 + m][vj], b = cen[1][i0 + m][vj];
+                        const AT fa = fabs(a - mu1) + (AT)1, fb = fabs(b - mu2) + (AT)1;
+                        const AT ev = (fb - fa) * Arith<AT>::rcp(fa);
+                        const AT art = ev > 0 ? ev : (AT)0, det = ev < 0 ? -ev : (AT)0;
+                        const AT d2 = d * d, a2 = art * art, t2 = det * det;
+                        acc[0] += d;
+                        acc[1] += d2 * d2;
+                        acc[2] += art;
+                        acc[3] += a2 * a2;
+                        acc[4] += det;
+                        acc[5] += t2 * t2;
+                    }
+                }
+            }
+            // wave sums of this channel in FP64, fixed order
+#pragma unroll
+            for (int k = 0; k < 6; k++) {
+                double x = (double)acc[k];
+#pragma unroll
+                for (int off = 32; off > 0; off >>= 1) x += __shfl_xor(x, off, 64);
+                if (lane == 0) red[wave][c * 6 + k] = x;
+            }
+            __syncthreads();
+        }
+
+        // ---- tile sums: the four waves in a fixed order
+        if (tid < kFeat) {
+            const double sum = ((red[0][tid] + red[1][tid]) + red[2][tid]) + red[3][tid];
+            partials[(int64_t)t * kFeat + tid] = sum;
+        }
+        __syncthreads();
+    }
+}
+
+// ---------------------------------------------------------------------------
+// finalize kernel
+// ---------------------------------------------------------------------------
+// The 108 published SSIMULACRA2 weights, reference order f[c*36 + s*6 + n*3 + m]
+// (src/ssimu2/score.hpp:272-381).
+__constant__ double kWeights[108] = {
+    // X
+    0.0, 0.0007376606707406586, 0.0, 0.0, 0.0007793481682867309, 0.0,
+    0.0, 0.0004371155730107379, 0.0, 1.1041726426657346, 0.00066284834129271, 0.00015231632783718752,
+    0.0, 0.0016406437456599754, 0.0, 1.8422455520539298, 11.441172603757666, 0.0,
+    0.0007989109436015163, 0.000176816438078653, 0.0, 1.8787594979546387, 10.94906990605142, 0.0,
+    0.0007289346991508072, 0.9677937080626833, 0.0, 0.00014003424285435884, 0.9981766977854967, 0.00031949755934435053,
+    0.0004550992113792063, 0.0, 0.0, 0.0013648766163243398, 0.0, 0.0,
+    // Y
+    0.0, 0.0, 0.0, 7.466890328078848, 0.0, 17.445833984131262,
+    0.0006235601634041466, 0.0, 0.0, 6.683678146179332, 0.00037724407979611296, 1.027889937768264,
+    225.20515300849274, 0.0, 0.0, 19.213238186143016, 0.0011401524586618361, 0.001237755635509985,
+    176.39317598450694, 0.0, 0.0, 24.43300999870476, 0.28520802612117757, 0.0004485436923833408,
+    0.0, 0.0, 0.0, 34.77906344483772, 44.835625328877896, 0.0,
+    0.0, 0.0, 0.0, 0.0, 0.0, 0.0,
+    // B
+    0.0, 0.0008680556573291698, 0.0, 0.0, 0.0, 0.0,
+    0.0, 0.0005313191874358747, 0.0, 0.00016533814161379112, 0.0, 0.0,
+    0.0, 0.0, 0.0, 0.0004179171803251336, 0.0017290828234722833, 0.0,
+    0.0020827005846636437, 0.0, 0.0, 8.826982764996862, 23.19243343998926, 0.0,
+    95.1080498811086, 0.9863978034400682, 0.9834382792465353, 0.0012286405048278493, 171.2667255897307, 0.9807858872435379,
+    0.0, 0.0, 0.0, 0.0005130064588990679, 0.0, 0.00010854057858411537,
+};
+
+__device__ double ssimu2_score_from_features(const double* f) {
+    double s = 0.0;
+    for (int i = 0; i < 108; i++) s = fma(kWeights[i], f[i], s);
+    s *= 0.9562382616834844;
+    s = (6.248496625763138e-5 * s * s) * s + 2.326765642916932 * s - 0.020884521182843837 * s * s;
+    return s > 0.0 ? 100.0 - 10.0 * pow(s, 0.6276336467831387) : 100.0;
+}
+
+__global__ __launch_bounds__(256) void finalize_kernel(const PairDev* __restrict__ pairs,
+                                                       const int32_t* __restrict__ seg_start,
+                                                       const int32_t* __restrict__ seg_count,
+                                                       const BatchHdr* __restrict__ hdr,
+                                                       const double* __restrict__ partials,
+                                                       double* __restrict__ scores, double* __restrict__ features) {
+    __shared__ double part[14][kFeat];
+    __shared__ double feat[108];
+    const int tid = threadIdx.x;
+    const int npairs = hdr->npairs;
+    for (int p = blockIdx.x; p < npairs; p += gridDim.x) {
+        const PairDev& P = pairs[p];
+        for (int s = 0; s < kScales; s++) {
+            const int t0 = seg_start[p * kScales + s], t1 = t0 + seg_count[p * kScales + s];
+            if (tid < 14 * kFeat) {
+                const int k = tid % kFeat, g = tid / kFeat;
+                double a = 0.0;
+                for (int t = t0 + g; t < t1; t += 14) a += partials[(int64_t)t * kFeat + k];
+                part[g][k] = a;
+            }
+            __syncthreads();
+            if (tid < kFeat) {
+                double a = 0.0;
+                for (int g = 0; g < 14; g++) a += part[g][tid];
+                const double n = (double)P.sw[s] * (double)P.sh[s];
+                const int c = tid / 6, mnorm = tid % 6, m = mnorm >> 1, nrm = mnorm & 1;
+                const double mean = a / n;
+                feat[c * 36 + s * 6 + nrm * 3 + m] = nrm ? sqrt(sqrt(mean)) : mean;
+            }
+            __syncthreads();
+        }
+        if (tid == 0) scores[p] = ssimu2_score_from_features(feat);
+        if (features && tid < 108) features[(int64_t)p * 108 + tid] = feat[tid];
+        __syncthreads();
+    }
+}
+
+// ---------------------------------------------------------------------------
+// host-side launchers
+// ---------------------------------------------------------------------------
+hipError_t launch_prep(bool fp16, int grid, const PairDev* pairs, const int32_t* prep_start, const BatchHdr* hdr,
+                       void* arena, hipStream_t st) {
+    if (fp16)
+        prep_kernel<__half><<<grid, 256, 0, st>>>(pairs, prep_start, hdr, reinterpret_cast<__half*>(arena));
+    else
+        prep_kernel<float><<<grid, 256, 0, st>>>(pairs, prep_start, hdr, reinterpret_cast<float*>(arena));
+    return hipGetLastError();
+}
+
+template <int TH, typename ST, typename AT>
+static void launch_map_t(int grid, const PairDev* pairs, const int32_t* order_seg, const int32_t* order_start,
+                         const BatchHdr* hdr, const void* arena, double* partials, const Taps& taps, hipStream_t st) {
+    map_kernel<TH, ST, AT><<<grid, 256, 0, st>>>(pairs, order_seg, order_start, hdr,
+                                                 reinterpret_cast<const ST*>(arena), partials, taps);
+}
+
+hipError_t launch_map(bool fp16, int cls, int th, int grid, const PairDev* pairs, const int32_t* order_seg,
+                      const int32_t* order_start, const BatchHdr* hdr, const void* arena, double* partials,
+                      const Taps& taps, hipStream_t st) {
+    if (cls == 1) {
+        if (fp16) launch_map_t<kSmallTH, __half, double>(grid, pairs, order_seg, order_start, hdr, arena, partials, taps, st);
+        else launch_map_t<kSmallTH, float, double>(grid, pairs, order_seg, order_start, hdr, arena, partials, taps, st);
+    } else if (th == 32) {
+        if (fp16) launch_map_t<32, __half, float>(grid, pairs, order_seg, order_start, hdr, arena, partials, taps, st);
+        else launch_map_t<32, float, float>(grid, pairs, order_seg, order_start, hdr, arena, partials, taps, st);
+    } else {
+        if (fp16) launch_map_t<64, __half, float>(grid, pairs, order_seg, order_start, hdr, arena, partials, taps, st);
+        else launch_map_t<64, float, float>(grid, pairs, order_seg, order_start, hdr, arena, partials, taps, st);
+    }
+    return hipGetLastError();
+}
+
+hipError_t launch_finalize(int grid, const PairDev* pairs, const int32_t* seg_start, const int32_t* seg_count,
+                           const BatchHdr* hdr, const double* partials, double* scores, double* features,
+                           hipStream_t st) {
+    finalize_kernel<<<grid, 256, 0, st>>>(pairs, seg_start, seg_count, hdr, partials, scores, features);
+    return hipGetLastError();
+}
+
+__global__ void check_kernel(int* out) { out[0] = 4320984; }
+
+hipError_t launch_check(int* out, hipStream_t st) {
+    check_kernel<<<1, 1, 0, st>>>(out);
+    return hipGetLastError();
+}
+
+}  // namespace vship

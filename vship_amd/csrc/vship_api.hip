@@ -1,0 +1,502 @@
+// C ABI of libvship.so: the reference's VshipAPI.h entry points
+// (src/VshipLib.cpp:71-250 of the reference) and the additive batched
+// entry points of VshipSSIMU2Batch.h. No C++ exception crosses this boundary.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <atomic>
+#include <cstring>
+#include <memory>
+#include <mutex>
+#include <string>
+#include <vector>
+
+#include "../../include/VshipAPI.h"
+#include "../../include/VshipSSIMU2Batch.h"
+#include "api_internal.hpp"
+#include "engine.hpp"
+#include "ssimu2_kernels.hpp"
+
+using namespace vship;
+
+namespace vship {
+
+// Message texts of the reference (src/util/VshipExceptions.hpp:41-86); callers
+// parse them, so they are kept word for word.
+const char* error_message(int code) {
+    switch (code) {
+    case E_NoError: return "NoError: Vship succeeded";
+    case E_OutOfVRAM:
+        return "OutOfVRAM: Vship was not able to perform GPU memory allocation. (Advice) Reduce or Set numStream "
+               "argument";
+    case E_OutOfRAM:
+        return "OutOfRAM: Vship was not able to allocate CPU memory. This is a rare error that should be reported. "
+               "Check your RAM usage";
+    case E_DifferingInputType:
+        return "DifferingInputType: Vship received 2 videos with different properties. (Advice) verify that they "
+               "have the same width, height and length";
+    case E_NonRGBSInput:
+        return "NonRGBSInput: Vship did not manage to get RGBS format of your inputs. This should not happen. "
+               "(Advice) try converting yourself to RGBS";
+    case E_DeviceCountError:
+        return "DeviceCountError: Vship was unable to verify the number of GPU on your system. (Advice) Did you "
+               "select the correct binary for your device AMD/NVIDIA. (Advice) if linux AMD, are you in video and "
+               "render groups?";
+    case E_NoDeviceDetected:
+        return "NoDeviceDetected: Vship found no device on your system. (Advice) Did you select the correct binary "
+               "for your device AMD/NVIDIA. (Advice) if linux AMD, are you in video and render groups?";
+    case E_BadDeviceArgument:
+        return "BadDeviceArgument: Vship received a bad gpu_id argument either you specified a number >= to your "
+               "gpu count, either it was negative";
+    case E_BadDeviceCode:
+        return "BadDeviceCode: Vship was unable to run a simple GPU Kernel. This usually indicate that the code was "
+               "compiled for the wrong architecture. (Advice) Try to compile vship yourself, eventually replace "
+               "--offload-arch=native to your arch";
+    case E_BadHandler:
+        return "BadHandler: The handler used was not allocated. This indicates an issue in Vship API Usage. Did you "
+               "call the Init function or altered the handler id?";
+    case E_BadPointer:
+        return "BadPointer: The pointer passed as an argument was not valid and returned an error. (Advice) If this "
+               "happened while trying to free, was it allocated using the same method and not modified?";
+    default: return "BadErrorType: There was an unknown error";
+    }
+}
+
+int device_count_or_throw() {
+    int n = 0;
+    if (hipGetDeviceCount(&n) != hipSuccess) VSHIP_THROW(E_DeviceCountError);
+    if (n == 0) VSHIP_THROW(E_NoDeviceDetected);
+    return n;
+}
+
+// Runs a one-thread kernel that writes a known value (src/util/gpuhelper.hpp:40-65).
+bool gpu_kernel_check() {
+    int* d = nullptr;
+    int h = 0;
+    if (hipMalloc(&d, sizeof(int)) != hipSuccess) return false;
+    bool ok = hipMemset(d, 0, sizeof(int)) == hipSuccess && launch_check(d, nullptr) == hipSuccess &&
+              hipMemcpy(&h, d, sizeof(int), hipMemcpyDeviceToHost) == hipSuccess;
+    (void)hipFree(d);
+    return ok && h == 4320984;
+}
+
+void gpu_full_check(int gpu_id) {
+    const int n = device_count_or_throw();
+    if (gpu_id < 0 || gpu_id >= n) VSHIP_THROW(E_BadDeviceArgument);
+    if (hipSetDevice(gpu_id) != hipSuccess) VSHIP_THROW(E_BadDeviceArgument);
+    if (!gpu_kernel_check()) VSHIP_THROW(E_BadDeviceCode);
+}
+
+// ---------------------------------------------------------------------------
+// per-frame handles (Vship_SSIMU2Handler)
+// ---------------------------------------------------------------------------
+FrameHandle::FrameHandle(int w, int h, int device) : width(w), height(h), device(device) {
+    EngineOptions o;
+    o.device = device;
+    o.max_pairs = 1;
+    o.map_th = default_map_th(w, h);
+    o.arena_bytes = pair_elems(w, h) * 4;
+    o.max_map_tiles = map_tiles(w, h, 32);
+    engine = std::make_unique<Engine>(o);
+}
+
+FrameHandle::~FrameHandle() {
+    if (staging) (void)hipFree(staging);
+}
+
+// Segments below kSmallSegment pixels run in the FP64 class with their own
+// tile height, so the FP32 class only sees scales of at least 32768 pixels,
+// where 64-row tiles waste little; a size-independent default also keeps a
+// pair's result independent of the batch it is in.
+int default_map_th(int, int) { return 64; }
+
+double FrameHandle::compute(int fmt, const uint8_t* const p1[3], const uint8_t* const p2[3], int64_t s1, int64_t s2) {
+    if (!p1 || !p2) VSHIP_THROW(E_BadPointer);
+    for (int i = 0; i < 3; i++)
+        if (!p1[i] || !p2[i]) VSHIP_THROW(E_BadPointer);
+    const int bps = fmt == IN_RGB_F32 ? 4 : (fmt == IN_RGB_U8 ? 1 : 2);
+    const int64_t row = (int64_t)width * bps;
+    if (s1 < row || s2 < row) VSHIP_THROW(E_BadPointer);
+    // rows are copied without their padding into a packed device staging area
+    const size_t need = (size_t)row * height * 6;
+    if (need > staging_bytes) {
+        if (staging) (void)hipFree(staging);
+        staging = nullptr;
+        staging_bytes = 0;
+        HIP_OR_THROW(hipSetDevice(device), E_BadDeviceArgument);
+        HIP_OR_THROW(hipMalloc(reinterpret_cast<void**>(&staging), need), E_OutOfVRAM);
+        staging_bytes = need;
+    }
+    hipStream_t st = engine->stream();
+    PairInput in{};
+    for (int im = 0; im < 2; im++) {
+        const uint8_t* const* src = im == 0 ? p1 : p2;
+        const int64_t sp = im == 0 ? s1 : s2;
+        for (int pl = 0; pl < 3; pl++) {
+            uint8_t* dst = staging + (size_t)row * height * (im * 3 + pl);
+            HIP_OR_THROW(hipMemcpy2DAsync(dst, row, src[pl], sp, row, height, hipMemcpyHostToDevice, st),
+                         E_BadPointer);
+            in.src[im][pl] = dst;
+            in.stride[im][pl] = row;
+        }
+    }
+    in.w = width;
+    in.h = height;
+    in.fmt = fmt;
+    double score = 0.0;
+    engine->submit(&in, 1);
+    engine->wait(&score, nullptr);
+    return score;
+}
+
+template <typename T>
+struct Table {
+    std::mutex lock;
+    std::vector<std::shared_ptr<T>> items;
+    std::vector<char> used;
+    int allocate() {
+        std::lock_guard<std::mutex> g(lock);
+        for (size_t i = 0; i < used.size(); i++)
+            if (!used[i]) {
+                used[i] = 1;
+                return (int)i;
+            }
+        items.emplace_back();
+        used.push_back(1);
+        return (int)items.size() - 1;
+    }
+    std::shared_ptr<T> get(int id) {
+        std::lock_guard<std::mutex> g(lock);
+        if (id < 0 || id >= (int)items.size() || !used[id]) return nullptr;
+        return items[id];
+    }
+    bool valid(int id) {
+        std::lock_guard<std::mutex> g(lock);
+        return id >= 0 && id < (int)items.size() && used[id];
+    }
+    void set(int id, std::shared_ptr<T> v) {
+        std::lock_guard<std::mutex> g(lock);
+        items[id] = std::move(v);
+    }
+    bool release(int id) {
+        std::shared_ptr<T> v;
+        {
+            std::lock_guard<std::mutex> g(lock);
+            if (id < 0 || id >= (int)items.size() || !used[id]) return false;
+            v = std::move(items[id]);
+            used[id] = 0;
+        }
+        return true;  // v is destroyed outside the lock
+    }
+};
+
+Table<FrameHandle>& frame_table() {
+    static Table<FrameHandle> t;
+    return t;
+}
+
+struct BatchHandle {
+    std::unique_ptr<Engine> engine;
+    std::atomic<bool> busy{false};
+};
+
+Table<BatchHandle>& batch_table() {
+    static Table<BatchHandle> t;
+    return t;
+}
+
+}  // namespace vship
+
+namespace {
+
+struct BusyGuard {
+    std::atomic<bool>& f;
+    bool ok;
+    explicit BusyGuard(std::atomic<bool>& flag) : f(flag), ok(!flag.exchange(true)) {}
+    ~BusyGuard() {
+        if (ok) f.store(false);
+    }
+};
+
+template <typename F>
+Vship_Exception guarded(F&& fn) {
+    try {
+        fn();
+    } catch (const VshipErr& e) {
+        return (Vship_Exception)e.code;
+    } catch (const std::bad_alloc&) {
+        return Vship_OutOfRAM;
+    } catch (...) {
+        return Vship_BadErrorType;
+    }
+    return Vship_NoError;
+}
+
+Vship_Exception compute_frame(Vship_SSIMU2Handler handler, double* score, const uint8_t* srcp1[3],
+                              const uint8_t* srcp2[3], int64_t s1, int64_t s2, int fmt) {
+    std::shared_ptr<FrameHandle> h = frame_table().get(handler.id);
+    if (!h || !h->engine) return Vship_BadHandler;
+    if (!score) return Vship_BadPointer;
+    BusyGuard g(h->busy);
+    if (!g.ok) return Vship_BadHandler;
+    double r = 0.0;
+    const Vship_Exception e = guarded([&] { r = h->compute(fmt, srcp1, srcp2, s1, s2); });
+    if (e == Vship_NoError) *score = r;
+    return e;
+}
+
+PairInput to_pair_input(const Vship_SSIMU2Pair& p) {
+    PairInput in{};
+    for (int i = 0; i < 3; i++) {
+        in.src[0][i] = p.ref[i];
+        in.src[1][i] = p.dist[i];
+        in.stride[0][i] = p.ref_stride[i];
+        in.stride[1][i] = p.dist_stride[i];
+    }
+    in.w = p.width;
+    in.h = p.height;
+    in.fmt = p.format;
+    in.bits = p.bits;
+    in.matrix = p.matrix;
+    return in;
+}
+
+}  // namespace
+
+extern "C" {
+
+Vship_Version Vship_GetVersion(void) {
+    Vship_Version v;
+    v.major = 3;
+    v.minor = 1;
+    v.minorMinor = 0;
+    v.backend = Vship_HIP;
+    return v;
+}
+
+Vship_Exception Vship_GetDeviceCount(int* number) {
+    if (!number) return Vship_BadPointer;
+    int n = 0;
+    const Vship_Exception e = guarded([&] { n = device_count_or_throw(); });
+    if (e == Vship_NoError) *number = n;
+    return e;
+}
+
+Vship_Exception Vship_SetDevice(int gpu_id) {
+    return guarded([&] {
+        const int n = device_count_or_throw();
+        if (gpu_id < 0 || gpu_id >= n) VSHIP_THROW(E_BadDeviceArgument);
+        if (hipSetDevice(gpu_id) != hipSuccess) VSHIP_THROW(E_BadDeviceArgument);
+    });
+}
+
+Vship_Exception Vship_GetDeviceInfo(Vship_DeviceInfo* info, int gpu_id) {
+    if (!info) return Vship_BadPointer;
+    return guarded([&] {
+        const int n = device_count_or_throw();
+        if (gpu_id < 0 || gpu_id >= n) VSHIP_THROW(E_BadDeviceArgument);
+        hipDeviceProp_t p;
+        if (hipGetDeviceProperties(&p, gpu_id) != hipSuccess) VSHIP_THROW(E_BadDeviceArgument);
+        std::memset(info->name, 0, sizeof(info->name));
+        std::memcpy(info->name, p.name, std::min(sizeof(info->name) - 1, sizeof(p.name)));
+        info->VRAMSize = p.totalGlobalMem;
+        info->integrated = p.integrated;
+        info->MultiProcessorCount = p.multiProcessorCount;
+        info->WarpSize = p.warpSize;
+    });
+}
+
+Vship_Exception Vship_GPUFullCheck(int gpu_id) {
+    return guarded([&] { gpu_full_check(gpu_id); });
+}
+
+int Vship_GetErrorMessage(Vship_Exception exception, char* out_message, int len) {
+    const char* msg = error_message((int)exception);
+    const int size = (int)std::strlen(msg);
+    if (len <= 0 || !out_message) return size + 1;
+    const int n = std::min(len - 1, size);
+    std::memcpy(out_message, msg, n);
+    out_message[n] = '\0';
+    return size + 1;
+}
+
+Vship_Exception Vship_PinnedMalloc(void** ptr, uint64_t size) {
+    if (!ptr) return Vship_BadPointer;
+    if (hipHostMalloc(ptr, size, hipHostMallocDefault) != hipSuccess) return Vship_OutOfRAM;
+    return Vship_NoError;
+}
+
+Vship_Exception Vship_PinnedFree(void* ptr) {
+    if (hipHostFree(ptr) != hipSuccess) return Vship_BadPointer;
+    return Vship_NoError;
+}
+
+Vship_Exception Vship_SSIMU2Init(Vship_SSIMU2Handler* handler, int width, int height) {
+    if (!handler) return Vship_BadPointer;
+    // the slot is taken even when initialisation fails, as in the reference
+    // (src/VshipLib.cpp:162-173); Vship_SSIMU2Free releases it
+    handler->id = frame_table().allocate();
+    if (width <= 0 || height <= 0) return Vship_OutOfVRAM;
+    return guarded([&] {
+        int dev = 0;
+        if (hipGetDevice(&dev) != hipSuccess) VSHIP_THROW(E_DeviceCountError);
+        frame_table().set(handler->id, std::make_shared<FrameHandle>(width, height, dev));
+    });
+}
+
+Vship_Exception Vship_SSIMU2Free(Vship_SSIMU2Handler handler) {
+    return frame_table().release(handler.id) ? Vship_NoError : Vship_BadHandler;
+}
+
+Vship_Exception Vship_ComputeSSIMU2Float(Vship_SSIMU2Handler handler, double* score, const uint8_t* srcp1[3],
+                                         const uint8_t* srcp2[3], int64_t stride) {
+    return compute_frame(handler, score, srcp1, srcp2, stride, stride, IN_RGB_F32);
+}
+Vship_Exception Vship_ComputeSSIMU2Floatv2(Vship_SSIMU2Handler handler, double* score, const uint8_t* srcp1[3],
+                                           const uint8_t* srcp2[3], int64_t stride, int64_t stride2) {
+    return compute_frame(handler, score, srcp1, srcp2, stride, stride2, IN_RGB_F32);
+}
+Vship_Exception Vship_ComputeSSIMU2Uint16(Vship_SSIMU2Handler handler, double* score, const uint8_t* srcp1[3],
+                                          const uint8_t* srcp2[3], int64_t stride) {
+    return compute_frame(handler, score, srcp1, srcp2, stride, stride, IN_RGB_U16);
+}
+Vship_Exception Vship_ComputeSSIMU2Uint16v2(Vship_SSIMU2Handler handler, double* score, const uint8_t* srcp1[3],
+                                            const uint8_t* srcp2[3], int64_t stride, int64_t stride2) {
+    return compute_frame(handler, score, srcp1, srcp2, stride, stride2, IN_RGB_U16);
+}
+
+// Butteraugli is outside this build (see VshipAPI.h).
+Vship_Exception Vship_ButteraugliInit(Vship_ButteraugliHandler*, int, int, float) { return Vship_BadErrorType; }
+Vship_Exception Vship_ButteraugliInitv2(Vship_ButteraugliHandler*, int, int, int, float) {
+    return Vship_BadErrorType;
+}
+Vship_Exception Vship_ButteraugliFree(Vship_ButteraugliHandler) { return Vship_BadHandler; }
+Vship_Exception Vship_ComputeButteraugliFloat(Vship_ButteraugliHandler, Vship_ButteraugliScore*, const uint8_t*,
+                                              int64_t, const uint8_t**, const uint8_t**, int64_t) {
+    return Vship_BadHandler;
+}
+Vship_Exception Vship_ComputeButteraugliFloatv2(Vship_ButteraugliHandler, Vship_ButteraugliScore*, const uint8_t*,
+                                                int64_t, const uint8_t**, const uint8_t**, int64_t, int64_t) {
+    return Vship_BadHandler;
+}
+Vship_Exception Vship_ComputeButteraugliUint16(Vship_ButteraugliHandler, Vship_ButteraugliScore*, const uint8_t*,
+                                               int64_t, const uint8_t**, const uint8_t**, int64_t) {
+    return Vship_BadHandler;
+}
+Vship_Exception Vship_ComputeButteraugliUint16v2(Vship_ButteraugliHandler, Vship_ButteraugliScore*, const uint8_t*,
+                                                 int64_t, const uint8_t**, const uint8_t**, int64_t, int64_t) {
+    return Vship_BadHandler;
+}
+
+// ---------------------------------------------------------------------------
+// batched entry points
+// ---------------------------------------------------------------------------
+uint64_t Vship_SSIMU2PairFootprint(int width, int height, int fp16_planes) {
+    if (width <= 0 || height <= 0) return 0;
+    return (uint64_t)pair_elems(width, height) * (fp16_planes ? 2 : 4);
+}
+
+Vship_Exception Vship_SSIMU2BatchInit(Vship_SSIMU2BatchHandler* handler, const Vship_SSIMU2BatchConfig* cfg) {
+    if (!handler || !cfg) return Vship_BadPointer;
+    if (cfg->max_pairs < 1 || cfg->max_width < 1 || cfg->max_height < 1) return Vship_BadPointer;
+    const int id = batch_table().allocate();
+    handler->id = id;
+    const Vship_Exception e = guarded([&] {
+        int dev = 0;
+        if (hipGetDevice(&dev) != hipSuccess) VSHIP_THROW(E_DeviceCountError);
+        EngineOptions o;
+        o.device = dev;
+        o.max_pairs = cfg->max_pairs;
+        o.fp16 = cfg->fp16_planes != 0;
+        o.map_th = cfg->map_tile_height == 32 || cfg->map_tile_height == 64
+                       ? cfg->map_tile_height
+                       : default_map_th(cfg->max_width, cfg->max_height);
+        o.arena_bytes = cfg->arena_bytes ? (int64_t)cfg->arena_bytes
+                                         : (int64_t)cfg->max_pairs *
+                                               (int64_t)Vship_SSIMU2PairFootprint(cfg->max_width, cfg->max_height,
+                                                                                  cfg->fp16_planes);
+        o.max_map_tiles = (int64_t)cfg->max_pairs * map_tiles(cfg->max_width, cfg->max_height, 32);
+        o.persistent_grid = cfg->persistent_grid;
+        o.use_graph = cfg->use_graph != 0;
+        o.profile = cfg->profile != 0;
+        auto b = std::make_shared<BatchHandle>();
+        b->engine = std::make_unique<Engine>(o);
+        batch_table().set(id, std::move(b));
+    });
+    if (e != Vship_NoError) batch_table().release(id);
+    return e;
+}
+
+Vship_Exception Vship_SSIMU2BatchFree(Vship_SSIMU2BatchHandler handler) {
+    return batch_table().release(handler.id) ? Vship_NoError : Vship_BadHandler;
+}
+
+Vship_Exception Vship_SSIMU2BatchSubmit(Vship_SSIMU2BatchHandler handler, const Vship_SSIMU2Pair* pairs,
+                                        int npairs) {
+    std::shared_ptr<BatchHandle> b = batch_table().get(handler.id);
+    if (!b) return Vship_BadHandler;
+    if (!pairs || npairs < 1) return Vship_BadPointer;
+    BusyGuard g(b->busy);
+    if (!g.ok) return Vship_BadHandler;
+    return guarded([&] {
+        std::vector<PairInput> in(npairs);
+        for (int i = 0; i < npairs; i++) in[i] = to_pair_input(pairs[i]);
+        b->engine->submit(in.data(), npairs);
+    });
+}
+
+Vship_Exception Vship_SSIMU2BatchWait(Vship_SSIMU2BatchHandler handler, double* scores, double* features) {
+    std::shared_ptr<BatchHandle> b = batch_table().get(handler.id);
+    if (!b) return Vship_BadHandler;
+    BusyGuard g(b->busy);
+    if (!g.ok) return Vship_BadHandler;
+    return guarded([&] { b->engine->wait(scores, features); });
+}
+
+Vship_Exception Vship_SSIMU2BatchCompute(Vship_SSIMU2BatchHandler handler, const Vship_SSIMU2Pair* pairs,
+                                         int npairs, double* scores, double* features) {
+    const Vship_Exception e = Vship_SSIMU2BatchSubmit(handler, pairs, npairs);
+    if (e != Vship_NoError) return e;
+    return Vship_SSIMU2BatchWait(handler, scores, features);
+}
+
+Vship_Exception Vship_SSIMU2BatchGetStats(Vship_SSIMU2BatchHandler handler, Vship_SSIMU2BatchStats* st) {
+    std::shared_ptr<BatchHandle> b = batch_table().get(handler.id);
+    if (!b) return Vship_BadHandler;
+    if (!st) return Vship_BadPointer;
+    const Engine& e = *b->engine;
+    st->prep_ms = e.timings().prep_ms;
+    st->map_ms = e.timings().map_ms;
+    st->map_fp32_ms = e.timings().map32_ms;
+    st->map_fp64_ms = e.timings().map64_ms;
+    st->finalize_ms = e.timings().finalize_ms;
+    st->total_ms = e.timings().total_ms;
+    st->npairs = e.last_npairs();
+    st->prep_tiles = e.last_prep_tiles();
+    st->map_tiles = e.last_map_tiles();
+    st->map_tiles_fp32 = e.last_map_tiles_class(0);
+    st->map_tiles_fp64 = e.last_map_tiles_class(1);
+    st->map_tile_height = e.options().map_th;
+    st->map_bytes = e.last_map_bytes();
+    st->map_bytes_fp32 = e.last_map_bytes_class(0);
+    st->prep_bytes = e.last_prep_bytes();
+    return Vship_NoError;
+}
+
+Vship_Exception Vship_DeviceMalloc(void** ptr, uint64_t size) {
+    if (!ptr) return Vship_BadPointer;
+    if (hipMalloc(ptr, size) != hipSuccess) return Vship_OutOfVRAM;
+    return Vship_NoError;
+}
+
+Vship_Exception Vship_DeviceFree(void* ptr) { return hipFree(ptr) == hipSuccess ? Vship_NoError : Vship_BadPointer; }
+
+Vship_Exception Vship_MemcpyHtoD(void* dst, const void* src, uint64_t size) {
+    return hipMemcpy(dst, src, size, hipMemcpyHostToDevice) == hipSuccess ? Vship_NoError : Vship_BadPointer;
+}
+
+Vship_Exception Vship_MemcpyDtoH(void* dst, const void* src, uint64_t size) {
+    return hipMemcpy(dst, src, size, hipMemcpyDeviceToHost) == hipSuccess ? Vship_NoError : Vship_BadPointer;
+}
+
+}  // extern "C"

@@ -59,7 +59,7 @@ typedef struct {
     int max_height;
     int fp16_planes;       /* 1: store the XYB pyramid in FP16 (FP32 math, FP64 sums) */
     uint64_t arena_bytes;  /* 0: max_pairs * Vship_SSIMU2PairFootprint(max_width, max_height, fp16_planes) */
-    int map_tile_height;   /* 0: tuned default; else 32 or 64 */
+    int map_tile_height;   /* 0: default (48); else 48 or 112 rows per map tile */
     int persistent_grid;   /* 0: one workgroup per tile; else fixed grid of this many workgroups */
     int use_graph;         /* 1: capture the batch launch into a hipGraph once and replay it */
     int profile;           /* 1: time each kernel with HIP events (eager launches only) */
@@ -111,6 +111,7 @@ EXPORTPREPROCESS Vship_Exception Vship_DeviceMalloc(void** ptr, uint64_t size);
 EXPORTPREPROCESS Vship_Exception Vship_DeviceFree(void* ptr);
 EXPORTPREPROCESS Vship_Exception Vship_MemcpyHtoD(void* dst, const void* src, uint64_t size);
 EXPORTPREPROCESS Vship_Exception Vship_MemcpyDtoH(void* dst, const void* src, uint64_t size);
+EXPORTPREPROCESS Vship_Exception Vship_MemcpyDtoD(void* dst, const void* src, uint64_t size);
 
 #ifdef __cplusplus
 } /* extern "C" */

@@ -134,6 +134,8 @@ def lib():
     L.Vship_MemcpyHtoD.argtypes = [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_uint64]
     L.Vship_MemcpyDtoH.restype = E
     L.Vship_MemcpyDtoH.argtypes = [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_uint64]
+    L.Vship_MemcpyDtoD.restype = E
+    L.Vship_MemcpyDtoD.argtypes = [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_uint64]
     _lib = L
     return L
 
@@ -201,6 +203,10 @@ class DeviceBuffer:
         out = np.empty(n, np.uint8)
         _check(lib().Vship_MemcpyDtoH(out.ctypes.data, self.ptr + offset, n), "Vship_MemcpyDtoH")
         return out
+
+    def copy_from(self, other, nbytes):
+        """Device-to-device copy of nbytes from another DeviceBuffer."""
+        _check(lib().Vship_MemcpyDtoD(self.ptr, other.ptr, int(nbytes)), "Vship_MemcpyDtoD")
 
     def free(self):
         if self.ptr:

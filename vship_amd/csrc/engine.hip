@@ -23,8 +23,8 @@ int input_bytes_per_sample(int fmt) {
 }
 
 // Normalised 17-tap Gaussian, sigma 1.5, radius 8 (src/ssimu2/gaussianblur.hpp:7-18,
-// src/util/preprocessor.hpp:89-92). Normalising to sum 1 does not change the
-// renormalised blur and makes interior positions need no division.
+// src/util/preprocessor.hpp:89-92), recomputed in double and checked against
+// the compile-time taps the kernels use.
 Taps make_taps() {
     double g[17], sum = 0.0;
     for (int i = 0; i < 17; i++) {
@@ -33,7 +33,10 @@ Taps make_taps() {
         sum += g[i];
     }
     Taps t;
-    for (int i = 0; i < 17; i++) t.g[i] = (float)(g[i] / sum);
+    for (int i = 0; i < 17; i++) {
+        t.g[i] = (float)(g[i] / sum);
+        if (t.g[i] != tap(i)) VSHIP_THROW(E_BadDeviceCode);
+    }
     return t;
 }
 
@@ -126,7 +129,7 @@ uint64_t prep_algorithmic_bytes(const PairInput& p, bool fp16) {
 
 Engine::Engine(const EngineOptions& opt) : opt_(opt) {
     if (opt_.max_pairs < 1) VSHIP_THROW(E_BadPointer);
-    if (opt_.map_th != 32 && opt_.map_th != 64) opt_.map_th = 64;
+    if (opt_.map_th != 48 && opt_.map_th != 64 && opt_.map_th != 112) opt_.map_th = 48;
     HIP_OR_THROW(hipSetDevice(opt_.device), E_BadDeviceArgument);
     HIP_OR_THROW(hipDeviceGetAttribute(&num_cus_, hipDeviceAttributeMultiprocessorCount, opt_.device),
                  E_BadDeviceCode);

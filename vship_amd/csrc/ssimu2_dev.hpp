@@ -55,9 +55,20 @@ struct BatchHdr {
 };
 
 constexpr int64_t kSmallSegment = 32768;
-constexpr int kSmallTH = 32;
+constexpr int kSmallTH = 48;
 
-// Tap tables of the renormalised 17-tap Gaussian, normalised to sum 1 on the host.
+// The 17-tap Gaussian (sigma 1.5, radius 8) of src/ssimu2/gaussianblur.hpp:7-18
+// and src/util/preprocessor.hpp:89-92, normalised to sum 1 and rounded to
+// float: kTap[i] = float(exp(-i^2 / 4.5) / sum_{k=-8..8} exp(-k^2 / 4.5)).
+// Normalising does not change the renormalised blur; it lets interior
+// positions skip the division. The host recomputes these at start-up and
+// refuses to run if they differ.
+constexpr float kTap[9] = {
+    0x1.105838p-2f, 0x1.b4272cp-3f, 0x1.bfdb5ap-4f, 0x1.26dcd8p-5f, 0x1.f1e5f2p-8f,
+    0x1.0d8866p-10f, 0x1.763768p-14f, 0x1.4d2166p-18f, 0x1.7c4a7ep-23f,
+};
+__host__ __device__ constexpr float tap(int k) { return kTap[k < 8 ? 8 - k : k - 8]; }  // k = 0..16
+
 struct Taps {
     float g[17];
 };

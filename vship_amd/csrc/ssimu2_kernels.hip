@@ -67,22 +67,6 @@ __device__ __forceinline__ void to_xyb(float r, float g, float b, float& X, floa
     Y = y + 0.01f;
 }
 
-// Multiply and add that the compiler may not fuse into an FMA: the map
-// kernel relies on separately rounded products for its exactness on
-// identical inputs.
-__device__ __forceinline__ float mul_nc(float a, float b) {
-#pragma clang fp contract(off)
-    return a * b;
-}
-__device__ __forceinline__ float add_nc(float a, float b) {
-#pragma clang fp contract(off)
-    return a + b;
-}
-__device__ __forceinline__ float sub_nc(float a, float b) {
-#pragma clang fp contract(off)
-    return a - b;
-}
-
 __device__ __forceinline__ float sample_u(const uint8_t* plane, int64_t stride, int x, int y, bool wide) {
     if (wide) return (float)reinterpret_cast<const uint16_t*>(plane + (int64_t)y * stride)[x];
     return (float)plane[(int64_t)y * stride + x];
@@ -92,19 +76,11 @@ __device__ __forceinline__ float sample_u(const uint8_t* plane, int64_t stride, 
 // prep kernel
 // ---------------------------------------------------------------------------
 struct PrepLds {
-    float lin0[3][64][65];
-    union {
-        struct {
-            float ch[2][36][37];  // staged chroma (clamped indices)
-            float hh[2][36][65];  // chroma upsampled horizontally to luma columns
-        } c;
-        struct {
-            float l1[3][32][33];
-            float l2[3][16][17];
-            float l3[3][8][9];
-            float l4[3][4][5];
-        } p;
-    } u;
+    float ch[2][36][37];  // staged chroma neighbourhood of the tile (clamped indices)
+    float l1[3][32][33];  // linear RGB of scales 1..4
+    float l2[3][16][17];
+    float l3[3][8][9];
+    float l4[3][4][5];
 };
 
 template <typename ST>
@@ -147,6 +123,20 @@ __device__ __forceinline__ void pyr_level(const float* par, float* chi, ST* aren
     }
 }
 
+// Reads sample x of row y of an input plane of the pair's format as float.
+__device__ __forceinline__ float rgb_sample(const uint8_t* plane, int64_t stride, int fmt, int x, int y) {
+    const uint8_t* row = plane + (int64_t)y * stride;
+    switch (fmt) {
+    case IN_RGB_U8: return (float)row[x] * (1.0f / 255.0f);
+    case IN_RGB_U16: return (float)reinterpret_cast<const uint16_t*>(row)[x] * (1.0f / 65535.0f);
+    case IN_RGB_F16: return __half2float(reinterpret_cast<const __half*>(row)[x]);
+    default: return reinterpret_cast<const float*>(row)[x];
+    }
+}
+
+// Each workgroup converts one 64x64 scale-0 tile of one image. A thread owns
+// 2x2 pixel quads, so scale 1 is formed in registers; scales 2..5 go through
+// LDS. Every XYB plane is written exactly once.
 template <typename ST>
 __global__ __launch_bounds__(256) void prep_kernel(const PairDev* __restrict__ pairs,
                                                    const int32_t* __restrict__ prep_start,
@@ -154,7 +144,7 @@ __global__ __launch_bounds__(256) void prep_kernel(const PairDev* __restrict__ p
     __shared__ PrepLds L;
     const int npairs = hdr->npairs;
     const int ntiles = hdr->prep_tiles;
-    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const int tid = threadIdx.x;
     for (int t = blockIdx.x; t < ntiles; t += gridDim.x) {
         int lo = 0, hi = npairs - 1;
         while (lo < hi) {
@@ -184,92 +174,111 @@ __global__ __launch_bounds__(256) void prep_kernel(const PairDev* __restrict__ p
                 const int pl = idx / 1296, rem = idx - pl * 1296;
                 const int r = rem / 36, c = rem - r * 36;
                 const int gy = min(max(cy0 + r, 0), chh - 1), gx = min(max(cx0 + c, 0), cw - 1);
-                L.u.c.ch[pl][r][c] = sample_u(src[pl + 1], stride[pl + 1], gx, gy, wide);
-            }
-            __syncthreads();
-            // horizontal Catmull-Rom, left-sited: even luma x = sample, odd = (-1, 9, 9, -1)/16
-            for (int idx = tid; idx < 2 * 36 * 64; idx += 256) {
-                const int pl = idx / 2304, rem = idx - pl * 2304;
-                const int r = rem >> 6, j = rem & 63;
-                const float* row = L.u.c.ch[pl][r];
-                float v;
-                if ((j & 1) == 0) {
-                    v = row[j / 2 + 1];
-                } else {
-                    const int k = (j - 1) / 2;
-                    v = (-row[k] + 9.0f * row[k + 1] + 9.0f * row[k + 2] - row[k + 3]) * (1.0f / 16.0f);
-                }
-                L.u.c.hh[pl][r][j] = v;
+                L.ch[pl][r][c] = sample_u(src[pl + 1], stride[pl + 1], gx, gy, wide);
             }
             __syncthreads();
         }
 
-        // scale 0: unpack, linearise, XYB, keep linear RGB in LDS
+        // limited-range YUV -> R'G'B' constants (uniform)
         const float sc = yuv ? (float)(1 << (P.bits - 8)) : 1.0f;
-        const float kr = P.kr, kb = P.kb, kg = 1.0f - P.kr - P.kb;
-        for (int j = 0; j < 16; j++) {
-            const int r = wave * 16 + j, c = lane;
-            const int x = x0 + c, y = y0 + r;
-            float rgb[3] = {0.f, 0.f, 0.f};
-            if (x < W && y < H) {
-                if (!yuv) {
+        const float ymul = 1.0f / (219.0f * sc), yadd = -16.0f / 219.0f;
+        const float cmul = 1.0f / (128.0f * 224.0f * sc), cadd = -128.0f / 224.0f;
+        const float kr = P.kr, kb = P.kb, ikg = 1.0f / (1.0f - P.kr - P.kb);
+        const float rcr = 2.0f * (1.0f - kr), bcb = 2.0f * (1.0f - kb);
+
+        // scale 0 (quads) and scale 1 (in registers)
+        const int nv1x = min(32, P.sw[1] - tx * 32), nv1y = min(32, P.sh[1] - ty * 32);
+#pragma unroll 1
+        for (int i = 0; i < 4; i++) {
+            const int q = tid + 256 * i;
+            const int qy = q >> 5, qx = q & 31;
+            const int xa = x0 + 2 * qx, ya = y0 + 2 * qy;
+            if (xa >= W || ya >= H) continue;  // no pixel of this quad is in the image
+            const bool hx = xa + 1 < W, hy = ya + 1 < H;
+            const int xb = hx ? xa + 1 : xa, yb = hy ? ya + 1 : ya;  // clamped second column/row
+            float rgb[4][3];  // quad pixels: (xa,ya), (xa,yb), (xb,ya), (xb,yb)
+            if (!yuv) {
 #pragma unroll
-                    for (int p = 0; p < 3; p++) {
-                        const uint8_t* row = src[p] + (int64_t)y * stride[p];
-                        float v;
-                        switch (fmt) {
-                        case IN_RGB_U8: v = (float)row[x] / 255.0f; break;
-                        case IN_RGB_U16: v = (float)reinterpret_cast<const uint16_t*>(row)[x] / 65535.0f; break;
-                        case IN_RGB_F16: v = __half2float(reinterpret_cast<const __half*>(row)[x]); break;
-                        default: v = reinterpret_cast<const float*>(row)[x]; break;
-                        }
-                        rgb[p] = v;
-                    }
-                } else {
-                    // vertical Catmull-Rom, centre-sited
-                    const bool odd = (r & 1) != 0;
-                    const int rb = odd ? (r - 1) / 2 + 1 : r / 2;
-                    const float w0 = odd ? -9.0f : -3.0f, w1 = odd ? 111.0f : 29.0f;
-                    const float w2 = odd ? 29.0f : 111.0f, w3 = odd ? -3.0f : -9.0f;
-                    float cbcr[2];
+                for (int p = 0; p < 3; p++) {
+                    rgb[0][p] = rgb_sample(src[p], stride[p], fmt, xa, ya);
+                    rgb[1][p] = rgb_sample(src[p], stride[p], fmt, xa, yb);
+                    rgb[2][p] = rgb_sample(src[p], stride[p], fmt, xb, ya);
+                    rgb[3][p] = rgb_sample(src[p], stride[p], fmt, xb, yb);
+                }
+            } else {
+                const float yv[4] = {sample_u(src[0], stride[0], xa, ya, wide), sample_u(src[0], stride[0], xa, yb, wide),
+                                     sample_u(src[0], stride[0], xb, ya, wide), sample_u(src[0], stride[0], xb, yb, wide)};
+                // chroma: local staged rows qy..qy+4, columns qx..qx+3
+                float cbcr[2][4];
 #pragma unroll
-                    for (int pl = 0; pl < 2; pl++) {
-                        const float v = w0 * L.u.c.hh[pl][rb][c] + w1 * L.u.c.hh[pl][rb + 1][c] +
-                                        w2 * L.u.c.hh[pl][rb + 2][c] + w3 * L.u.c.hh[pl][rb + 3][c];
-                        cbcr[pl] = (v * (1.0f / 128.0f) - 128.0f * sc) / (224.0f * sc);
+                for (int pl = 0; pl < 2; pl++) {
+                    float he[5], ho[5];  // horizontal: even luma column (left-sited sample), odd (Catmull-Rom at 0.5)
+#pragma unroll
+                    for (int r = 0; r < 5; r++) {
+                        const float* row = L.ch[pl][qy + r];
+                        he[r] = row[qx + 1];
+                        ho[r] = (9.0f * (row[qx + 1] + row[qx + 2]) - (row[qx] + row[qx + 3])) * (1.0f / 16.0f);
                     }
-                    const float Yv = (sample_u(src[0], stride[0], x, y, wide) - 16.0f * sc) / (219.0f * sc);
-                    const float R = Yv + 2.0f * (1.0f - kr) * cbcr[1];
-                    const float B = Yv + 2.0f * (1.0f - kb) * cbcr[0];
-                    const float G = (Yv - kr * R - kb * B) / kg;
-                    rgb[0] = R;
-                    rgb[1] = G;
-                    rgb[2] = B;
+                    // vertical, centre-sited: even luma row t = 0.75 from rows 0..3, odd row t = 0.25 from rows 1..4
+                    const float ve_e = -3.0f * he[0] + 29.0f * he[1] + 111.0f * he[2] - 9.0f * he[3];
+                    const float vo_e = -9.0f * he[1] + 111.0f * he[2] + 29.0f * he[3] - 3.0f * he[4];
+                    const float ve_o = -3.0f * ho[0] + 29.0f * ho[1] + 111.0f * ho[2] - 9.0f * ho[3];
+                    const float vo_o = -9.0f * ho[1] + 111.0f * ho[2] + 29.0f * ho[3] - 3.0f * ho[4];
+                    // (xa,ya): even col, even row; (xa,yb): even col, odd row; (xb,..): odd col.
+                    // A clamped column/row repeats the first one.
+                    cbcr[pl][0] = fmaf(ve_e, cmul, cadd);
+                    cbcr[pl][1] = fmaf(hy ? vo_e : ve_e, cmul, cadd);
+                    cbcr[pl][2] = fmaf(hx ? ve_o : ve_e, cmul, cadd);
+                    cbcr[pl][3] = fmaf(hx ? (hy ? vo_o : ve_o) : (hy ? vo_e : ve_e), cmul, cadd);
                 }
 #pragma unroll
-                for (int p = 0; p < 3; p++) rgb[p] = linearize(rgb[p]);
-                store_xyb(arena, P, img, 0, x, y, rgb[0], rgb[1], rgb[2]);
+                for (int k = 0; k < 4; k++) {
+                    const float Yn = fmaf(yv[k], ymul, yadd);
+                    const float R = fmaf(rcr, cbcr[1][k], Yn);
+                    const float B = fmaf(bcb, cbcr[0][k], Yn);
+                    const float G = (Yn - kr * R - kb * B) * ikg;
+                    rgb[k][0] = R;
+                    rgb[k][1] = G;
+                    rgb[k][2] = B;
+                }
             }
 #pragma unroll
-            for (int p = 0; p < 3; p++) L.lin0[p][r][c] = rgb[p];
+            for (int k = 0; k < 4; k++)
+#pragma unroll
+                for (int p = 0; p < 3; p++) rgb[k][p] = linearize(rgb[k][p]);
+            store_xyb(arena, P, img, 0, xa, ya, rgb[0][0], rgb[0][1], rgb[0][2]);
+            if (hy) store_xyb(arena, P, img, 0, xa, yb, rgb[1][0], rgb[1][1], rgb[1][2]);
+            if (hx) store_xyb(arena, P, img, 0, xb, ya, rgb[2][0], rgb[2][1], rgb[2][2]);
+            if (hx && hy) store_xyb(arena, P, img, 0, xb, yb, rgb[3][0], rgb[3][1], rgb[3][2]);
+            // scale-1 child (src/ssimu2/downsample.hpp:3-19 summation order)
+            if (qx < nv1x && qy < nv1y) {
+                float v[3];
+#pragma unroll
+                for (int p = 0; p < 3; p++) {
+                    float a = rgb[0][p];
+                    a += rgb[1][p];
+                    a += rgb[2][p];
+                    a += rgb[3][p];
+                    v[p] = a * 0.25f;
+                }
+                store_xyb(arena, P, img, 1, tx * 32 + qx, ty * 32 + qy, v[0], v[1], v[2]);
+#pragma unroll
+                for (int p = 0; p < 3; p++) L.l1[p][qy][qx] = v[p];
+            }
         }
         __syncthreads();
 
-        // scales 1..5 from LDS
-        const int nv0x = min(64, W - x0), nv0y = min(64, H - y0);
-        pyr_level<32, 65, 33>(&L.lin0[0][0][0], &L.u.p.l1[0][0][0], arena, P, img, 1, tx, ty, nv0x, nv0y);
-        __syncthreads();
-        const int nv1x = min(32, P.sw[1] - tx * 32), nv1y = min(32, P.sh[1] - ty * 32);
-        pyr_level<16, 33, 17>(&L.u.p.l1[0][0][0], &L.u.p.l2[0][0][0], arena, P, img, 2, tx, ty, nv1x, nv1y);
+        // scales 2..5 from LDS
+        pyr_level<16, 33, 17>(&L.l1[0][0][0], &L.l2[0][0][0], arena, P, img, 2, tx, ty, nv1x, nv1y);
         __syncthreads();
         const int nv2x = min(16, P.sw[2] - tx * 16), nv2y = min(16, P.sh[2] - ty * 16);
-        pyr_level<8, 17, 9>(&L.u.p.l2[0][0][0], &L.u.p.l3[0][0][0], arena, P, img, 3, tx, ty, nv2x, nv2y);
+        pyr_level<8, 17, 9>(&L.l2[0][0][0], &L.l3[0][0][0], arena, P, img, 3, tx, ty, nv2x, nv2y);
         __syncthreads();
         const int nv3x = min(8, P.sw[3] - tx * 8), nv3y = min(8, P.sh[3] - ty * 8);
-        pyr_level<4, 9, 5>(&L.u.p.l3[0][0][0], &L.u.p.l4[0][0][0], arena, P, img, 4, tx, ty, nv3x, nv3y);
+        pyr_level<4, 9, 5>(&L.l3[0][0][0], &L.l4[0][0][0], arena, P, img, 4, tx, ty, nv3x, nv3y);
         __syncthreads();
         const int nv4x = min(4, P.sw[4] - tx * 4), nv4y = min(4, P.sh[4] - ty * 4);
-        pyr_level<2, 5, 1>(&L.u.p.l4[0][0][0], (float*)nullptr, arena, P, img, 5, tx, ty, nv4x, nv4y);
+        pyr_level<2, 5, 1>(&L.l4[0][0][0], (float*)nullptr, arena, P, img, 5, tx, ty, nv4x, nv4y);
         __syncthreads();
     }
 }
@@ -294,14 +303,22 @@ __device__ __forceinline__ float4 Row4<__half>::load(const __half* p) {
     return make_float4(fa.x, fa.y, fb.x, fb.y);
 }
 
+#ifndef VSHIP_MAP_OUT
+#define VSHIP_MAP_OUT 8
+#endif
+
 template <int TH, typename AT>
 struct MapCfg {
     static constexpr int TW = kMapTW;
-    static constexpr int HR = TH + 16;  // rows of the row-pass result
-    static constexpr int R = TH / 8;    // output rows per thread in the column pass
-    // LDS row pitch (elements): the two half-waves of the column pass land on
-    // disjoint banks
-    static constexpr int LDW = sizeof(AT) == 4 ? ((R == 8) ? 36 : 40) : 34;
+    static constexpr int HR = TH + 16;              // rows of the row-pass result
+    static constexpr int OUT = VSHIP_MAP_OUT;       // row-pass outputs per thread
+    static constexpr int PASS = 256 / (TW / OUT);   // rows per row pass of the 256 threads
+    static constexpr int R = TH / 8;                // output rows per thread in the column pass
+    // LDS row pitch (elements); for FP32 the two half-waves of the column pass
+    // land on disjoint banks
+    static constexpr int LDW = sizeof(AT) == 4 ? (R == 6 ? 48 : (R == 8 ? 36 : 48)) : 34;
+    static_assert(TH % 8 == 0, "column pass: 8 groups of R rows");
+    static_assert(TW % OUT == 0 && OUT % 4 == 0, "row pass: whole float4 groups per thread");
 };
 
 // Arithmetic of the moment accumulation type: FP32 with the hardware
@@ -317,17 +334,13 @@ struct Arith<double> {
     static __device__ __forceinline__ double rcp(double x) { return 1.0 / x; }
 };
 
-__device__ __forceinline__ double mul_nc(double a, double b) {
-#pragma clang fp contract(off)
-    return a * b;
-}
-__device__ __forceinline__ double add_nc(double a, double b) {
-#pragma clang fp contract(off)
-    return a + b;
-}
-__device__ __forceinline__ double sub_nc(double a, double b) {
-#pragma clang fp contract(off)
-    return a - b;
+// Virtual workgroup index such that the workgroups of one XCD (hardware
+// workgroup b runs on XCD b % 8) hold one contiguous range [0, nwg).
+__device__ __forceinline__ int xcd_block(int b, int nwg) {
+    constexpr int kXcd = 8;
+    const int x = b % kXcd, j = b / kXcd;
+    const int q = nwg / kXcd, r = nwg % kXcd;
+    return x < r ? x * (q + 1) + j : r * (q + 1) + (x - r) * q + j;
 }
 
 // Sum of the in-bounds normalised taps at position p of an axis of length n,
@@ -340,16 +353,28 @@ __device__ __forceinline__ AT tap_total(const AT* cum, int p, int n) {
 }
 
 // One map tile class: AT is the accumulation type of the moment blur.
+//
+// Moment planes, per channel, of the centred samples a = x1 - cc, b = x2 - cc
+// (cc: one constant per tile; a renormalised blur commutes with the shift):
+//   mu1 = blur(a), mu2 = blur(b), E2 = blur((x1 - x2)^2), Q = blur(ab).
+// With md = mu1 - mu2, Vd = E2 - md^2 = var(x1 - x2) = s11 + s22 - 2 s12 and
+// num_s = 2 (Q - mu1 mu2) + C2, the reference's SSIM term
+//   d = 1 - (1 - md^2) num_s / (s11 + s22 + C2)      (src/ssimu2/score.hpp:141-148)
+// is, exactly in real arithmetic,
+//   d = (Vd + md^2 num_s) / (Vd + num_s).
+// The variance that matters is taken of the difference image, which keeps
+// FP32 cancellation small, and identical inputs give d = 0 exactly.
 template <int TH, typename ST, typename AT>
-__global__ __launch_bounds__(256) void map_kernel(const PairDev* __restrict__ pairs,
+__global__ __launch_bounds__(256, (sizeof(AT) == 4 && TH <= 64) ? 2 : 1) void map_kernel(const PairDev* __restrict__ pairs,
                                                   const int32_t* __restrict__ order_seg,
                                                   const int32_t* __restrict__ order_start,
                                                   const BatchHdr* __restrict__ hdr, const ST* __restrict__ arena,
                                                   double* __restrict__ partials, const Taps tp) {
     using C = MapCfg<TH, AT>;
-    constexpr int TW = C::TW, HR = C::HR, R = C::R, LDW = C::LDW;
+    constexpr int TW = C::TW, HR = C::HR, R = C::R, LDW = C::LDW, OUT = C::OUT, PASS = C::PASS;
+    constexpr int NIN = OUT + 16;   // samples a row-pass thread reads
     constexpr int cls = sizeof(AT) == 4 ? 0 : 1;
-    __shared__ AT hb[4][HR][LDW];   // row-pass results: a, b, a^2 + b^2, ab
+    __shared__ AT hb[4][HR][LDW];   // row-pass results: a, b, (x1 - x2)^2, ab
     __shared__ AT cen[2][TH][LDW];  // centred samples of the output pixels
     __shared__ double red[4][kFeat];
     __shared__ AT cum[18];
@@ -358,7 +383,7 @@ __global__ __launch_bounds__(256) void map_kernel(const PairDev* __restrict__ pa
         AT a = 0;
         cum[0] = 0;
         for (int k = 0; k < 17; k++) {
-            a += (AT)tp.g[k];
+            a += (AT)tap(k);
             cum[k + 1] = a;
         }
     }
@@ -367,11 +392,20 @@ __global__ __launch_bounds__(256) void map_kernel(const PairDev* __restrict__ pa
     const int seg_hi = seg_lo + hdr->nseg[cls] - 1;
     const int t_begin = cls == 0 ? 0 : hdr->tiles[0];
     const int t_end = t_begin + hdr->tiles[cls];
-    AT gr[17];
-#pragma unroll
-    for (int k = 0; k < 17; k++) gr[k] = (AT)tp.g[k];
+    // taps as compile-time constants (instruction literals, no registers)
+#define gr(k) ((AT)tap(k))
 
-    for (int t = t_begin + blockIdx.x; t < t_end; t += gridDim.x) {
+    // XCD-aware tile order: workgroups are dispatched round-robin over the 8
+    // XCDs, each with its own L2. Renumber them so that each XCD owns one
+    // contiguous range of virtual workgroups, and give each workgroup a
+    // contiguous range of tiles: neighbouring tiles, whose halos overlap, then
+    // run on the same XCD and share its L2.
+    const int nwg = gridDim.x;
+    const int vb = xcd_block(blockIdx.x, nwg);
+    const int ntl = t_end - t_begin;
+    const int t_lo = t_begin + (int)(((int64_t)vb * ntl) / nwg);
+    const int t_hi = t_begin + (int)(((int64_t)(vb + 1) * ntl) / nwg);
+    for (int t = t_lo; t < t_hi; t++) {
         int lo = seg_lo, hi = seg_hi;
         while (lo < hi) {
             const int mid = (lo + hi + 1) >> 1;
@@ -392,94 +426,94 @@ __global__ __launch_bounds__(256) void map_kernel(const PairDev* __restrict__ pa
         const int64_t pe = P.plane_elems[s];
         const bool xin = (x0 >= 8) && (x0 + TW + 8 <= W);
 
-        // column-pass coordinates: 32 columns x 8 groups of R rows
+        // column-pass coordinates: 32 columns x 8 groups of R rows, and the
+        // renormalisation of this thread's outputs (channel independent)
         const int vj = tid & 31, vg = tid >> 5;
         const int vx = x0 + vj;
         const int i0 = vg * R;
         const AT invh = Arith<AT>::rcp(tap_total(cum, vx, W));
+        AT inv[R];
+#pragma unroll
+        for (int m = 0; m < R; m++) inv[m] = invh * Arith<AT>::rcp(tap_total(cum, y0 + i0 + m, H));
 
-        // row-pass coordinates: 8 threads per row, 4 adjacent outputs each
-        const int hr0 = tid >> 3, j0 = (tid & 7) * 4;
+        // row-pass coordinates: 4 threads per row, OUT adjacent outputs each
+        constexpr int TPR = TW / OUT;  // threads per row
+        const int hr0 = tid / TPR, j0 = (tid % TPR) * OUT;
         const int xs = x0 + j0 - 8;
 
 #pragma unroll 1
         for (int c = 0; c < 3; c++) {
             const ST* p1 = base1 + c * pe;
             const ST* p2 = base2 + c * pe;
-            // per-tile centring constant: the moments are taken of (value - cc),
-            // exact for a renormalised blur, which keeps E[x^2] - E[x]^2 from
-            // cancelling away the FP32 precision
             const int cxp = min(x0 + TW / 2, W - 1), cyp = min(y0 + TH / 2, H - 1);
             const AT cc = (AT)ldf(p1 + (int64_t)cyp * pitch + cxp);
 
             // ---- row pass: rows y0-8 .. y0+TH+7, columns x0 .. x0+31
 #pragma unroll 1
-            for (int r = hr0; r < HR; r += 32) {
+            for (int r = hr0; r < HR; r += PASS) {
                 const int y = y0 - 8 + r;
-                AT ha[4] = {0, 0, 0, 0}, hbv[4] = {0, 0, 0, 0};
-                AT hp[4] = {0, 0, 0, 0}, hq[4] = {0, 0, 0, 0};
-                AT ca[4] = {0, 0, 0, 0}, cb[4] = {0, 0, 0, 0};
+                AT ha[OUT], hbv[OUT], he[OUT], hq[OUT];
+#pragma unroll
+                for (int o = 0; o < OUT; o++) ha[o] = hbv[o] = he[o] = hq[o] = 0;
+                float u1[NIN], u2[NIN];
+                bool in[NIN];
                 if (y >= 0 && y < H) {
                     const ST* r1 = p1 + (int64_t)y * pitch;
                     const ST* r2 = p2 + (int64_t)y * pitch;
-                    AT av[20], bv[20];
                     if (xin) {
 #pragma unroll
-                        for (int q = 0; q < 5; q++) {
+                        for (int q = 0; q < NIN / 4; q++) {
                             const float4 u = Row4<ST>::load(r1 + xs + 4 * q);
                             const float4 w = Row4<ST>::load(r2 + xs + 4 * q);
-                            av[4 * q + 0] = (AT)u.x - cc; av[4 * q + 1] = (AT)u.y - cc;
-                            av[4 * q + 2] = (AT)u.z - cc; av[4 * q + 3] = (AT)u.w - cc;
-                            bv[4 * q + 0] = (AT)w.x - cc; bv[4 * q + 1] = (AT)w.y - cc;
-                            bv[4 * q + 2] = (AT)w.z - cc; bv[4 * q + 3] = (AT)w.w - cc;
+                            u1[4 * q + 0] = u.x; u1[4 * q + 1] = u.y; u1[4 * q + 2] = u.z; u1[4 * q + 3] = u.w;
+                            u2[4 * q + 0] = w.x; u2[4 * q + 1] = w.y; u2[4 * q + 2] = w.z; u2[4 * q + 3] = w.w;
                         }
+#pragma unroll
+                        for (int k = 0; k < NIN; k++) in[k] = true;
                     } else {
 #pragma unroll
-                        for (int k = 0; k < 20; k++) {
+                        for (int k = 0; k < NIN; k++) {
                             const int x = xs + k;
-                            const bool in = (x >= 0) && (x < W);
+                            in[k] = (x >= 0) && (x < W);
                             const int xc = min(max(x, 0), W - 1);
-                            const AT u = (AT)ldf(r1 + xc), w = (AT)ldf(r2 + xc);
-                            av[k] = in ? u - cc : (AT)0;
-                            bv[k] = in ? w - cc : (AT)0;
+                            u1[k] = ldf(r1 + xc);
+                            u2[k] = ldf(r2 + xc);
                         }
                     }
 #pragma unroll
-                    for (int o = 0; o < 4; o++) {
-                        ca[o] = av[8 + o];
-                        cb[o] = bv[8 + o];
-                    }
+                    for (int k = 0; k < NIN; k++) {
+                        const AT a = in[k] ? (AT)u1[k] - cc : (AT)0;
+                        const AT b = in[k] ? (AT)u2[k] - cc : (AT)0;
+                        const AT dd = in[k] ? (AT)u1[k] - (AT)u2[k] : (AT)0;
+                        const AT ee = dd * dd, qq = a * b;
 #pragma unroll
-                    for (int k = 0; k < 20; k++) {
-                        const AT a = av[k], b = bv[k];
-                        // products rounded separately so that identical inputs give
-                        // a^2 + b^2 == 2ab bit for bit
-                        const AT pp = add_nc(mul_nc(a, a), mul_nc(b, b)), qq = mul_nc(a, b);
-#pragma unroll
-                        for (int o = 0; o < 4; o++) {
+                        for (int o = 0; o < OUT; o++) {
                             const int kk = k - o;
                             if (kk >= 0 && kk <= 16) {
-                                ha[o] = fma(gr[kk], a, ha[o]);
-                                hbv[o] = fma(gr[kk], b, hbv[o]);
-                                hp[o] = fma(gr[kk], pp, hp[o]);
-                                hq[o] = fma(gr[kk], qq, hq[o]);
+                                ha[o] = fma(gr(kk), a, ha[o]);
+                                hbv[o] = fma(gr(kk), b, hbv[o]);
+                                he[o] = fma(gr(kk), ee, he[o]);
+                                hq[o] = fma(gr(kk), qq, hq[o]);
                             }
                         }
                     }
+                    if (r >= 8 && r < TH + 8) {
+#pragma unroll
+                        for (int o = 0; o < OUT; o++) {
+                            cen[0][r - 8][j0 + o] = in[8 + o] ? (AT)u1[8 + o] - cc : (AT)0;
+                            cen[1][r - 8][j0 + o] = in[8 + o] ? (AT)u2[8 + o] - cc : (AT)0;
+                        }
+                    }
+                } else if (r >= 8 && r < TH + 8) {
+#pragma unroll
+                    for (int o = 0; o < OUT; o++) cen[0][r - 8][j0 + o] = cen[1][r - 8][j0 + o] = 0;
                 }
 #pragma unroll
-                for (int o = 0; o < 4; o++) {
+                for (int o = 0; o < OUT; o++) {
                     hb[0][r][j0 + o] = ha[o];
                     hb[1][r][j0 + o] = hbv[o];
-                    hb[2][r][j0 + o] = hp[o];
+                    hb[2][r][j0 + o] = he[o];
                     hb[3][r][j0 + o] = hq[o];
-                }
-                if (r >= 8 && r < TH + 8) {
-#pragma unroll
-                    for (int o = 0; o < 4; o++) {
-                        cen[0][r - 8][j0 + o] = ca[o];
-                        cen[1][r - 8][j0 + o] = cb[o];
-                    }
                 }
             }
             __syncthreads();
@@ -499,36 +533,28 @@ __global__ __launch_bounds__(256) void map_kernel(const PairDev* __restrict__ pa
                 for (int m = 0; m < R; m++) {
                     const int kk = k - m;
                     if (kk >= 0 && kk <= 16) {
-                        v0[m] = fma(gr[kk], h0, v0[m]);
-                        v1[m] = fma(gr[kk], h1, v1[m]);
-                        v2[m] = fma(gr[kk], h2, v2[m]);
-                        v3[m] = fma(gr[kk], h3, v3[m]);
+                        v0[m] = fma(gr(kk), h0, v0[m]);
+                        v1[m] = fma(gr(kk), h1, v1[m]);
+                        v2[m] = fma(gr(kk), h2, v2[m]);
+                        v3[m] = fma(gr(kk), h3, v3[m]);
                     }
                 }
                 if (k >= 16) {
                     const int m = k - 16;
                     const int y = y0 + i0 + m;
                     if (vx < W && y < H) {
-                        // Written so that identical inputs give exactly 0: the third
-                        // moment plane is then exactly twice the fourth (FMA chains
-                        // scale exactly by 2), so den_s == num_s bit for bit, and both
-                        // ratios are formed as (difference) * 1/denominator.
-                        const AT iv = invh * Arith<AT>::rcp(tap_total(cum, y, H));
+                        const AT iv = inv[m];
                         const AT mu1 = v0[m] * iv, mu2 = v1[m] * iv;
-                        const AT S = mul_nc((AT)0.5, mul_nc(v2[m], iv)), Q = mul_nc(v3[m], iv);
-                        const AT m11 = mul_nc(mu1, mu1), m22 = mul_nc(mu2, mu2);
-                        const AT m12 = mul_nc(mu1, mu2);
+                        const AT E2 = v2[m] * iv, Q = v3[m] * iv;
                         const AT md = mu1 - mu2;
-                        const AT num_m = fma(-md, md, (AT)1);
-                        const AT e1 = sub_nc(S, mul_nc((AT)0.5, add_nc(m11, m22)));  // (sigma11 + sigma22) / 2
-                        const AT e2 = sub_nc(Q, m12);                                 // sigma12
-                        const AT num_s = fma((AT)2, e2, (AT)0.0009);
-                        const AT den_s = fma((AT)2, e1, (AT)0.0009);
-                        AT d = fma(-num_m, num_s, den_s) * Arith<AT>::rcp(den_s);
+                        const AT md2 = md * md;
+                        const AT vd = E2 - md2;                          // var(x1 - x2)
+                        const AT num_s = fma((AT)2, Q - mu1 * mu2, (AT)0.0009);
+                        AT d = fma(md2, num_s, vd) * Arith<AT>::rcp(vd + num_s);
                         d = d > 0 ? d : (AT)0;
                         const AT a = cen[0][i0 + m][vj], b = cen[1][i0 + m][vj];
                         const AT fa = fabs(a - mu1) + (AT)1, fb = fabs(b - mu2) + (AT)1;
-                        const AT ev = (fb - fa) * Arith<AT>::rcp(fa);
+                        const AT ev = (fb - fa) * Arith<AT>::rcp(fa);   // src/ssimu2/score.hpp:150-154
                         const AT art = ev > 0 ? ev : (AT)0, det = ev < 0 ? -ev : (AT)0;
                         const AT d2 = d * d, a2 = art * art, t2 = det * det;
                         acc[0] += d;
@@ -558,6 +584,7 @@ __global__ __launch_bounds__(256) void map_kernel(const PairDev* __restrict__ pa
         }
         __syncthreads();
     }
+#undef gr
 }
 
 // ---------------------------------------------------------------------------
@@ -659,12 +686,15 @@ hipError_t launch_map(bool fp16, int cls, int th, int grid, const PairDev* pairs
     if (cls == 1) {
         if (fp16) launch_map_t<kSmallTH, __half, double>(grid, pairs, order_seg, order_start, hdr, arena, partials, taps, st);
         else launch_map_t<kSmallTH, float, double>(grid, pairs, order_seg, order_start, hdr, arena, partials, taps, st);
-    } else if (th == 32) {
-        if (fp16) launch_map_t<32, __half, float>(grid, pairs, order_seg, order_start, hdr, arena, partials, taps, st);
-        else launch_map_t<32, float, float>(grid, pairs, order_seg, order_start, hdr, arena, partials, taps, st);
-    } else {
+    } else if (th == 112) {
+        if (fp16) launch_map_t<112, __half, float>(grid, pairs, order_seg, order_start, hdr, arena, partials, taps, st);
+        else launch_map_t<112, float, float>(grid, pairs, order_seg, order_start, hdr, arena, partials, taps, st);
+    } else if (th == 64) {
         if (fp16) launch_map_t<64, __half, float>(grid, pairs, order_seg, order_start, hdr, arena, partials, taps, st);
         else launch_map_t<64, float, float>(grid, pairs, order_seg, order_start, hdr, arena, partials, taps, st);
+    } else {
+        if (fp16) launch_map_t<48, __half, float>(grid, pairs, order_seg, order_start, hdr, arena, partials, taps, st);
+        else launch_map_t<48, float, float>(grid, pairs, order_seg, order_start, hdr, arena, partials, taps, st);
     }
     return hipGetLastError();
 }

@@ -104,11 +104,9 @@ FrameHandle::~FrameHandle() {
     if (staging) (void)hipFree(staging);
 }
 
-// Segments below kSmallSegment pixels run in the FP64 class with their own
-// tile height, so the FP32 class only sees scales of at least 32768 pixels,
-// where 64-row tiles waste little; a size-independent default also keeps a
-// pair's result independent of the batch it is in.
-int default_map_th(int, int) { return 64; }
+// A size-independent default keeps a pair's result independent of the batch
+// it is in (the tile decomposition, and so the per-tile centring, is fixed).
+int default_map_th(int, int) { return 48; }
 
 double FrameHandle::compute(int fmt, const uint8_t* const p1[3], const uint8_t* const p2[3], int64_t s1, int64_t s2) {
     if (!p1 || !p2) VSHIP_THROW(E_BadPointer);
@@ -408,7 +406,7 @@ Vship_Exception Vship_SSIMU2BatchInit(Vship_SSIMU2BatchHandler* handler, const V
         o.device = dev;
         o.max_pairs = cfg->max_pairs;
         o.fp16 = cfg->fp16_planes != 0;
-        o.map_th = cfg->map_tile_height == 32 || cfg->map_tile_height == 64
+        o.map_th = cfg->map_tile_height == 48 || cfg->map_tile_height == 64 || cfg->map_tile_height == 112
                        ? cfg->map_tile_height
                        : default_map_th(cfg->max_width, cfg->max_height);
         o.arena_bytes = cfg->arena_bytes ? (int64_t)cfg->arena_bytes
@@ -497,6 +495,10 @@ Vship_Exception Vship_MemcpyHtoD(void* dst, const void* src, uint64_t size) {
 
 Vship_Exception Vship_MemcpyDtoH(void* dst, const void* src, uint64_t size) {
     return hipMemcpy(dst, src, size, hipMemcpyDeviceToHost) == hipSuccess ? Vship_NoError : Vship_BadPointer;
+}
+
+Vship_Exception Vship_MemcpyDtoD(void* dst, const void* src, uint64_t size) {
+    return hipMemcpy(dst, src, size, hipMemcpyDeviceToDevice) == hipSuccess ? Vship_NoError : Vship_BadPointer;
 }
 
 }  // extern "C"

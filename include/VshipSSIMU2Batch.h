@@ -106,6 +106,60 @@ typedef struct {
 EXPORTPREPROCESS Vship_Exception Vship_SSIMU2BatchGetStats(Vship_SSIMU2BatchHandler handler,
                                                            Vship_SSIMU2BatchStats* stats);
 
+/* ---- continuous-batching server ------------------------------------------
+ * Clips of device-resident frame pairs are submitted at any time and scored
+ * by a worker thread. Before each GPU step, waiting clips are admitted in
+ * arrival order while their per-pair device footprint
+ * (Vship_SSIMU2PairFootprint) fits the memory budget and a slot is free;
+ * the step then packs the next frames of every admitted clip. Two steps are
+ * in flight at a time on two streams, so planning overlaps execution. A
+ * clip's scores are the same whichever clips it shares steps with. */
+typedef struct {
+    int id;
+} Vship_SSIMU2ServerHandler;
+
+typedef struct {
+    uint64_t memory_budget;  /* device bytes for the XYB arenas of the two in-flight steps */
+    int max_pairs_per_step;  /* pairs packed into one GPU step */
+    int max_inflight_clips;  /* admitted clips at a time (0: max_pairs_per_step) */
+    int max_width;           /* largest frame a clip may have */
+    int max_height;
+    int fp16_planes;
+    int map_tile_height;     /* 0: default */
+    int persistent_grid;     /* 0: one workgroup per tile */
+    int use_graph;
+} Vship_SSIMU2ServerConfig;
+
+typedef struct {
+    int64_t pairs_done;
+    int64_t steps;
+    int64_t clips_done;
+    double host_ms;          /* host time spent planning and enqueueing steps */
+    int waiting_clips;
+    int active_clips;
+} Vship_SSIMU2ServerStats;
+
+EXPORTPREPROCESS Vship_Exception Vship_SSIMU2ServerInit(Vship_SSIMU2ServerHandler* handler,
+                                                        const Vship_SSIMU2ServerConfig* config);
+/* Drains the admitted and waiting work, then stops the worker. */
+EXPORTPREPROCESS Vship_Exception Vship_SSIMU2ServerFree(Vship_SSIMU2ServerHandler handler);
+/* frames: nframes pair descriptors (device planes, which must stay valid until
+ * the clip is finished). arrival_ms: the request's arrival time on
+ * Vship_SSIMU2ServerClockMs's clock, or 0 for "now"; latency is measured
+ * from it. */
+EXPORTPREPROCESS Vship_Exception Vship_SSIMU2ServerSubmitClip(Vship_SSIMU2ServerHandler handler,
+                                                              const Vship_SSIMU2Pair* frames, int nframes,
+                                                              double arrival_ms, int* clip_id);
+/* *done = 1 and scores[nframes] / latency written once the clip is finished. */
+EXPORTPREPROCESS Vship_Exception Vship_SSIMU2ServerPollClip(Vship_SSIMU2ServerHandler handler, int clip_id,
+                                                            int* done, double* scores, double* latency_ms);
+EXPORTPREPROCESS Vship_Exception Vship_SSIMU2ServerWaitClip(Vship_SSIMU2ServerHandler handler, int clip_id,
+                                                            double* scores, double* latency_ms);
+EXPORTPREPROCESS Vship_Exception Vship_SSIMU2ServerGetStats(Vship_SSIMU2ServerHandler handler,
+                                                            Vship_SSIMU2ServerStats* stats);
+/* Monotonic clock (ms) the server measures latency with. */
+EXPORTPREPROCESS double Vship_SSIMU2ServerClockMs(void);
+
 /* Device memory helpers so callers without a GPU framework can stage inputs. */
 EXPORTPREPROCESS Vship_Exception Vship_DeviceMalloc(void** ptr, uint64_t size);
 EXPORTPREPROCESS Vship_Exception Vship_DeviceFree(void* ptr);

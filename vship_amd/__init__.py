@@ -8,6 +8,7 @@ reference's API surface for this path:
   ``Vship_SSIMU2Free`` (the reference's src/VshipAPI.h:87-111);
 * :class:`SSIMU2Batch` — the additive batched entry points
   (``include/VshipSSIMU2Batch.h``) over device-resident frame pairs;
+* :class:`SSIMU2Server` — the continuous-batching request server;
 * :class:`DeviceBuffer` — device memory through the library's helpers.
 
 Nothing here falls back to a CPU implementation: every call goes through the
@@ -19,6 +20,7 @@ from .api import (  # noqa: F401
     PinnedBuffer,
     SSIMU2Batch,
     SSIMU2Handler,
+    SSIMU2Server,
     VshipError,
     VshipNotBuilt,
     device_count,

@@ -70,6 +70,18 @@ class BatchStats(ctypes.Structure):
                 ("map_bytes_fp32", ctypes.c_uint64), ("prep_bytes", ctypes.c_uint64)]
 
 
+class ServerConfig(ctypes.Structure):
+    _fields_ = [("memory_budget", ctypes.c_uint64), ("max_pairs_per_step", ctypes.c_int),
+                ("max_inflight_clips", ctypes.c_int), ("max_width", ctypes.c_int), ("max_height", ctypes.c_int),
+                ("fp16_planes", ctypes.c_int), ("map_tile_height", ctypes.c_int), ("persistent_grid", ctypes.c_int),
+                ("use_graph", ctypes.c_int)]
+
+
+class ServerStats(ctypes.Structure):
+    _fields_ = [("pairs_done", ctypes.c_int64), ("steps", ctypes.c_int64), ("clips_done", ctypes.c_int64),
+                ("host_ms", ctypes.c_double), ("waiting_clips", ctypes.c_int), ("active_clips", ctypes.c_int)]
+
+
 _lib = None
 P3 = ctypes.c_void_p * 3
 
@@ -136,6 +148,22 @@ def lib():
     L.Vship_MemcpyDtoH.argtypes = [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_uint64]
     L.Vship_MemcpyDtoD.restype = E
     L.Vship_MemcpyDtoD.argtypes = [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_uint64]
+    L.Vship_SSIMU2ServerInit.restype = E
+    L.Vship_SSIMU2ServerInit.argtypes = [ctypes.POINTER(Handler), ctypes.POINTER(ServerConfig)]
+    L.Vship_SSIMU2ServerFree.restype = E
+    L.Vship_SSIMU2ServerFree.argtypes = [Handler]
+    L.Vship_SSIMU2ServerSubmitClip.restype = E
+    L.Vship_SSIMU2ServerSubmitClip.argtypes = [Handler, ctypes.POINTER(Pair), ctypes.c_int, ctypes.c_double,
+                                              ctypes.POINTER(ctypes.c_int)]
+    L.Vship_SSIMU2ServerPollClip.restype = E
+    L.Vship_SSIMU2ServerPollClip.argtypes = [Handler, ctypes.c_int, ctypes.POINTER(ctypes.c_int), ctypes.c_void_p,
+                                            ctypes.POINTER(ctypes.c_double)]
+    L.Vship_SSIMU2ServerWaitClip.restype = E
+    L.Vship_SSIMU2ServerWaitClip.argtypes = [Handler, ctypes.c_int, ctypes.c_void_p, ctypes.POINTER(ctypes.c_double)]
+    L.Vship_SSIMU2ServerGetStats.restype = E
+    L.Vship_SSIMU2ServerGetStats.argtypes = [Handler, ctypes.POINTER(ServerStats)]
+    L.Vship_SSIMU2ServerClockMs.restype = ctypes.c_double
+    L.Vship_SSIMU2ServerClockMs.argtypes = []
     _lib = L
     return L
 
@@ -366,6 +394,70 @@ class SSIMU2Batch:
     def close(self):
         if self._h is not None:
             _check(lib().Vship_SSIMU2BatchFree(self._h), "Vship_SSIMU2BatchFree")
+            self._h = None
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
+
+class SSIMU2Server:
+    """Continuous-batching server over device-resident clips (Vship_SSIMU2Server*)."""
+
+    def __init__(self, memory_budget, max_pairs_per_step, max_width, max_height, max_inflight_clips=0,
+                 fp16=False, map_tile_height=0, persistent_grid=0, use_graph=False):
+        cfg = ServerConfig(memory_budget=int(memory_budget), max_pairs_per_step=max_pairs_per_step,
+                           max_inflight_clips=max_inflight_clips, max_width=max_width, max_height=max_height,
+                           fp16_planes=int(bool(fp16)), map_tile_height=map_tile_height,
+                           persistent_grid=persistent_grid, use_graph=int(bool(use_graph)))
+        self._h = Handler()
+        _check(lib().Vship_SSIMU2ServerInit(ctypes.byref(self._h), ctypes.byref(cfg)), "Vship_SSIMU2ServerInit")
+        self._n = {}
+
+    @staticmethod
+    def clock_ms():
+        return lib().Vship_SSIMU2ServerClockMs()
+
+    def submit(self, descs, n=None, arrival_ms=0.0):
+        """descs: ctypes array of Pair (device planes that stay valid until the clip finishes)."""
+        n = len(descs) if n is None else n
+        cid = ctypes.c_int()
+        _check(lib().Vship_SSIMU2ServerSubmitClip(self._h, descs, n, float(arrival_ms), ctypes.byref(cid)),
+               "Vship_SSIMU2ServerSubmitClip")
+        self._n[cid.value] = n
+        return cid.value
+
+    def poll(self, clip_id):
+        """(scores, latency_ms) once finished, else None."""
+        n = self._n[clip_id]
+        scores = np.zeros(n)
+        done = ctypes.c_int()
+        lat = ctypes.c_double()
+        _check(lib().Vship_SSIMU2ServerPollClip(self._h, clip_id, ctypes.byref(done), scores.ctypes.data,
+                                                ctypes.byref(lat)), "Vship_SSIMU2ServerPollClip")
+        if not done.value:
+            return None
+        del self._n[clip_id]
+        return scores, lat.value
+
+    def wait(self, clip_id):
+        n = self._n.pop(clip_id)
+        scores = np.zeros(n)
+        lat = ctypes.c_double()
+        _check(lib().Vship_SSIMU2ServerWaitClip(self._h, clip_id, scores.ctypes.data, ctypes.byref(lat)),
+               "Vship_SSIMU2ServerWaitClip")
+        return scores, lat.value
+
+    def stats(self):
+        st = ServerStats()
+        _check(lib().Vship_SSIMU2ServerGetStats(self._h, ctypes.byref(st)), "Vship_SSIMU2ServerGetStats")
+        return {k: getattr(st, k) for k, _ in ServerStats._fields_}
+
+    def close(self):
+        if self._h is not None:
+            _check(lib().Vship_SSIMU2ServerFree(self._h), "Vship_SSIMU2ServerFree")
             self._h = None
 
     def __enter__(self):

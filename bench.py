@@ -1,21 +1,31 @@
 """SSIMULACRA2 throughput on one MI355X.
 
-Default workload: 240 frame pairs of 1920x1080 YUV420P8 (BT.709, limited
-range), synthetic (seeded blob field translating 2 px/frame; distorted by
-8x8 DCT quantisation and a 3x3 box blur on half the blocks). A step is one
-pass of the hot path over the batch of 240 pairs already resident in HBM:
-convert + pyramid + XYB, blur/SSIM/edge maps, FP64 norms and scores.
+Workloads (--config), all synthetic (seeded blob field translating 2 px per
+frame; distorted by 8x8 DCT quantisation and a 3x3 box blur on half the
+blocks; vship_amd/synth/synth.c holds the generator):
 
-Prints one JSON line (rank 0). Besides throughput it reports the map
-kernel's roofline position (HIP events on the library's stream), the
-per-frame path, p50/p99 step latency, and the CPU baseline: the FP32 CPU
-restatement timed on the host cores on a bounded sample, plus |dscore| of
-the GPU against the FP64 oracle on a sample of the same pairs.
+  1080p   (default) 240 pairs 1920x1080 YUV420P8 BT.709 limited, FP32. A step
+          is one pass of the hot path over the 240 pairs already resident in
+          HBM: convert + pyramid + XYB, blur/SSIM/edge maps, FP64 scores.
+  rgb256  one 256x256 planar RGB8 full-range pair (the CPU-runnable case).
+  4k10    120 pairs 3840x2160 YUV420P10 BT.2020 limited, XYB pyramid stored
+          in FP16 (FP32 math, FP64 sums); per-frame |dscore| vs the FP64 oracle.
+  mixed   64 pairs per step, resolutions drawn by seed from 640x360..3840x2160,
+          YUV420P8; launch shape from the autotuner, graph-captured launch.
+  served  Poisson clip arrivals (200 clips/s), 8..64 pairs per clip at the
+          mixed resolutions, 8/10-bit 50/50, continuous-batching server under a
+          200 GB budget; frames/s and p50/p99 clip latency.
+
+Prints one JSON line. Besides throughput: the dominant kernel's roofline
+position (HIP events on the library's stream), the per-frame path, p50/p99
+latency, and the CPU baseline (the FP32 CPU restatement timed on the host
+cores on a bounded sample) with |dscore| against the FP64 oracle on a sample.
 """
 import argparse
 import json
 import os
 import sys
+import threading
 import time
 
 import numpy as np
@@ -24,12 +34,7 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0
-
-CONFIGS = {
-    # name: (w, h, bits, npairs, fp16)
-    "1080p": (1920, 1080, 8, 240, False),
-    "4k10": (3840, 2160, 10, 120, True),
-}
+CONFIGS = ["1080p", "rgb256", "4k10", "mixed", "served"]
 
 
 def parse():
@@ -37,145 +42,337 @@ def parse():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=3)
-    ap.add_argument("--config", default="1080p", choices=sorted(CONFIGS))
+    ap.add_argument("--config", default="1080p", choices=CONFIGS)
     ap.add_argument("--map-th", type=int, default=0)
     ap.add_argument("--grid", type=int, default=0)
     ap.add_argument("--cpu-seconds", type=float, default=12.0, help="budget of the CPU-baseline sample")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-per-frame", action="store_true", help="skip the one-pair-per-launch measurement")
+    ap.add_argument("--oracle-pairs", type=int, default=3)
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
                     help="first run device-to-device copies of this many MiB (HBM counter calibration)")
-    ap.add_argument("--oracle-pairs", type=int, default=3)
+    ap.add_argument("--arrival-seconds", type=float, default=2.0, help="served: length of the arrival window")
+    ap.add_argument("--rate", type=float, default=200.0, help="served: clip arrival rate (clips/s)")
     return ap.parse_args()
+
+
+def fmt_of(bits):
+    from vship_amd import api
+
+    if bits == 8:
+        return api.PF_YUV420_U8, api.MATRIX_BT709
+    return api.PF_YUV420_U16, api.MATRIX_BT2020NCL
+
+
+def video_pairs(w, h, bits, n, seed):
+    from vship_amd import synth
+
+    clip = synth.video_clip(w, h, bits, n, seed)
+    pairs = [clip.frame(f) for f in range(n)]
+    clip.close()
+    return pairs
+
+
+def copy_calibration(mib):
+    from vship_amd import api
+
+    n = mib << 20
+    src_b, dst_b = api.DeviceBuffer(n), api.DeviceBuffer(n)
+    for _ in range(3):
+        dst_b.copy_from(src_b, n)
+    print(f"# calibration: 3 device copies of {n} bytes", flush=True)
+    src_b.free()
+    dst_b.free()
+
+
+def timed_steps(batch, descs, n, steps, warmup):
+    scores = None
+    for _ in range(warmup):
+        scores = batch.compute(descs, n)
+    step_ms, k = [], {"prep": [], "map": [], "map64": [], "fin": []}
+    t_start = time.perf_counter()
+    for _ in range(steps):
+        t1 = time.perf_counter()
+        scores = batch.compute(descs, n)
+        step_ms.append((time.perf_counter() - t1) * 1e3)
+        st = batch.stats()
+        k["prep"].append(st["prep_ms"])
+        k["map"].append(st["map_fp32_ms"])
+        k["map64"].append(st["map_fp64_ms"])
+        k["fin"].append(st["finalize_ms"])
+    elapsed = time.perf_counter() - t_start
+    return scores, elapsed, step_ms, k, batch.stats()
+
+
+def per_frame_rate(descs, n, w, h, fp16):
+    """One pair per launch sequence, eager, same device inputs."""
+    from vship_amd import api
+
+    single = api.SSIMU2Batch(1, w, h, fp16=fp16)
+    nsingle = min(n, 48)
+    one = (api.Pair * 1)()
+    for i in range(3):
+        one[0] = descs[i % n]
+        single.compute(one, 1)
+    t1 = time.perf_counter()
+    for i in range(nsingle):
+        one[0] = descs[i]
+        single.compute(one, 1)
+    rate = nsingle / (time.perf_counter() - t1)
+    single.close()
+    return rate
+
+
+def roofline(st, map_ms, fp16):
+    achieved = st["map_bytes_fp32"] / (map_ms * 1e-3) / 1e9
+    elem = 2 if fp16 else 4
+    return {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+            "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
+            "kernel": f"map_kernel<{st['map_tile_height']},{'__half' if fp16 else 'float'},float>",
+            "algorithmic_bytes_per_launch": st["map_bytes_fp32"],
+            "note": f"2 images x 3 XYB planes x {elem} B read per pixel of every scale in the FP32 class; "
+                    "the kernel is FP32-VALU bound, not HBM bound"}
+
+
+def record(a, value, elapsed, steps, dtype, workload, extra_cfg):
+    return {
+        "metric": "SSIMU2 frame-pairs/s",
+        "value": round(value, 2),
+        "unit": "frame-pairs/s",
+        "n_gpus": 1,
+        "steps": steps,
+        "warmup": a.warmup,
+        "ms_per_step": round(elapsed * 1e3 / max(steps, 1), 3),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": dtype,
+        "data": "synthetic",
+        "config": dict({"workload": workload}, **extra_cfg),
+    }
+
+
+def run_batched(a, pairs, bits, fp16, workload, tune=False, graph=False):
+    """Times `a.steps` passes over the device-resident pairs; returns (line, scores)."""
+    from vship_amd import api
+
+    npairs = len(pairs)
+    w = max(p[0][0].shape[1] for p in pairs)
+    h = max(p[0][0].shape[0] for p in pairs)
+    if bits == 0:  # planar RGB8 full range
+        dp = api.DevicePairs(pairs, api.PF_RGB_U8, 8, 0)
+    else:
+        fmt, matrix = fmt_of(bits)
+        dp = api.DevicePairs(pairs, fmt, bits, matrix)
+    print(f"# {npairs} pairs up to {w}x{h}, {dp.nbytes / 1e9:.2f} GB of inputs on device", flush=True)
+    th, grid, tuned = a.map_th, a.grid, None
+    if tune:
+        tuned = api.autotune(dp.descs, npairs, fp16=fp16, repeats=3)
+        th, grid, graph = tuned["map_tile_height"], tuned["persistent_grid"], bool(tuned["use_graph"])
+        print("# autotune best: " + json.dumps({k: tuned[k] for k in
+                                                ("map_tile_height", "persistent_grid", "use_graph", "best_ms")}),
+              flush=True)
+    # eager handle with per-kernel HIP events
+    batch = api.SSIMU2Batch(npairs, w, h, fp16=fp16, map_tile_height=th, persistent_grid=grid, profile=True)
+    scores, elapsed, step_ms, k, st = timed_steps(batch, dp.descs, npairs, a.steps, a.warmup)
+    value, graph_used = npairs * a.steps / elapsed, False
+    out_extra = {}
+    if graph:
+        # the same shape captured once in a hipGraph and replayed each step
+        gb = api.SSIMU2Batch(npairs, w, h, fp16=fp16, map_tile_height=th, persistent_grid=grid, use_graph=True)
+        gscores, gel, gstep, _, _ = timed_steps(gb, dp.descs, npairs, a.steps, a.warmup)
+        gb.close()
+        if not np.array_equal(gscores, scores):
+            raise RuntimeError("graph replay scores differ from eager launches")
+        gval = npairs * a.steps / gel
+        out_extra["eager_value"] = round(value, 2)
+        out_extra["graph_value"] = round(gval, 2)
+        value, elapsed, step_ms, graph_used = gval, gel, gstep, True
+    map_ms = float(np.mean(k["map"]))
+    out = record(a, value, elapsed, a.steps, "f16-storage/f32" if fp16 else "f32", workload,
+                 {"pairs_per_step": npairs, "map_tile_height": st["map_tile_height"],
+                  "persistent_grid": grid, "hip_graph": graph_used})
+    out.update(out_extra)
+    out["latency_ms"] = {"p50": round(float(np.percentile(step_ms, 50)), 3),
+                         "p99": round(float(np.percentile(step_ms, 99)), 3), "per": f"step of {npairs} pairs"}
+    out["kernels_ms"] = {"prep": round(float(np.mean(k["prep"])), 3), "map_fp32": round(map_ms, 3),
+                         "map_fp64_small_segments": round(float(np.mean(k["map64"])), 3),
+                         "finalize": round(float(np.mean(k["fin"])), 3)}
+    if tuned is not None:
+        out["autotune"] = tuned
+    if not a.no_per_frame:
+        pf = per_frame_rate(dp.descs, npairs, w, h, fp16)
+        out["per_frame_path"] = {"value": round(pf, 2), "unit": "frame-pairs/s",
+                                 "note": "one pair per launch sequence, eager, same device inputs",
+                                 "batched_over_per_frame": round(value / pf, 2)}
+    out["roofline"] = roofline(st, map_ms, fp16)
+    batch.close()
+    dp.free()
+    return out, scores
+
+
+def run_served(a):
+    """Poisson clip stream through the continuous-batching server.
+    Returns (line, host pairs of the frame pools, their scores alone)."""
+    from vship_amd import api, synth
+
+    rng = np.random.default_rng(4)
+    t, arrivals = 0.0, []
+    while True:
+        t += rng.exponential(1.0 / a.rate)
+        if t > a.arrival_seconds:
+            break
+        w, h = synth.RESOLUTIONS[int(rng.integers(0, len(synth.RESOLUTIONS)))]
+        bits = 8 if rng.random() < 0.5 else 10
+        arrivals.append((t, (w, h, bits), int(rng.integers(8, 65))))
+    # device-resident pools of 8 distinct pairs per (resolution, depth); a clip
+    # of n pairs cycles through its pool
+    pools, host_pool = {}, {}
+    for (w, h) in synth.RESOLUTIONS:
+        for bits in (8, 10):
+            fmt, matrix = fmt_of(bits)
+            pp = video_pairs(w, h, bits, 8, seed=w * 7 + bits)
+            host_pool[(w, h, bits)] = pp
+            pools[(w, h, bits)] = api.DevicePairs(pp, fmt, bits, matrix)
+    clips = []
+    for (t, key, n) in arrivals:
+        pool = pools[key]
+        d = (api.Pair * n)()
+        for i in range(n):
+            d[i] = pool.descs[i % pool.n]
+        clips.append((t, key, d, n))
+    total_pairs = sum(c[3] for c in clips)
+    print(f"# {len(clips)} clips, {total_pairs} pairs over {a.arrival_seconds:.1f}s of arrivals", flush=True)
+    budget = 200 * 10 ** 9
+    srv = api.SSIMU2Server(budget, 64, 3840, 2160, max_inflight_clips=64)
+    # warm-up clip (first launches, code-object load) before the clock starts
+    srv.wait(srv.submit(clips[0][2], clips[0][3]))
+    ids = [None] * len(clips)
+    t0 = srv.clock_ms()
+
+    def feeder():
+        for k, (t, _, d, n) in enumerate(clips):
+            target = t0 + t * 1e3
+            now = srv.clock_ms()
+            if target > now:
+                time.sleep((target - now) / 1e3)
+            ids[k] = srv.submit(d, n, arrival_ms=target)
+
+    th = threading.Thread(target=feeder)
+    th.start()
+    th.join()
+    lat, results = [], []
+    for cid in ids:
+        s, l = srv.wait(cid)
+        lat.append(l)
+        results.append(s)
+    t_end = srv.clock_ms()
+    st = srv.stats()
+    srv.close()
+    elapsed = (t_end - t0) / 1e3
+    value = total_pairs / elapsed
+    # each clip's scores equal its pool's pairs scored alone
+    alone = {}
+    for key, pool in pools.items():
+        with api.SSIMU2Batch(pool.n, key[0], key[1]) as b:
+            alone[key] = b.compute(pool.descs, pool.n)
+    same = sum(int(np.array_equal(np.array([alone[key][i % 8] for i in range(n)]), results[k]))
+               for k, (_, key, _, n) in enumerate(clips))
+    out = record(a, value, elapsed, 1, "f32",
+                 "served ssimulacra2 clips: Poisson arrivals, 8-64 pairs per clip, 640x360..3840x2160 "
+                 "YUV420 8/10-bit, inputs resident in HBM",
+                 {"rate_clips_per_s": a.rate, "arrival_seconds": a.arrival_seconds, "clips": len(clips),
+                  "pairs": total_pairs, "memory_budget_bytes": budget, "max_pairs_per_step": 64})
+    out["steps"] = int(st["steps"])
+    out["ms_per_step"] = round(elapsed * 1e3 / max(1, int(st["steps"])), 3)
+    out["latency_ms"] = {"p50": round(float(np.percentile(lat, 50)), 2),
+                         "p99": round(float(np.percentile(lat, 99)), 2), "per": "clip, arrival to last score"}
+    out["server"] = {k: st[k] for k in ("pairs_done", "steps", "clips_done")}
+    out["clips_equal_alone"] = f"{same}/{len(clips)}"
+    for p in pools.values():
+        p.free()
+    return out, host_pool, alone
+
+
+def cpu_baseline(pairs, bits, gpu_scores, a):
+    """FP32 CPU restatement timed on the host cores (bounded sample) and the FP64
+    oracle on a few pairs for |dscore|. bits: 0 for RGB8, else per-pair list or int."""
+    from tests import oracle_lib as ol
+
+    threads = min(os.cpu_count() or 1, 16)
+    ol.set_threads(threads)
+    bl = bits if isinstance(bits, list) else [bits] * len(pairs)
+
+    def score(i, prec):
+        r, d = pairs[i]
+        if bl[i] == 0:
+            return ol.score_rgb(ol.SO_RGB_U8, r, d, prec)[0]
+        m = ol.SO_MAT_BT709 if bl[i] == 8 else ol.SO_MAT_BT2020NCL
+        return ol.score_yuv420(r, d, bl[i], m, prec)[0]
+
+    n, t0 = 0, time.perf_counter()
+    while n < len(pairs) and time.perf_counter() - t0 < a.cpu_seconds:
+        score(n, "f32")
+        n += 1
+    cpu_rate = n / (time.perf_counter() - t0)
+    idx = sorted(set(np.linspace(0, len(pairs) - 1, a.oracle_pairs).astype(int).tolist()))
+    diffs = [abs(float(gpu_scores[i]) - score(i, "f64")) for i in idx]
+    base = {"value": round(cpu_rate, 4), "unit": "frame-pairs/s", "cores": threads, "kind": "port",
+            "sample": f"first {n} pairs of the workload, FP32 CPU restatement (OpenMP over rows), "
+                      f"{a.cpu_seconds:.0f}s budget"}
+    delta = {"pairs": idx, "abs": [float(x) for x in diffs], "max_abs": float(np.max(diffs)),
+             "mean_abs": float(np.mean(diffs))}
+    return base, delta
 
 
 def main():
     a = parse()
     if a.gpus != 1:
-        print(json.dumps({"error": "this build measures one GPU; every box here has one GPU, "
-                                   "and the path shards by frame pair with no inter-GPU traffic"}))
+        print(json.dumps({"error": "this build measures one GPU: every box here has one GPU, and the path "
+                                   "shards by frame pair with no inter-GPU traffic"}))
         sys.exit(2)
     import vship_amd
-    from vship_amd import api, synth
+    from vship_amd import synth
 
-    w, h, bits, npairs, fp16 = CONFIGS[a.config]
-    fmt = api.PF_YUV420_U8 if bits == 8 else api.PF_YUV420_U16
-    matrix = api.MATRIX_BT709 if bits == 8 else api.MATRIX_BT2020NCL
-    t0 = time.time()
-    clip = synth.video_clip(w, h, bits, npairs, seed=1 if bits == 8 else 2)
-    pairs = [clip.frame(f) for f in range(npairs)]
-    t_gen = time.time() - t0
     vship_amd.gpu_full_check(0)
     if a.copy_calibration:
-        n = a.copy_calibration << 20
-        src_b, dst_b = api.DeviceBuffer(n), api.DeviceBuffer(n)
-        for _ in range(3):
-            dst_b.copy_from(src_b, n)
-        print(f"# calibration: 3 device copies of {n} bytes", flush=True)
-        src_b.free()
-        dst_b.free()
-    dp = api.DevicePairs(pairs, fmt, bits, matrix)
-    print(f"# generated {npairs} pairs {w}x{h} in {t_gen:.1f}s, {dp.nbytes / 1e9:.2f} GB on device", flush=True)
-
-    batch = api.SSIMU2Batch(npairs, w, h, fp16=fp16, map_tile_height=a.map_th, persistent_grid=a.grid,
-                            profile=True)
-    for _ in range(a.warmup):
-        scores = batch.compute(dp.descs)
-    step_ms, map_ms, map64_ms, prep_ms, fin_ms = [], [], [], [], []
-    t_start = time.perf_counter()
-    for _ in range(a.steps):
-        t1 = time.perf_counter()
-        scores = batch.compute(dp.descs)
-        step_ms.append((time.perf_counter() - t1) * 1e3)
-        st = batch.stats()
-        map_ms.append(st["map_fp32_ms"])
-        prep_ms.append(st["prep_ms"])
-        map64_ms.append(st["map_fp64_ms"])
-        fin_ms.append(st["finalize_ms"])
-    elapsed = time.perf_counter() - t_start
-    st = batch.stats()
-    value = npairs * a.steps / elapsed
-    map_avg = float(np.mean(map_ms))
-    achieved = st["map_bytes_fp32"] / (map_avg * 1e-3) / 1e9
-
-    # per-frame path: one pair per launch sequence, no batching, same device inputs
-    per_frame = None
-    if not a.no_per_frame:
-        single = api.SSIMU2Batch(1, w, h, fp16=fp16)
-        nsingle = min(npairs, 48)
-        for i in range(3):
-            single.compute(dp.descs, 1)
-        t1 = time.perf_counter()
-        one = (api.Pair * 1)()
-        for i in range(nsingle):
-            one[0] = dp.descs[i]
-            single.compute(one, 1)
-        per_frame = nsingle / (time.perf_counter() - t1)
-        single.close()
-
-    out = {
-        "metric": "SSIMU2 frame-pairs/s",
-        "value": round(value, 2),
-        "unit": "frame-pairs/s",
-        "n_gpus": 1,
-        "steps": a.steps,
-        "warmup": a.warmup,
-        "ms_per_step": round(elapsed * 1e3 / a.steps, 3),
-        "higher_is_better": True,
-        "scaling": "weak",
-        "vs_baseline": None,
-        "dtype": "f32" if not fp16 else "f16-storage/f32",
-        "data": "synthetic",
-        "config": {"workload": f"ssimulacra2 {w}x{h} YUV420P{bits} {'BT.709' if bits == 8 else 'BT.2020'} "
-                               f"limited, {npairs} pairs per step, inputs resident in HBM",
-                   "pairs_per_step": npairs, "map_tile_height": st["map_tile_height"]},
-        "latency_ms": {"p50": round(float(np.percentile(step_ms, 50)), 3),
-                       "p99": round(float(np.percentile(step_ms, 99)), 3),
-                       "per": f"step of {npairs} pairs"},
-        "kernels_ms": {"prep": round(float(np.mean(prep_ms)), 3), "map_fp32": round(map_avg, 3),
-                       "map_fp64_small_segments": round(float(np.mean(map64_ms)), 3),
-                       "finalize": round(float(np.mean(fin_ms)), 3)},
-        "per_frame_path": {"value": round(per_frame, 2) if per_frame else None, "unit": "frame-pairs/s",
-                           "note": "one pair per launch sequence, eager, same device inputs"},
-        "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
-                     "kernel": f"map_kernel<{st['map_tile_height']},float,float>", "algorithmic_bytes_per_launch": st["map_bytes_fp32"],
-                     "note": "map kernel is FP32-VALU bound; algorithmic bytes = 2 images x 3 planes x "
-                             f"{2 if fp16 else 4} B x sum of scale pixels"},
-    }
-
+        copy_calibration(a.copy_calibration)
+    t0 = time.time()
+    if a.config == "1080p":
+        pairs, bits = video_pairs(1920, 1080, 8, 240, seed=1), 8
+        print(f"# generated in {time.time() - t0:.1f}s", flush=True)
+        out, scores = run_batched(a, pairs, 8, False, "ssimulacra2 1920x1080 YUV420P8 BT.709 limited, "
+                                  "240 pairs per step, inputs resident in HBM")
+    elif a.config == "rgb256":
+        pairs, bits = [synth.config0()], 0
+        out, scores = run_batched(a, pairs, 0, False, "ssimulacra2 256x256 planar RGB8 full range, 1 pair per step")
+    elif a.config == "4k10":
+        pairs, bits = video_pairs(3840, 2160, 10, 120, seed=2), 10
+        print(f"# generated in {time.time() - t0:.1f}s", flush=True)
+        out, scores = run_batched(a, pairs, 10, True, "ssimulacra2 3840x2160 YUV420P10 BT.2020 limited, FP16 "
+                                  "pyramid planes, 120 pairs per step, inputs resident in HBM")
+    elif a.config == "mixed":
+        res = synth.batch_resolutions(64, seed=3)
+        clips, pairs = {}, []
+        for i, (w, h) in enumerate(res):
+            if (w, h) not in clips:
+                clips[(w, h)] = synth.video_clip(w, h, 8, 64, seed=w + h)
+            pairs.append(clips[(w, h)].frame(i))
+        bits = 8
+        print(f"# generated in {time.time() - t0:.1f}s", flush=True)
+        out, scores = run_batched(a, pairs, 8, False, "ssimulacra2 uneven batch of 64 pairs, 640x360..3840x2160 "
+                                  "YUV420P8 BT.709 limited, inputs resident in HBM", tune=True, graph=True)
+        out["config"]["resolutions"] = {f"{w}x{h}": res.count((w, h)) for (w, h) in sorted(set(res))}
+    else:
+        out, host_pool, alone = run_served(a)
+        pairs, scores, bits = [], [], []
+        for key, pp in host_pool.items():
+            pairs += pp
+            scores += list(alone[key])
+            bits += [key[2]] * len(pp)
     if not a.no_cpu:
-        out["cpu_baseline"], out["score_vs_fp64_oracle"] = cpu_baseline(pairs, scores, bits, a)
+        out["cpu_baseline"], out["score_vs_fp64_oracle"] = cpu_baseline(pairs, bits, scores, a)
     print(json.dumps(out), flush=True)
-    batch.close()
-    dp.free()
-
-
-def cpu_baseline(pairs, gpu_scores, bits, a):
-    """FP32 CPU restatement timed on the host cores (bounded sample) and the FP64
-    oracle on a few pairs for |dscore|."""
-    from tests import oracle_lib as ol
-
-    threads = min(os.cpu_count() or 1, 16)
-    ol.set_threads(threads)
-    matrix = ol.SO_MAT_BT709 if bits == 8 else ol.SO_MAT_BT2020NCL
-    n, t0 = 0, time.perf_counter()
-    while n < len(pairs) and time.perf_counter() - t0 < a.cpu_seconds:
-        r, d = pairs[n]
-        ol.score_yuv420(r, d, bits, matrix, "f32")
-        n += 1
-    cpu_rate = n / (time.perf_counter() - t0)
-    idx = np.linspace(0, len(pairs) - 1, a.oracle_pairs).astype(int)
-    diffs = []
-    for i in idx:
-        r, d = pairs[i]
-        s64, _ = ol.score_yuv420(r, d, bits, matrix, "f64")
-        diffs.append(abs(float(gpu_scores[i]) - s64))
-    base = {"value": round(cpu_rate, 4), "unit": "frame-pairs/s", "cores": threads, "kind": "port",
-            "sample": f"first {n} pairs of the workload, FP32 CPU restatement (OpenMP over rows), "
-                      f"{a.cpu_seconds:.0f}s budget"}
-    delta = {"pairs": [int(i) for i in idx], "max_abs": float(np.max(diffs)), "mean_abs": float(np.mean(diffs))}
-    return base, delta
 
 
 if __name__ == "__main__":

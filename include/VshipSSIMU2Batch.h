@@ -59,8 +59,8 @@ typedef struct {
     int max_height;
     int fp16_planes;       /* 1: store the XYB pyramid in FP16 (FP32 math, FP64 sums) */
     uint64_t arena_bytes;  /* 0: max_pairs * Vship_SSIMU2PairFootprint(max_width, max_height, fp16_planes) */
-    int map_tile_height;   /* 0: default (48); else 48 or 112 rows per map tile */
-    int persistent_grid;   /* 0: one workgroup per tile; else fixed grid of this many workgroups */
+    int map_tile_height;   /* 0: tuned shape if recorded, else 48; or 48, 64, 112 rows per map tile */
+    int persistent_grid;   /* 0: tuned shape if recorded, else one workgroup per tile; or a fixed grid */
     int use_graph;         /* 1: capture the batch launch into a hipGraph once and replay it */
     int profile;           /* 1: time each kernel with HIP events (eager launches only) */
 } Vship_SSIMU2BatchConfig;
@@ -101,6 +101,35 @@ typedef struct {
     uint64_t map_bytes_fp32; /* the part read by the FP32 map kernel */
     uint64_t prep_bytes;     /* algorithmic bytes the prep kernel reads and writes */
 } Vship_SSIMU2BatchStats;
+
+/* ---- autotuner -------------------------------------------------------------
+ * Times the batch on the sample for every map tile height (48, 64, 112),
+ * persistent grid (one workgroup per tile, 2/4/8 x CU count) and graph
+ * capture, returns the table and the fastest shape, and records it for the
+ * sample's frame-size class: Vship_SSIMU2BatchInit then uses it when
+ * map_tile_height and persistent_grid are 0. */
+#define VSHIP_TUNE_MAX_CANDIDATES 24
+typedef struct {
+    int map_tile_height;
+    int persistent_grid;
+    int use_graph;
+    float ms;
+} Vship_SSIMU2TuneCandidate;
+
+typedef struct {
+    int map_tile_height; /* best shape */
+    int persistent_grid;
+    int use_graph;
+    float best_ms;       /* wall time of one batch with the best shape */
+    int ncandidates;
+    Vship_SSIMU2TuneCandidate candidates[VSHIP_TUNE_MAX_CANDIDATES];
+} Vship_SSIMU2TuneResult;
+
+EXPORTPREPROCESS Vship_Exception Vship_SSIMU2BatchTune(const Vship_SSIMU2Pair* pairs, int npairs, int fp16_planes,
+                                                       int repeats, Vship_SSIMU2TuneResult* result);
+/* Text table of the recorded shapes (one line per size class and precision). */
+EXPORTPREPROCESS Vship_Exception Vship_SSIMU2TuneSave(const char* path);
+EXPORTPREPROCESS Vship_Exception Vship_SSIMU2TuneLoad(const char* path);
 
 /* Per-kernel HIP-event timings of the last waited batch (profile = 1). */
 EXPORTPREPROCESS Vship_Exception Vship_SSIMU2BatchGetStats(Vship_SSIMU2BatchHandler handler,

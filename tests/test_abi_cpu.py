@@ -30,7 +30,7 @@ def exported_symbols():
 
 def test_library_exports_every_declared_symbol():
     decl = declared_symbols()
-    assert len(decl) >= 39
+    assert len(decl) >= 42
     missing = [d for d in decl if d not in exported_symbols()]
     assert not missing, missing
 

@@ -70,6 +70,16 @@ class BatchStats(ctypes.Structure):
                 ("map_bytes_fp32", ctypes.c_uint64), ("prep_bytes", ctypes.c_uint64)]
 
 
+class TuneCandidate(ctypes.Structure):
+    _fields_ = [("map_tile_height", ctypes.c_int), ("persistent_grid", ctypes.c_int), ("use_graph", ctypes.c_int),
+                ("ms", ctypes.c_float)]
+
+
+class TuneResult(ctypes.Structure):
+    _fields_ = [("map_tile_height", ctypes.c_int), ("persistent_grid", ctypes.c_int), ("use_graph", ctypes.c_int),
+                ("best_ms", ctypes.c_float), ("ncandidates", ctypes.c_int), ("candidates", TuneCandidate * 24)]
+
+
 class ServerConfig(ctypes.Structure):
     _fields_ = [("memory_budget", ctypes.c_uint64), ("max_pairs_per_step", ctypes.c_int),
                 ("max_inflight_clips", ctypes.c_int), ("max_width", ctypes.c_int), ("max_height", ctypes.c_int),
@@ -162,6 +172,13 @@ def lib():
     L.Vship_SSIMU2ServerWaitClip.argtypes = [Handler, ctypes.c_int, ctypes.c_void_p, ctypes.POINTER(ctypes.c_double)]
     L.Vship_SSIMU2ServerGetStats.restype = E
     L.Vship_SSIMU2ServerGetStats.argtypes = [Handler, ctypes.POINTER(ServerStats)]
+    L.Vship_SSIMU2BatchTune.restype = E
+    L.Vship_SSIMU2BatchTune.argtypes = [ctypes.POINTER(Pair), ctypes.c_int, ctypes.c_int, ctypes.c_int,
+                                        ctypes.POINTER(TuneResult)]
+    L.Vship_SSIMU2TuneSave.restype = E
+    L.Vship_SSIMU2TuneSave.argtypes = [ctypes.c_char_p]
+    L.Vship_SSIMU2TuneLoad.restype = E
+    L.Vship_SSIMU2TuneLoad.argtypes = [ctypes.c_char_p]
     L.Vship_SSIMU2ServerClockMs.restype = ctypes.c_double
     L.Vship_SSIMU2ServerClockMs.argtypes = []
     _lib = L
@@ -465,3 +482,24 @@ class SSIMU2Server:
 
     def __exit__(self, *a):
         self.close()
+
+
+def autotune(descs, n=None, fp16=False, repeats=3):
+    """Times every launch shape on the sample batch (Vship_SSIMU2BatchTune); the
+    best one is recorded for the sample's size class. Returns a dict."""
+    n = len(descs) if n is None else n
+    r = TuneResult()
+    _check(lib().Vship_SSIMU2BatchTune(descs, n, int(bool(fp16)), int(repeats), ctypes.byref(r)),
+           "Vship_SSIMU2BatchTune")
+    table = [{"map_tile_height": c.map_tile_height, "persistent_grid": c.persistent_grid,
+              "use_graph": c.use_graph, "ms": c.ms} for c in r.candidates[:r.ncandidates]]
+    return {"map_tile_height": r.map_tile_height, "persistent_grid": r.persistent_grid, "use_graph": r.use_graph,
+            "best_ms": r.best_ms, "candidates": table}
+
+
+def tune_save(path):
+    _check(lib().Vship_SSIMU2TuneSave(str(path).encode()), "Vship_SSIMU2TuneSave")
+
+
+def tune_load(path):
+    _check(lib().Vship_SSIMU2TuneLoad(str(path).encode()), "Vship_SSIMU2TuneLoad")

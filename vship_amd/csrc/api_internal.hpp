@@ -14,6 +14,19 @@ bool gpu_kernel_check();
 void gpu_full_check(int gpu_id);
 int default_map_th(int w, int h);
 
+// Autotuned launch shape of one frame-size class (tune.hip).
+struct TuneEntry {
+    int size_class = 0;
+    int fp16 = 0;
+    int th = 48;
+    int grid = 0;
+    int graph = 0;
+    double ms_per_pair = 0;
+};
+int size_class(int w, int h);
+bool tuned_shape(int w, int h, bool fp16, int* th, int* grid, int* graph);
+void record_shape(const TuneEntry& e);
+
 // One per-frame engine of fixed size, as a Vship_SSIMU2Handler refers to
 // (src/ssimu2/main.hpp:153-191 of the reference): host RGB planes are copied
 // row-packed into a device staging area and scored as a batch of one.

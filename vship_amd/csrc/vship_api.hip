@@ -409,13 +409,19 @@ Vship_Exception Vship_SSIMU2BatchInit(Vship_SSIMU2BatchHandler* handler, const V
         o.map_th = cfg->map_tile_height == 48 || cfg->map_tile_height == 64 || cfg->map_tile_height == 112
                        ? cfg->map_tile_height
                        : default_map_th(cfg->max_width, cfg->max_height);
+        o.persistent_grid = cfg->persistent_grid;
+        int tth = 0, tgrid = 0, tgraph = 0;
+        if (cfg->map_tile_height == 0 && cfg->persistent_grid == 0 &&
+            tuned_shape(cfg->max_width, cfg->max_height, cfg->fp16_planes != 0, &tth, &tgrid, &tgraph)) {
+            o.map_th = tth;
+            o.persistent_grid = tgrid;
+        }
         o.arena_bytes = cfg->arena_bytes ? (int64_t)cfg->arena_bytes
                                          : (int64_t)cfg->max_pairs *
                                                (int64_t)Vship_SSIMU2PairFootprint(cfg->max_width, cfg->max_height,
                                                                                   cfg->fp16_planes);
         o.max_map_tiles = (int64_t)cfg->max_pairs * map_tiles(cfg->max_width, cfg->max_height, 32);
-        o.persistent_grid = cfg->persistent_grid;
-        o.use_graph = cfg->use_graph != 0;
+        o.use_graph = cfg->use_graph != 0;  // a captured graph runs a fixed grid (4 x CUs when 0)
         o.profile = cfg->profile != 0;
         auto b = std::make_shared<BatchHandle>();
         b->engine = std::make_unique<Engine>(o);

@@ -8,8 +8,9 @@ blocks; vship_amd/synth/synth.c holds the generator):
           is one pass of the hot path over the 240 pairs already resident in
           HBM: convert + pyramid + XYB, blur/SSIM/edge maps, FP64 scores.
   rgb256  one 256x256 planar RGB8 full-range pair (the CPU-runnable case).
-  4k10    120 pairs 3840x2160 YUV420P10 BT.2020 limited, XYB pyramid stored
-          in FP16 (FP32 math, FP64 sums); per-frame |dscore| vs the FP64 oracle.
+  4k10    120 pairs 3840x2160 YUV420P10 BT.2020 limited, XYB scales 0-1
+          stored in FP16, scales 2-5 in FP32 (FP32 math, FP64 sums);
+          per-frame |dscore| vs the FP64 oracle.
   mixed   64 pairs per step, resolutions drawn by seed from 640x360..3840x2160,
           YUV420P8; launch shape from the autotuner, graph-captured launch.
   served  Poisson clip arrivals (200 clips/s), 8..64 pairs per clip at the
@@ -125,12 +126,12 @@ def per_frame_rate(descs, n, w, h, fp16):
 
 def roofline(st, map_ms, fp16):
     achieved = st["map_bytes_fp32"] / (map_ms * 1e-3) / 1e9
-    elem = 2 if fp16 else 4
+    elem = "2 B (scales 0-1) or 4 B (scales 2-5)" if fp16 else "4 B"
     return {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
             "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
             "kernel": f"map_kernel<{st['map_tile_height']},{'__half' if fp16 else 'float'},float>",
             "algorithmic_bytes_per_launch": st["map_bytes_fp32"],
-            "note": f"2 images x 3 XYB planes x {elem} B read per pixel of every scale in the FP32 class; "
+            "note": f"2 images x 3 XYB planes x {elem} read per pixel of every scale in the FP32 class; "
                     "the kernel is FP32-VALU bound, not HBM bound"}
 
 
@@ -350,7 +351,7 @@ def main():
         pairs, bits = video_pairs(3840, 2160, 10, 120, seed=2), 10
         print(f"# generated in {time.time() - t0:.1f}s", flush=True)
         out, scores = run_batched(a, pairs, 10, True, "ssimulacra2 3840x2160 YUV420P10 BT.2020 limited, FP16 "
-                                  "pyramid planes, 120 pairs per step, inputs resident in HBM")
+                                  "planes for scales 0-1, 120 pairs per step, inputs resident in HBM")
     elif a.config == "mixed":
         res = synth.batch_resolutions(64, seed=3)
         clips, pairs = {}, []

@@ -57,7 +57,7 @@ typedef struct {
     int max_pairs;         /* pairs per batch */
     int max_width;         /* largest frame: sizes the default arena */
     int max_height;
-    int fp16_planes;       /* 1: store the XYB pyramid in FP16 (FP32 math, FP64 sums) */
+    int fp16_planes;       /* 1: store XYB scales 0 and 1 in FP16, scales 2-5 in FP32 (FP32 math, FP64 sums) */
     uint64_t arena_bytes;  /* 0: max_pairs * Vship_SSIMU2PairFootprint(max_width, max_height, fp16_planes) */
     int map_tile_height;   /* 0: tuned shape if recorded, else 48; or 48, 64, 112 rows per map tile */
     int persistent_grid;   /* 0: tuned shape if recorded, else one workgroup per tile; or a fixed grid */

@@ -123,11 +123,14 @@ def test_device_count_without_gpu_reports_an_error_code():
 
 def test_pair_footprint_matches_layout():
     # 6 scales x 3 planes x 2 images, each plane row padded to 64 elements
+    # (FP16 storage: scales 0 and 1 in 2 bytes, scales 2-5 in 4)
     w, h = 1920, 1080
-    tot = 0
+    tot = half = 0
     for s in range(6):
-        tot += ((w + 63) // 64 * 64) * h
+        plane = ((w + 63) // 64 * 64) * h
+        tot += plane * 4
+        half += plane * (2 if s < 2 else 4)
         w, h = (w - 1) // 2 + 1, (h - 1) // 2 + 1
-    assert vship_amd.pair_footprint(1920, 1080) == tot * 3 * 2 * 4
-    assert vship_amd.pair_footprint(1920, 1080, fp16=True) == tot * 3 * 2 * 2
+    assert vship_amd.pair_footprint(1920, 1080) == tot * 3 * 2
+    assert vship_amd.pair_footprint(1920, 1080, fp16=True) == half * 3 * 2
     assert vship_amd.pair_footprint(0, 10) == 0

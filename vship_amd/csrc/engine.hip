@@ -74,7 +74,7 @@ void validate_planes(const PairInput& p) {
 
 }  // namespace
 
-PairLayout pair_layout(int w, int h) {
+PairLayout pair_layout(int w, int h, bool fp16) {
     PairLayout L;
     int64_t off = 0;
     int ww = w, hh = h;
@@ -84,15 +84,15 @@ PairLayout pair_layout(int w, int h) {
         L.pitch[s] = (int)round_up(ww, 64);
         L.plane_elems[s] = (int64_t)L.pitch[s] * hh;
         L.scale_off[s] = off;
-        off += 3 * L.plane_elems[s];
+        off += 3 * L.plane_elems[s] * scale_elem_bytes(s, fp16);
         ww = (ww - 1) / 2 + 1;
         hh = (hh - 1) / 2 + 1;
     }
-    L.img_elems = off;
+    L.img_bytes = off;
     return L;
 }
 
-int64_t pair_elems(int w, int h) { return 2 * pair_layout(w, h).img_elems; }
+int64_t pair_bytes(int w, int h, bool fp16) { return 2 * pair_layout(w, h, fp16).img_bytes; }
 
 int prep_tiles(int w, int h) { return 2 * ((w + kPrepTile - 1) / kPrepTile) * ((h + kPrepTile - 1) / kPrepTile); }
 
@@ -110,11 +110,11 @@ uint64_t map_algorithmic_bytes(int w, int h, bool fp16) {
     uint64_t t = 0;
     int ww = w, hh = h;
     for (int s = 0; s < kScales; s++) {
-        t += (uint64_t)ww * hh;
+        t += (uint64_t)ww * hh * scale_elem_bytes(s, fp16);
         ww = (ww - 1) / 2 + 1;
         hh = (hh - 1) / 2 + 1;
     }
-    return t * 2 /*images*/ * 3 /*planes*/ * (fp16 ? 2 : 4);
+    return t * 2 /*images*/ * 3 /*planes*/;
 }
 
 uint64_t prep_algorithmic_bytes(const PairInput& p, bool fp16) {
@@ -148,10 +148,9 @@ Engine::Engine(const EngineOptions& opt) : opt_(opt) {
                      E_OutOfRAM);
         std::memset(desc_h_, 0, desc_bytes_);
         HIP_OR_THROW(hipMalloc(reinterpret_cast<void**>(&desc_d_), desc_bytes_), E_OutOfVRAM);
-        const int es = opt_.fp16 ? 2 : 4;
-        arena_elems_ = opt_.arena_bytes / es;
-        if (arena_elems_ <= 0) VSHIP_THROW(E_OutOfVRAM);
-        HIP_OR_THROW(hipMalloc(&arena_d_, (size_t)arena_elems_ * es), E_OutOfVRAM);
+        arena_bytes_ = opt_.arena_bytes / 256 * 256;
+        if (arena_bytes_ <= 0) VSHIP_THROW(E_OutOfVRAM);
+        HIP_OR_THROW(hipMalloc(&arena_d_, (size_t)arena_bytes_), E_OutOfVRAM);
         max_map_tiles_ = opt_.max_map_tiles;
         if (max_map_tiles_ <= 0) VSHIP_THROW(E_OutOfVRAM);
         HIP_OR_THROW(hipMalloc(reinterpret_cast<void**>(&partials_d_), sizeof(double) * kFeat * max_map_tiles_),
@@ -212,7 +211,7 @@ void Engine::plan(const PairInput* in, int n) {
         if (p.w < 1 || p.h < 1 || p.fmt < IN_RGB_U8 || p.fmt > IN_YUV420_U16) VSHIP_THROW(E_BadPointer);
         if (p.fmt == IN_YUV420_U16 && (p.bits < 9 || p.bits > 16)) VSHIP_THROW(E_BadPointer);
         validate_planes(p);
-        const PairLayout L = pair_layout(p.w, p.h);
+        const PairLayout L = pair_layout(p.w, p.h, opt_.fp16);
         PairDev& d = pd[i];
         std::memset(&d, 0, sizeof(d));
         for (int im = 0; im < 2; im++)
@@ -226,7 +225,7 @@ void Engine::plan(const PairInput* in, int n) {
         d.w = p.w;
         d.h = p.h;
         d.arena_off = arena_off;
-        d.img_elems = L.img_elems;
+        d.img_bytes = L.img_bytes;
         for (int s = 0; s < kScales; s++) {
             d.scale_off[s] = L.scale_off[s];
             d.plane_elems[s] = L.plane_elems[s];
@@ -234,50 +233,55 @@ void Engine::plan(const PairInput* in, int n) {
             d.sw[s] = L.sw[s];
             d.sh[s] = L.sh[s];
         }
-        arena_off += 2 * L.img_elems;
-        if (arena_off > arena_elems_) VSHIP_THROW(E_OutOfVRAM);
+        arena_off += 2 * L.img_bytes;
+        if (arena_off > arena_bytes_) VSHIP_THROW(E_OutOfVRAM);
         prep_start[i] = (int32_t)ptiles;
         ptiles += prep_tiles(p.w, p.h);
         mbytes += map_algorithmic_bytes(p.w, p.h, opt_.fp16);
         pbytes += prep_algorithmic_bytes(p, opt_.fp16);
     }
-    // map segments: class 0 (FP32 moments) numbered first, then class 1 (FP64)
+    // map segments, group by group: class 0 (FP32 moments) before class 1
+    // (FP64), FP32 planes before FP16 planes within a class
     int64_t mtiles = 0;
     int k = 0;
-    cls_bytes_[0] = cls_bytes_[1] = 0;
-    int nseg_cls[2] = {0, 0};
-    int64_t tiles_cls[2] = {0, 0};
-    for (int cls = 0; cls < 2; cls++) {
+    int nseg_grp[kMapGroups] = {}, tiles_grp[kMapGroups] = {};
+    for (int& c : ntiles_cls_) c = 0;
+    for (uint64_t& b : cls_bytes_) b = 0;
+    for (int g = 0; g < kMapGroups; g++) {
+        const int cls = g / 2;
+        const bool half = g & 1;
         for (int i = 0; i < n; i++)
             for (int s = 0; s < kScales; s++) {
                 const int sw = pd[i].sw[s], sh = pd[i].sh[s];
-                const int c = (int64_t)sw * sh < kSmallSegment ? 1 : 0;
-                if (c != cls) continue;
-                const int th = c ? kSmallTH : opt_.map_th;
+                if (((int64_t)sw * sh < kSmallSegment ? 1 : 0) != cls) continue;
+                if ((opt_.fp16 && s < kHalfScales) != half) continue;
+                const int th = cls ? kSmallTH : opt_.map_th;
                 const int64_t cnt = (int64_t)((sw + kMapTW - 1) / kMapTW) * ((sh + th - 1) / th);
                 const int seg = i * kScales + s;
-                cls_bytes_[cls] += (uint64_t)sw * sh * 2 * 3 * (opt_.fp16 ? 2 : 4);
+                cls_bytes_[cls] += (uint64_t)sw * sh * 2 * 3 * scale_elem_bytes(s, opt_.fp16);
                 seg_start[seg] = (int32_t)mtiles;
                 seg_count[seg] = (int32_t)cnt;
                 order_seg[k] = seg;
                 order_start[k] = (int32_t)mtiles;
                 k++;
                 mtiles += cnt;
-                nseg_cls[cls]++;
-                tiles_cls[cls] += cnt;
+                nseg_grp[g]++;
+                tiles_grp[g] += (int)cnt;
+                if (mtiles > max_map_tiles_ || mtiles > INT32_MAX) VSHIP_THROW(E_OutOfVRAM);
             }
     }
     order_start[k] = (int32_t)mtiles;
-    if (mtiles > max_map_tiles_ || mtiles > INT32_MAX || ptiles > INT32_MAX) VSHIP_THROW(E_OutOfVRAM);
+    if (ptiles > INT32_MAX) VSHIP_THROW(E_OutOfVRAM);
     prep_start[n] = (int32_t)ptiles;
     hdr->npairs = n;
     hdr->prep_tiles = (int32_t)ptiles;
     hdr->map_tiles = (int32_t)mtiles;
     hdr->map_th = opt_.map_th;
-    for (int c = 0; c < 2; c++) {
-        hdr->nseg[c] = nseg_cls[c];
-        hdr->tiles[c] = (int32_t)tiles_cls[c];
-        ntiles_cls_[c] = (int)tiles_cls[c];
+    for (int g = 0; g < kMapGroups; g++) {
+        hdr->nseg[g] = nseg_grp[g];
+        hdr->tiles[g] = tiles_grp[g];
+        ntiles_grp_[g] = tiles_grp[g];
+        ntiles_cls_[g / 2] += tiles_grp[g];
     }
     npairs_ = n;
     ntiles_prep_ = (int)ptiles;
@@ -303,23 +307,27 @@ void Engine::enqueue_body() {
         return std::max(1, work);
     };
     const int g_prep = grid_for(ntiles_prep_, fixed);
-    const int g_map0 = grid_for(ntiles_cls_[0], fixed);
-    const int g_map1 = grid_for(ntiles_cls_[1], std::max(1, fixed / 4));
     const int g_fin = grid_for(npairs_, std::min(opt_.max_pairs, fixed));
-    const size_t copy_bytes = graph ? desc_bytes_ : desc_bytes_;
+    const size_t copy_bytes = desc_bytes_;
     if (prof) HIP_OR_THROW(hipEventRecord(ev_[0], stream_), E_BadDeviceCode);
     HIP_OR_THROW(hipMemcpyAsync(desc_d_, desc_h_, copy_bytes, hipMemcpyHostToDevice, stream_), E_BadDeviceCode);
     if (prof) HIP_OR_THROW(hipEventRecord(ev_[1], stream_), E_BadDeviceCode);
     HIP_OR_THROW(launch_prep(opt_.fp16, g_prep, pd, prep_start, hdr, arena_d_, stream_), E_BadDeviceCode);
     if (prof) HIP_OR_THROW(hipEventRecord(ev_[2], stream_), E_BadDeviceCode);
-    HIP_OR_THROW(launch_map(opt_.fp16, 0, opt_.map_th, g_map0, pd, order_seg, order_start, hdr, arena_d_,
-                            partials_d_, taps_, stream_),
-                 E_BadDeviceCode);
-    if (prof) HIP_OR_THROW(hipEventRecord(ev_[3], stream_), E_BadDeviceCode);
-    HIP_OR_THROW(launch_map(opt_.fp16, 1, kSmallTH, g_map1, pd, order_seg, order_start, hdr, arena_d_,
-                            partials_d_, taps_, stream_),
-                 E_BadDeviceCode);
-    if (prof) HIP_OR_THROW(hipEventRecord(ev_[4], stream_), E_BadDeviceCode);
+    // one launch per map group; FP16-plane groups exist only with FP16 storage,
+    // and an eager launch is skipped when its group is empty
+    for (int cls = 0; cls < 2; cls++) {
+        for (int half = 0; half < 2; half++) {
+            const int g = map_group(cls, half);
+            if (half && !opt_.fp16) continue;
+            if (!graph && ntiles_grp_[g] == 0) continue;
+            const int grid = grid_for(ntiles_grp_[g], cls == 0 ? fixed : std::max(1, fixed / 4));
+            HIP_OR_THROW(launch_map(half, cls, cls ? kSmallTH : opt_.map_th, grid, pd, order_seg, order_start, hdr,
+                                    arena_d_, partials_d_, taps_, stream_),
+                         E_BadDeviceCode);
+        }
+        if (prof) HIP_OR_THROW(hipEventRecord(ev_[3 + cls], stream_), E_BadDeviceCode);
+    }
     double* scores_d = out_d_;
     double* feats_d = out_d_ + opt_.max_pairs;
     HIP_OR_THROW(launch_finalize(g_fin, pd, seg_start, seg_count, hdr, partials_d_, scores_d, feats_d, stream_),

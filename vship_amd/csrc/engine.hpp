@@ -71,15 +71,17 @@ struct Timings {
     float prep_ms = 0, map_ms = 0, map32_ms = 0, map64_ms = 0, finalize_ms = 0, total_ms = 0;
 };
 
-// Geometry of one pair in the arena.
+// Geometry of one pair in the arena (offsets in bytes, pitches in elements).
 struct PairLayout {
     int sw[kScales], sh[kScales], pitch[kScales];
     int64_t scale_off[kScales], plane_elems[kScales];
-    int64_t img_elems;
+    int64_t img_bytes;
 };
 
-PairLayout pair_layout(int w, int h);
-int64_t pair_elems(int w, int h);  // both images
+constexpr int kMapGroups = 4;  // (FP32 | FP64 moments) x (FP32 | FP16 planes)
+inline int scale_elem_bytes(int s, bool fp16) { return fp16 && s < kHalfScales ? 2 : 4; }
+PairLayout pair_layout(int w, int h, bool fp16);
+int64_t pair_bytes(int w, int h, bool fp16);  // arena bytes of both images
 int prep_tiles(int w, int h);
 int map_tiles(int w, int h, int th);
 uint64_t map_algorithmic_bytes(int w, int h, bool fp16);
@@ -126,7 +128,7 @@ private:
     uint8_t* desc_h_ = nullptr;  // pinned
     uint8_t* desc_d_ = nullptr;
     void* arena_d_ = nullptr;
-    int64_t arena_elems_ = 0;
+    int64_t arena_bytes_ = 0;
     double* partials_d_ = nullptr;
     int64_t max_map_tiles_ = 0;
     double* out_d_ = nullptr;   // scores [max_pairs] then features [max_pairs*108]
@@ -135,7 +137,7 @@ private:
     hipGraph_t graph_ = nullptr;
     hipGraphExec_t graph_exec_ = nullptr;
 
-    int npairs_ = 0, ntiles_prep_ = 0, ntiles_map_ = 0, ntiles_cls_[2] = {0, 0};
+    int npairs_ = 0, ntiles_prep_ = 0, ntiles_map_ = 0, ntiles_cls_[2] = {0, 0}, ntiles_grp_[kMapGroups] = {};
     uint64_t map_bytes_ = 0, prep_bytes_ = 0, cls_bytes_[2] = {0, 0};
     bool pending_ = false;
     Timings timings_;

@@ -75,7 +75,7 @@ public:
         o.persistent_grid = cfg_.persistent_grid;
         o.use_graph = cfg_.use_graph != 0;
         step_budget_ = o.arena_bytes;
-        es_ = o.fp16 ? 2 : 4;
+        fp16_ = o.fp16;
         for (int i = 0; i < 2; i++) eng_[i] = std::make_unique<Engine>(o);
         worker_ = std::thread([this] { loop(); });
     }
@@ -109,7 +109,7 @@ public:
             in.matrix = p.matrix;
             if (p.width < 1 || p.height < 1 || p.width > cfg_.max_width || p.height > cfg_.max_height)
                 VSHIP_THROW(E_BadPointer);
-            c->footprint = std::max<int64_t>(c->footprint, pair_elems(p.width, p.height) * es_);
+            c->footprint = std::max<int64_t>(c->footprint, pair_bytes(p.width, p.height, fp16_));
         }
         if (c->footprint > step_budget_) VSHIP_THROW(E_OutOfVRAM);
         c->scores.assign(n, 0.0);
@@ -269,7 +269,7 @@ private:
     Vship_SSIMU2ServerConfig cfg_;
     int device_;
     int64_t step_budget_ = 0;
-    int es_ = 4;
+    bool fp16_ = false;
     std::unique_ptr<Engine> eng_[2];
     std::thread worker_;
     std::mutex mu_;

@@ -4,7 +4,11 @@
 // formats. For each pair the engine reserves one region of the XYB arena:
 //   image (0 = reference, 1 = distorted) -> scale 0..5 -> plane X, Y, B
 // every plane row-major with a pitch that is a multiple of 64 elements, every
-// plane offset a multiple of 64 elements (256-byte aligned rows in FP32).
+// plane offset a multiple of 128 bytes. Planes are FP32, except that with
+// FP16 storage the two finest scales (s < kHalfScales, 15/16 of the plane
+// bytes) are FP16: rounding the coarse scales to FP16 moves the score by up
+// to 2e-2 (the coarse-scale features carry large weights), rounding scales
+// 0 and 1 by about 2e-4.
 #pragma once
 #include <stdint.h>
 
@@ -14,6 +18,7 @@ constexpr int kScales = 6;
 constexpr int kPrepTile = 64;   // scale-0 tile edge of the convert/pyramid kernel
 constexpr int kMapTW = 32;      // map tile width (columns)
 constexpr int kFeat = 18;       // partial sums per map tile: 3 channels x 3 maps x {sum d, sum d^4}
+constexpr int kHalfScales = 2;  // scales stored in FP16 when FP16 storage is on
 
 // Input sample formats. RGB formats are the reference's InputMemType
 // (src/util/preprocessor.hpp:94); YUV 4:2:0 is the fused front end.
@@ -33,10 +38,10 @@ struct PairDev {
     int32_t bits;               // YUV sample depth
     float kr, kb;               // YUV matrix coefficients
     int32_t w, h;               // scale-0 size
-    int64_t arena_off;          // element offset of the pair's region
-    int64_t img_elems;          // elements of one image region
-    int64_t scale_off[kScales]; // element offset of scale s (X plane) in an image region
-    int64_t plane_elems[kScales];
+    int64_t arena_off;          // byte offset of the pair's region
+    int64_t img_bytes;          // bytes of one image region
+    int64_t scale_off[kScales]; // byte offset of scale s (X plane) in an image region
+    int64_t plane_elems[kScales];  // elements of one plane (pitch x height)
     int32_t pitch[kScales];
     int32_t sw[kScales], sh[kScales];
 };
@@ -44,15 +49,19 @@ struct PairDev {
 struct BatchHdr {
     int32_t npairs;
     int32_t prep_tiles;  // total convert/pyramid tiles
-    int32_t map_tiles;   // total map tiles of both classes
-    int32_t map_th;      // map tile height of class 0
+    int32_t map_tiles;   // total map tiles of all groups
+    int32_t map_th;      // map tile height of the FP32 groups
     // Map segments (one per pair and scale) fall in two classes: class 0
     // accumulates the moment blur in FP32, class 1 (segments of fewer than
     // kSmallSegment pixels, where FP32 rounding noise would not average out)
-    // in FP64 with tiles of kSmallTH rows. Class-0 tiles are numbered first.
-    int32_t nseg[2];
-    int32_t tiles[2];
+    // in FP64 with tiles of kSmallTH rows. Each class splits by plane storage
+    // into group 2*class (FP32 planes) and 2*class+1 (FP16 planes); segments
+    // and tiles are numbered group by group.
+    int32_t nseg[4];
+    int32_t tiles[4];
 };
+
+__host__ __device__ constexpr int map_group(int cls, bool half) { return 2 * cls + (half ? 1 : 0); }
 
 constexpr int64_t kSmallSegment = 32768;
 constexpr int kSmallTH = 48;

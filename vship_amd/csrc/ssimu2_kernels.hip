@@ -83,23 +83,33 @@ struct PrepLds {
     float l4[3][4][5];
 };
 
-template <typename ST>
-__device__ __forceinline__ void store_xyb(ST* arena, const PairDev& P, int img, int s, int x, int y, float r,
-                                          float g, float b) {
-    float X, Y, B;
-    to_xyb(r, g, b, X, Y, B);
-    ST* base = arena + P.arena_off + img * P.img_elems + P.scale_off[s] + (int64_t)y * P.pitch[s] + x;
+template <typename T>
+__device__ __forceinline__ void store_planes(uint8_t* arena, const PairDev& P, int img, int s, int x, int y, float X,
+                                             float Y, float B) {
+    T* base = reinterpret_cast<T*>(arena + P.arena_off + img * P.img_bytes + P.scale_off[s]) +
+              (int64_t)y * P.pitch[s] + x;
     const int64_t pe = P.plane_elems[s];
     stf(base, X);
     stf(base + pe, Y);
     stf(base + 2 * pe, B);
 }
 
+// XYB of one pixel of scale s into the arena: planes of type ST for the two
+// finest scales, FP32 for the rest.
+template <typename ST>
+__device__ __forceinline__ void store_xyb(uint8_t* arena, const PairDev& P, int img, int s, int x, int y, float r,
+                                          float g, float b) {
+    float X, Y, B;
+    to_xyb(r, g, b, X, Y, B);
+    if (s < kHalfScales) store_planes<ST>(arena, P, img, s, x, y, X, Y, B);
+    else store_planes<float>(arena, P, img, s, x, y, X, Y, B);
+}
+
 // One pyramid level in LDS: child(cx, cy) = mean of the 2x2 parents with the
 // parent index clamped to the last valid row/column
 // (src/ssimu2/downsample.hpp:3-19, summation order kept).
 template <int TS, int PP, int CP, typename ST>
-__device__ __forceinline__ void pyr_level(const float* par, float* chi, ST* arena, const PairDev& P, int img, int s,
+__device__ __forceinline__ void pyr_level(const float* par, float* chi, uint8_t* arena, const PairDev& P, int img, int s,
                                           int tx, int ty, int npx, int npy) {
     const int x0 = tx * TS, y0 = ty * TS;
     const int nvx = min(TS, P.sw[s] - x0), nvy = min(TS, P.sh[s] - y0);
@@ -119,7 +129,7 @@ __device__ __forceinline__ void pyr_level(const float* par, float* chi, ST* aren
             v[c] = a * 0.25f;
             if (chi) chi[(c * TS + cy) * CP + cx] = v[c];
         }
-        store_xyb(arena, P, img, s, x0 + cx, y0 + cy, v[0], v[1], v[2]);
+        store_xyb<ST>(arena, P, img, s, x0 + cx, y0 + cy, v[0], v[1], v[2]);
     }
 }
 
@@ -140,7 +150,7 @@ __device__ __forceinline__ float rgb_sample(const uint8_t* plane, int64_t stride
 template <typename ST>
 __global__ __launch_bounds__(256) void prep_kernel(const PairDev* __restrict__ pairs,
                                                    const int32_t* __restrict__ prep_start,
-                                                   const BatchHdr* __restrict__ hdr, ST* __restrict__ arena) {
+                                                   const BatchHdr* __restrict__ hdr, uint8_t* __restrict__ arena) {
     __shared__ PrepLds L;
     const int npairs = hdr->npairs;
     const int ntiles = hdr->prep_tiles;
@@ -246,10 +256,10 @@ __global__ __launch_bounds__(256) void prep_kernel(const PairDev* __restrict__ p
             for (int k = 0; k < 4; k++)
 #pragma unroll
                 for (int p = 0; p < 3; p++) rgb[k][p] = linearize(rgb[k][p]);
-            store_xyb(arena, P, img, 0, xa, ya, rgb[0][0], rgb[0][1], rgb[0][2]);
-            if (hy) store_xyb(arena, P, img, 0, xa, yb, rgb[1][0], rgb[1][1], rgb[1][2]);
-            if (hx) store_xyb(arena, P, img, 0, xb, ya, rgb[2][0], rgb[2][1], rgb[2][2]);
-            if (hx && hy) store_xyb(arena, P, img, 0, xb, yb, rgb[3][0], rgb[3][1], rgb[3][2]);
+            store_xyb<ST>(arena, P, img, 0, xa, ya, rgb[0][0], rgb[0][1], rgb[0][2]);
+            if (hy) store_xyb<ST>(arena, P, img, 0, xa, yb, rgb[1][0], rgb[1][1], rgb[1][2]);
+            if (hx) store_xyb<ST>(arena, P, img, 0, xb, ya, rgb[2][0], rgb[2][1], rgb[2][2]);
+            if (hx && hy) store_xyb<ST>(arena, P, img, 0, xb, yb, rgb[3][0], rgb[3][1], rgb[3][2]);
             // scale-1 child (src/ssimu2/downsample.hpp:3-19 summation order)
             if (qx < nv1x && qy < nv1y) {
                 float v[3];
@@ -261,7 +271,7 @@ __global__ __launch_bounds__(256) void prep_kernel(const PairDev* __restrict__ p
                     a += rgb[3][p];
                     v[p] = a * 0.25f;
                 }
-                store_xyb(arena, P, img, 1, tx * 32 + qx, ty * 32 + qy, v[0], v[1], v[2]);
+                store_xyb<ST>(arena, P, img, 1, tx * 32 + qx, ty * 32 + qy, v[0], v[1], v[2]);
 #pragma unroll
                 for (int p = 0; p < 3; p++) L.l1[p][qy][qx] = v[p];
             }
@@ -269,16 +279,16 @@ __global__ __launch_bounds__(256) void prep_kernel(const PairDev* __restrict__ p
         __syncthreads();
 
         // scales 2..5 from LDS
-        pyr_level<16, 33, 17>(&L.l1[0][0][0], &L.l2[0][0][0], arena, P, img, 2, tx, ty, nv1x, nv1y);
+        pyr_level<16, 33, 17, ST>(&L.l1[0][0][0], &L.l2[0][0][0], arena, P, img, 2, tx, ty, nv1x, nv1y);
         __syncthreads();
         const int nv2x = min(16, P.sw[2] - tx * 16), nv2y = min(16, P.sh[2] - ty * 16);
-        pyr_level<8, 17, 9>(&L.l2[0][0][0], &L.l3[0][0][0], arena, P, img, 3, tx, ty, nv2x, nv2y);
+        pyr_level<8, 17, 9, ST>(&L.l2[0][0][0], &L.l3[0][0][0], arena, P, img, 3, tx, ty, nv2x, nv2y);
         __syncthreads();
         const int nv3x = min(8, P.sw[3] - tx * 8), nv3y = min(8, P.sh[3] - ty * 8);
-        pyr_level<4, 9, 5>(&L.l3[0][0][0], &L.l4[0][0][0], arena, P, img, 4, tx, ty, nv3x, nv3y);
+        pyr_level<4, 9, 5, ST>(&L.l3[0][0][0], &L.l4[0][0][0], arena, P, img, 4, tx, ty, nv3x, nv3y);
         __syncthreads();
         const int nv4x = min(4, P.sw[4] - tx * 4), nv4y = min(4, P.sh[4] - ty * 4);
-        pyr_level<2, 5, 1>(&L.l4[0][0][0], (float*)nullptr, arena, P, img, 5, tx, ty, nv4x, nv4y);
+        pyr_level<2, 5, 1, ST>(&L.l4[0][0][0], (float*)nullptr, arena, P, img, 5, tx, ty, nv4x, nv4y);
         __syncthreads();
     }
 }
@@ -368,12 +378,12 @@ template <int TH, typename ST, typename AT>
 __global__ __launch_bounds__(256, (sizeof(AT) == 4 && TH <= 64) ? 2 : 1) void map_kernel(const PairDev* __restrict__ pairs,
                                                   const int32_t* __restrict__ order_seg,
                                                   const int32_t* __restrict__ order_start,
-                                                  const BatchHdr* __restrict__ hdr, const ST* __restrict__ arena,
+                                                  const BatchHdr* __restrict__ hdr, const uint8_t* __restrict__ arena,
                                                   double* __restrict__ partials, const Taps tp) {
     using C = MapCfg<TH, AT>;
     constexpr int TW = C::TW, HR = C::HR, R = C::R, LDW = C::LDW, OUT = C::OUT, PASS = C::PASS;
     constexpr int NIN = OUT + 16;   // samples a row-pass thread reads
-    constexpr int cls = sizeof(AT) == 4 ? 0 : 1;
+    constexpr int grp = map_group(sizeof(AT) == 4 ? 0 : 1, sizeof(ST) == 2);
     __shared__ AT hb[4][HR][LDW];   // row-pass results: a, b, (x1 - x2)^2, ab
     __shared__ AT cen[2][TH][LDW];  // centred samples of the output pixels
     __shared__ double red[4][kFeat];
@@ -388,10 +398,14 @@ __global__ __launch_bounds__(256, (sizeof(AT) == 4 && TH <= 64) ? 2 : 1) void ma
         }
     }
     __syncthreads();
-    const int seg_lo = cls == 0 ? 0 : hdr->nseg[0];
-    const int seg_hi = seg_lo + hdr->nseg[cls] - 1;
-    const int t_begin = cls == 0 ? 0 : hdr->tiles[0];
-    const int t_end = t_begin + hdr->tiles[cls];
+    int seg_lo = 0, t_begin = 0;
+#pragma unroll
+    for (int g = 0; g < grp; g++) {
+        seg_lo += hdr->nseg[g];
+        t_begin += hdr->tiles[g];
+    }
+    const int seg_hi = seg_lo + hdr->nseg[grp] - 1;
+    const int t_end = t_begin + hdr->tiles[grp];
     // taps as compile-time constants (instruction literals, no registers)
 #define gr(k) ((AT)tap(k))
 
@@ -421,8 +435,8 @@ __global__ __launch_bounds__(256, (sizeof(AT) == 4 && TH <= 64) ? 2 : 1) void ma
         const int local = t - order_start[lo];
         const int ty = local / ntx, tx = local - ty * ntx;
         const int x0 = tx * TW, y0 = ty * TH;
-        const ST* base1 = arena + P.arena_off + P.scale_off[s];
-        const ST* base2 = base1 + P.img_elems;
+        const ST* base1 = reinterpret_cast<const ST*>(arena + P.arena_off + P.scale_off[s]);
+        const ST* base2 = reinterpret_cast<const ST*>(arena + P.arena_off + P.img_bytes + P.scale_off[s]);
         const int64_t pe = P.plane_elems[s];
         const bool xin = (x0 >= 8) && (x0 + TW + 8 <= W);
 
@@ -667,9 +681,9 @@ __global__ __launch_bounds__(256) void finalize_kernel(const PairDev* __restrict
 hipError_t launch_prep(bool fp16, int grid, const PairDev* pairs, const int32_t* prep_start, const BatchHdr* hdr,
                        void* arena, hipStream_t st) {
     if (fp16)
-        prep_kernel<__half><<<grid, 256, 0, st>>>(pairs, prep_start, hdr, reinterpret_cast<__half*>(arena));
+        prep_kernel<__half><<<grid, 256, 0, st>>>(pairs, prep_start, hdr, reinterpret_cast<uint8_t*>(arena));
     else
-        prep_kernel<float><<<grid, 256, 0, st>>>(pairs, prep_start, hdr, reinterpret_cast<float*>(arena));
+        prep_kernel<float><<<grid, 256, 0, st>>>(pairs, prep_start, hdr, reinterpret_cast<uint8_t*>(arena));
     return hipGetLastError();
 }
 
@@ -677,23 +691,23 @@ template <int TH, typename ST, typename AT>
 static void launch_map_t(int grid, const PairDev* pairs, const int32_t* order_seg, const int32_t* order_start,
                          const BatchHdr* hdr, const void* arena, double* partials, const Taps& taps, hipStream_t st) {
     map_kernel<TH, ST, AT><<<grid, 256, 0, st>>>(pairs, order_seg, order_start, hdr,
-                                                 reinterpret_cast<const ST*>(arena), partials, taps);
+                                                 reinterpret_cast<const uint8_t*>(arena), partials, taps);
 }
 
-hipError_t launch_map(bool fp16, int cls, int th, int grid, const PairDev* pairs, const int32_t* order_seg,
+hipError_t launch_map(bool half, int cls, int th, int grid, const PairDev* pairs, const int32_t* order_seg,
                       const int32_t* order_start, const BatchHdr* hdr, const void* arena, double* partials,
                       const Taps& taps, hipStream_t st) {
     if (cls == 1) {
-        if (fp16) launch_map_t<kSmallTH, __half, double>(grid, pairs, order_seg, order_start, hdr, arena, partials, taps, st);
+        if (half) launch_map_t<kSmallTH, __half, double>(grid, pairs, order_seg, order_start, hdr, arena, partials, taps, st);
         else launch_map_t<kSmallTH, float, double>(grid, pairs, order_seg, order_start, hdr, arena, partials, taps, st);
     } else if (th == 112) {
-        if (fp16) launch_map_t<112, __half, float>(grid, pairs, order_seg, order_start, hdr, arena, partials, taps, st);
+        if (half) launch_map_t<112, __half, float>(grid, pairs, order_seg, order_start, hdr, arena, partials, taps, st);
         else launch_map_t<112, float, float>(grid, pairs, order_seg, order_start, hdr, arena, partials, taps, st);
     } else if (th == 64) {
-        if (fp16) launch_map_t<64, __half, float>(grid, pairs, order_seg, order_start, hdr, arena, partials, taps, st);
+        if (half) launch_map_t<64, __half, float>(grid, pairs, order_seg, order_start, hdr, arena, partials, taps, st);
         else launch_map_t<64, float, float>(grid, pairs, order_seg, order_start, hdr, arena, partials, taps, st);
     } else {
-        if (fp16) launch_map_t<48, __half, float>(grid, pairs, order_seg, order_start, hdr, arena, partials, taps, st);
+        if (half) launch_map_t<48, __half, float>(grid, pairs, order_seg, order_start, hdr, arena, partials, taps, st);
         else launch_map_t<48, float, float>(grid, pairs, order_seg, order_start, hdr, arena, partials, taps, st);
     }
     return hipGetLastError();

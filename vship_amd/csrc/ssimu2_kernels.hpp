@@ -8,8 +8,10 @@ namespace vship {
 
 hipError_t launch_prep(bool fp16, int grid, const PairDev* pairs, const int32_t* prep_start, const BatchHdr* hdr,
                        void* arena, hipStream_t st);
-// cls 0: FP32 moments, tile height th (32 or 64); cls 1: FP64 moments, tile height kSmallTH
-hipError_t launch_map(bool fp16, int cls, int th, int grid, const PairDev* pairs, const int32_t* order_seg,
+// Map tiles of one group: cls 0 FP32 moments with tile height th (48, 64 or
+// 112), cls 1 FP64 moments with tile height kSmallTH; half: the group's
+// planes are FP16.
+hipError_t launch_map(bool half, int cls, int th, int grid, const PairDev* pairs, const int32_t* order_seg,
                       const int32_t* order_start, const BatchHdr* hdr, const void* arena, double* partials,
                       const Taps& taps, hipStream_t st);
 hipError_t launch_finalize(int grid, const PairDev* pairs, const int32_t* seg_start, const int32_t* seg_count,

@@ -99,7 +99,7 @@ Vship_Exception Vship_SSIMU2BatchTune(const Vship_SSIMU2Pair* pairs, int npairs,
             x.matrix = p.matrix;
             mw = std::max(mw, p.width);
             mh = std::max(mh, p.height);
-            arena += pair_elems(p.width, p.height) * (fp16_planes ? 2 : 4);
+            arena += pair_bytes(p.width, p.height, fp16_planes != 0);
         }
         const int ths[3] = {48, 64, 112};
         const int grids[4] = {0, 2 * cus, 4 * cus, 8 * cus};

@@ -95,7 +95,7 @@ FrameHandle::FrameHandle(int w, int h, int device) : width(w), height(h), device
     o.device = device;
     o.max_pairs = 1;
     o.map_th = default_map_th(w, h);
-    o.arena_bytes = pair_elems(w, h) * 4;
+    o.arena_bytes = pair_bytes(w, h, false);
     o.max_map_tiles = map_tiles(w, h, 32);
     engine = std::make_unique<Engine>(o);
 }
@@ -391,7 +391,7 @@ Vship_Exception Vship_ComputeButteraugliUint16v2(Vship_ButteraugliHandler, Vship
 // ---------------------------------------------------------------------------
 uint64_t Vship_SSIMU2PairFootprint(int width, int height, int fp16_planes) {
     if (width <= 0 || height <= 0) return 0;
-    return (uint64_t)pair_elems(width, height) * (fp16_planes ? 2 : 4);
+    return (uint64_t)pair_bytes(width, height, fp16_planes != 0);
 }
 
 Vship_Exception Vship_SSIMU2BatchInit(Vship_SSIMU2BatchHandler* handler, const Vship_SSIMU2BatchConfig* cfg) {

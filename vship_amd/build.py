@@ -71,7 +71,7 @@ def build_oracle(with_reference=True):
     _run(["make", "-s", "-C", os.path.join(ROOT, "oracle")])
     ref = os.environ.get("VSHIP_REFERENCE", "/root/reference")
     if with_reference and os.path.exists(os.path.join(ref, "src", "ssimu2", "main.hpp")):
-        _run(["make", "-s", "-C", os.path.join(ROOT, "oracle"), "ref", f"REFERENCE={ref}"])
+        _run(["make", "-s", "-C", os.path.join(ROOT, "oracle"), "ref", "clients", f"REFERENCE={ref}"])
 
 
 def main(argv=None):

@@ -11,6 +11,11 @@
 //   (src/ssimu2/main.hpp:153-191), as Vship_ComputeSSIMU2* does.
 // vref_ssimu2_features: the same stages (src/ssimu2/main.hpp:106-151,45-98)
 //   driven step by step so the 108 features can be read back.
+// vref_ssimu2_time: one handle (init once), `iters` calls of run() on the
+//   same host planes, as an ABI client of Vship_ComputeSSIMU2* does; returns
+//   the wall time of the calls in milliseconds.
+#include <chrono>
+
 #include "ssimu2/main.hpp"
 
 namespace {
@@ -107,6 +112,33 @@ int vref_ssimu2_features(int fmt, const uint8_t* p1[3], const uint8_t* p2[3], in
     } catch (const VshipError& e) {
         return (int)e.type;
     }
+}
+
+int vref_ssimu2_time(int fmt, const uint8_t* p1[3], const uint8_t* p2[3], int64_t s1, int64_t s2, int w, int h,
+                     int iters, double* score, double* ms) {
+    try {
+        ssimu2::SSIMU2ComputingImplementation impl;
+        impl.init(w, h);
+        double r = 0;
+        auto once = [&]() {
+            switch (fmt) {
+            case 0: r = impl.run<InputMemType::UINT8>(p1, p2, s1, s2); break;
+            case 1: r = impl.run<InputMemType::UINT16>(p1, p2, s1, s2); break;
+            case 2: r = impl.run<InputMemType::HALF>(p1, p2, s1, s2); break;
+            default: r = impl.run<InputMemType::FLOAT>(p1, p2, s1, s2); break;
+            }
+        };
+        once();  // warm-up
+        const auto t0 = std::chrono::steady_clock::now();
+        for (int i = 0; i < iters; i++) once();
+        const auto t1 = std::chrono::steady_clock::now();
+        impl.destroy();
+        *score = r;
+        *ms = std::chrono::duration<double, std::milli>(t1 - t0).count();
+    } catch (const VshipError& e) {
+        return (int)e.type;
+    }
+    return 0;
 }
 
 }  // extern "C"

@@ -221,6 +221,9 @@ def ref_lib():
         R.vref_ssimu2.argtypes = [ctypes.c_int, P3, P3, I64, I64, ctypes.c_int, ctypes.c_int, D]
         R.vref_ssimu2_features.restype = ctypes.c_int
         R.vref_ssimu2_features.argtypes = [ctypes.c_int, P3, P3, I64, I64, ctypes.c_int, ctypes.c_int, F]
+        R.vref_ssimu2_time.restype = ctypes.c_int
+        R.vref_ssimu2_time.argtypes = [ctypes.c_int, P3, P3, I64, I64, ctypes.c_int, ctypes.c_int, ctypes.c_int, D,
+                                       D]
         _ref = R
     return _ref
 
@@ -239,6 +242,18 @@ def ref_score_rgb(fmt, img1, img2):
                                 f.ctypes.data_as(F))
     assert rc == 0, rc
     return s.value, f.astype(np.float64)
+
+
+def ref_time_rgb(fmt, img1, img2, iters):
+    """(score, ms per call) of the reference's per-frame path: one handle, host planes."""
+    p1 = [np.ascontiguousarray(p) for p in img1]
+    p2 = [np.ascontiguousarray(p) for p in img2]
+    h, w = p1[0].shape
+    s, ms = ctypes.c_double(), ctypes.c_double()
+    rc = ref_lib().vref_ssimu2_time(fmt, _planes(p1), _planes(p2), p1[0].strides[0], p2[0].strides[0], w, h,
+                                    int(iters), ctypes.byref(s), ctypes.byref(ms))
+    assert rc == 0, rc
+    return s.value, ms.value / iters
 
 
 def reference_feature_mismatch(f_ref, f_oracle):

@@ -103,3 +103,25 @@ def test_reference_apibenchmark_client_runs_on_this_library():
     say(r.stdout)
     assert r.returncode == 0
     assert "SSIMU2 Process Uint16 time" in r.stdout
+
+
+@needs_ref
+def test_per_frame_path_against_reference_on_this_gpu():
+    """Throughput of the plain per-frame path, one 1920x1080 u16 RGB pair per
+    call from host memory, for the reference's own implementation and for this
+    library's Vship_ComputeSSIMU2Uint16 on the same planes (printed; the
+    scores agree to the reference's FP32 error)."""
+    import time
+
+    ref, dist = rgb_pair(1920, 1080, api.PF_RGB_U16, 21)
+    iters = 30
+    s_r, ms_r = ol.ref_time_rgb(ol.SO_RGB_U16, ref, dist, iters)
+    with vship_amd.SSIMU2Handler(1920, 1080) as h:
+        s_g = h.compute_uint16(ref, dist)
+        t0 = time.perf_counter()
+        for _ in range(iters):
+            s_g = h.compute_uint16(ref, dist)
+        ms_g = (time.perf_counter() - t0) * 1e3 / iters
+    say(f"per-frame 1080p u16 from host: reference {ms_r:.3f} ms/pair ({1e3 / ms_r:.1f} pairs/s) score {s_r:.6f}; "
+        f"this library {ms_g:.3f} ms/pair ({1e3 / ms_g:.1f} pairs/s) score {s_g:.6f}")
+    assert abs(s_g - s_r) <= 1e-2

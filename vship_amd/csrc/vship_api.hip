@@ -115,8 +115,12 @@ double FrameHandle::compute(int fmt, const uint8_t* const p1[3], const uint8_t* 
     const int bps = fmt == IN_RGB_F32 ? 4 : (fmt == IN_RGB_U8 ? 1 : 2);
     const int64_t row = (int64_t)width * bps;
     if (s1 < row || s2 < row) VSHIP_THROW(E_BadPointer);
-    // rows are copied without their padding into a packed device staging area
-    const size_t need = (size_t)row * height * 6;
+    // each plane is copied as one contiguous block of (height - 1) strides
+    // plus one row, keeping the caller's stride: a 1-D copy from pageable
+    // memory runs at copy-engine speed, a 2-D one row by row
+    const size_t blk1 = (size_t)s1 * (height - 1) + row, blk2 = (size_t)s2 * (height - 1) + row;
+    const size_t a1 = (blk1 + 255) / 256 * 256, a2 = (blk2 + 255) / 256 * 256;
+    const size_t need = 3 * (a1 + a2);
     if (need > staging_bytes) {
         if (staging) (void)hipFree(staging);
         staging = nullptr;
@@ -130,12 +134,12 @@ double FrameHandle::compute(int fmt, const uint8_t* const p1[3], const uint8_t* 
     for (int im = 0; im < 2; im++) {
         const uint8_t* const* src = im == 0 ? p1 : p2;
         const int64_t sp = im == 0 ? s1 : s2;
+        const size_t blk = im == 0 ? blk1 : blk2;
         for (int pl = 0; pl < 3; pl++) {
-            uint8_t* dst = staging + (size_t)row * height * (im * 3 + pl);
-            HIP_OR_THROW(hipMemcpy2DAsync(dst, row, src[pl], sp, row, height, hipMemcpyHostToDevice, st),
-                         E_BadPointer);
+            uint8_t* dst = staging + (im == 0 ? pl * a1 : 3 * a1 + pl * a2);
+            HIP_OR_THROW(hipMemcpyAsync(dst, src[pl], blk, hipMemcpyHostToDevice, st), E_BadPointer);
             in.src[im][pl] = dst;
-            in.stride[im][pl] = row;
+            in.stride[im][pl] = sp;
         }
     }
     in.w = width;

@@ -67,6 +67,19 @@ def build_synth():
     return SYNTH_LIB
 
 
+MOCK_SRC = os.path.join(ROOT, "tests", "vs_mock_host.cpp")
+MOCK_LIB = os.path.join(ROOT, "tests", "_build", "libvs_mock_host.so")
+
+
+def build_test_helpers():
+    """The VapourSynth mock host the plugin tests drive (tests/vs_mock_host.cpp)."""
+    os.makedirs(os.path.dirname(MOCK_LIB), exist_ok=True)
+    tmp = MOCK_LIB + ".tmp"
+    _run(["g++", "-O2", "-std=c++17", "-fPIC", "-shared", "-pthread", "-Wall", "-o", tmp, MOCK_SRC, "-ldl"])
+    os.replace(tmp, MOCK_LIB)
+    return MOCK_LIB
+
+
 def build_oracle(with_reference=True):
     _run(["make", "-s", "-C", os.path.join(ROOT, "oracle")])
     ref = os.environ.get("VSHIP_REFERENCE", "/root/reference")
@@ -82,6 +95,7 @@ def main(argv=None):
     a = ap.parse_args(argv)
     print(build_lib(a.j, a.v))
     print(build_synth())
+    print(build_test_helpers())
     if a.all:
         build_oracle()
         print("oracle built")

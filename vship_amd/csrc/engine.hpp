@@ -32,8 +32,11 @@ enum ErrCode : int {
 
 struct VshipErr : std::exception {
     ErrCode code;
+    std::string file;
+    int line;
     std::string where;
-    VshipErr(ErrCode c, const char* file, int line) : code(c), where(std::string(file) + ":" + std::to_string(line)) {}
+    VshipErr(ErrCode c, const char* f, int l)
+        : code(c), file(f), line(l), where(std::string(f) + ":" + std::to_string(l)) {}
     const char* what() const noexcept override { return where.c_str(); }
 };
 #define VSHIP_THROW(code) throw ::vship::VshipErr((code), __FILE__, __LINE__)

@@ -129,10 +129,12 @@ def roofline(st, map_ms, fp16):
     elem = "2 B (scales 0-1) or 4 B (scales 2-5)" if fp16 else "4 B"
     return {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
             "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
-            "kernel": f"map_kernel<{st['map_tile_height']},{'__half' if fp16 else 'float'},float>",
+            "kernel": (f"map_mx_kernel<{'__half' if fp16 else 'float'}> (matrix cores, 64x64 tiles)"
+                       if st["map_tile_height"] == 1 else
+                       f"map_kernel<{st['map_tile_height']},{'__half' if fp16 else 'float'},float>"),
             "algorithmic_bytes_per_launch": st["map_bytes_fp32"],
             "note": f"2 images x 3 XYB planes x {elem} read per pixel of every scale in the FP32 class; "
-                    "the kernel is FP32-VALU bound, not HBM bound"}
+                    "the kernel is bound by vector-ALU issue (operand splitting and the maps), not HBM"}
 
 
 def record(a, value, elapsed, steps, dtype, workload, extra_cfg):

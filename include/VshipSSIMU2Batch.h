@@ -53,13 +53,17 @@ typedef struct {
     int id;
 } Vship_SSIMU2BatchHandler;
 
+#define VSHIP_MAP_MFMA 1
+
 typedef struct {
     int max_pairs;         /* pairs per batch */
     int max_width;         /* largest frame: sizes the default arena */
     int max_height;
     int fp16_planes;       /* 1: store XYB scales 0 and 1 in FP16, scales 2-5 in FP32 (FP32 math, FP64 sums) */
     uint64_t arena_bytes;  /* 0: max_pairs * Vship_SSIMU2PairFootprint(max_width, max_height, fp16_planes) */
-    int map_tile_height;   /* 0: tuned shape if recorded, else 48; or 48, 64, 112 rows per map tile */
+    int map_tile_height;   /* 0: tuned shape if recorded, else VSHIP_MAP_MFMA; VSHIP_MAP_MFMA: 64x64 tiles
+                              with the blur on the matrix cores; 48, 64, 112: 32-column tiles of that many
+                              rows on the vector ALUs */
     int persistent_grid;   /* 0: tuned shape if recorded, else one workgroup per tile; or a fixed grid */
     int use_graph;         /* 1: capture the batch launch into a hipGraph once and replay it */
     int profile;           /* 1: time each kernel with HIP events (eager launches only) */
@@ -103,12 +107,12 @@ typedef struct {
 } Vship_SSIMU2BatchStats;
 
 /* ---- autotuner -------------------------------------------------------------
- * Times the batch on the sample for every map tile height (48, 64, 112),
+ * Times the batch on the sample for every map kernel shape (VSHIP_MAP_MFMA; 48, 64, 112),
  * persistent grid (one workgroup per tile, 2/4/8 x CU count) and graph
  * capture, returns the table and the fastest shape, and records it for the
  * sample's frame-size class: Vship_SSIMU2BatchInit then uses it when
  * map_tile_height and persistent_grid are 0. */
-#define VSHIP_TUNE_MAX_CANDIDATES 24
+#define VSHIP_TUNE_MAX_CANDIDATES 32
 typedef struct {
     int map_tile_height;
     int persistent_grid;

@@ -1,0 +1,93 @@
+"""The matrix-core map kernel (map_tile_height = 1: 64x64 tiles, both blur
+passes on v_mfma_f32_16x16x32_bf16 with operands split exactly into three
+bf16 parts) against the FP64 oracle (|dscore| <= 1e-3, features as in the
+parity tests) and against the vector-ALU kernel (the same FP32 maps to
+rounding: |dscore| <= 1e-4)."""
+import numpy as np
+import pytest
+
+import vship_amd
+from vship_amd import api, synth
+from tests import oracle_lib as ol
+from tests.gpu_helpers import FMT_NAMES, gpu_batch, oracle_rgb, rgb_pair, say
+
+pytestmark = pytest.mark.gpu
+
+MFMA = 1
+SCORE_TOL = 1e-3
+
+
+def feat_close(g, o):
+    return np.all(np.abs(g - o) <= 2e-5 + 1e-3 * np.abs(o))
+
+
+@pytest.fixture(scope="module", autouse=True)
+def gpu_ready():
+    vship_amd.gpu_full_check(0)
+    yield
+
+
+@pytest.mark.parametrize("fmt", [api.PF_RGB_U8, api.PF_RGB_U16, api.PF_RGB_F16, api.PF_RGB_F32])
+def test_mfma_sizes_and_formats(fmt):
+    # sizes with FP32-class segments (>= 32768 px) that are multiples of the
+    # 64x64 tile, odd, thin, and wider than tall
+    for i, (w, h) in enumerate([(256, 256), (300, 211), (193, 177), (1031, 33), (64, 600), (517, 389)]):
+        ref, dist = rgb_pair(w, h, fmt, 300 + i)
+        say(f"case: mfma map, {FMT_NAMES[fmt]} {w}x{h}")
+        s_o, f_o = oracle_rgb(ref, dist, fmt)
+        (s_m,), (f_m,) = gpu_batch([(ref, dist)], fmt, th=MFMA)
+        (s_v,), (f_v,) = gpu_batch([(ref, dist)], fmt, th=48)
+        say(f"  oracle {s_o:.7f} mfma {s_m:.7f} ({s_m - s_o:+.1e}) valu {s_v:.7f} ({s_v - s_o:+.1e})")
+        assert abs(s_m - s_o) <= SCORE_TOL, (w, h, s_m, s_o)
+        assert feat_close(f_m, f_o), (w, h, np.max(np.abs(f_m - f_o)))
+        assert abs(s_m - s_v) <= 1e-4
+
+
+def test_mfma_identical_inputs_score_exactly_100():
+    ref, _ = rgb_pair(333, 250, api.PF_RGB_F32, 9)
+    say("case: mfma map, identical 333x250 images")
+    (s,), (f,) = gpu_batch([(ref, ref)], api.PF_RGB_F32, th=MFMA)
+    assert s == 100.0
+    assert np.all(f == 0)
+
+
+def test_mfma_yuv_1080p_and_10bit():
+    clip = synth.video_clip(1920, 1080, 8, 240, 1)
+    pairs = [clip.frame(f) for f in (0, 77)]
+    say("case: mfma map, 1920x1080 YUV420P8 frames 0 and 77")
+    s_m = gpu_batch(pairs, api.PF_YUV420_U8, th=MFMA, features=False)
+    for k, (r, d) in enumerate(pairs):
+        s_o, _ = ol.score_yuv420(r, d, 8, ol.SO_MAT_BT709)
+        say(f"  oracle {s_o:.7f} mfma {s_m[k]:.7f} ({s_m[k] - s_o:+.1e})")
+        assert abs(s_m[k] - s_o) <= SCORE_TOL
+    clip10 = synth.video_clip(960, 540, 10, 3, 4)
+    r, d = clip10.frame(2)
+    say("case: mfma map, 960x540 YUV420P10 BT.2020, FP32 and FP16 planes")
+    s_o, _ = ol.score_yuv420(r, d, 10, ol.SO_MAT_BT2020NCL)
+    (s32,) = gpu_batch([(r, d)], api.PF_YUV420_U16, bits=10, matrix=api.MATRIX_BT2020NCL, th=MFMA, features=False)
+    (s16,) = gpu_batch([(r, d)], api.PF_YUV420_U16, bits=10, matrix=api.MATRIX_BT2020NCL, th=MFMA, fp16=True,
+                       features=False)
+    say(f"  oracle {s_o:.7f} mfma fp32 planes {s32:.7f} fp16 planes {s16:.7f}")
+    assert abs(s32 - s_o) <= SCORE_TOL
+    assert abs(s16 - s_o) <= SCORE_TOL
+
+
+def test_mfma_batch_grid_and_graph_are_bit_exact():
+    pairs = [rgb_pair(w, h, api.PF_RGB_U16, 40 + i) for i, (w, h) in enumerate([(400, 300), (190, 190), (640, 360)])]
+    dp = api.DevicePairs(pairs, api.PF_RGB_U16)
+    say("case: mfma map, uneven batch of 3: alone vs batch vs persistent grid vs graph replays")
+    alone = []
+    one = (api.Pair * 1)()
+    with api.SSIMU2Batch(1, 640, 360, map_tile_height=MFMA) as b:
+        for i in range(3):
+            one[0] = dp.descs[i]
+            alone.append(b.compute(one, 1)[0])
+    alone = np.array(alone)
+    with api.SSIMU2Batch(3, 640, 360, map_tile_height=MFMA) as b:
+        assert np.array_equal(b.compute(dp.descs), alone)
+    with api.SSIMU2Batch(3, 640, 360, map_tile_height=MFMA, persistent_grid=300) as b:
+        assert np.array_equal(b.compute(dp.descs), alone)
+    with api.SSIMU2Batch(3, 640, 360, map_tile_height=MFMA, use_graph=True) as b:
+        for _ in range(3):
+            assert np.array_equal(b.compute(dp.descs), alone)
+    dp.free()

@@ -77,7 +77,7 @@ class TuneCandidate(ctypes.Structure):
 
 class TuneResult(ctypes.Structure):
     _fields_ = [("map_tile_height", ctypes.c_int), ("persistent_grid", ctypes.c_int), ("use_graph", ctypes.c_int),
-                ("best_ms", ctypes.c_float), ("ncandidates", ctypes.c_int), ("candidates", TuneCandidate * 24)]
+                ("best_ms", ctypes.c_float), ("ncandidates", ctypes.c_int), ("candidates", TuneCandidate * 32)]
 
 
 class ServerConfig(ctypes.Structure):

@@ -99,7 +99,7 @@ int prep_tiles(int w, int h) { return 2 * ((w + kPrepTile - 1) / kPrepTile) * ((
 int map_tiles(int w, int h, int th) {
     int n = 0, ww = w, hh = h;
     for (int s = 0; s < kScales; s++) {
-        n += ((ww + kMapTW - 1) / kMapTW) * ((hh + th - 1) / th);
+        n += ((ww + map_tile_w(th) - 1) / map_tile_w(th)) * ((hh + map_tile_h(th) - 1) / map_tile_h(th));
         ww = (ww - 1) / 2 + 1;
         hh = (hh - 1) / 2 + 1;
     }
@@ -129,7 +129,7 @@ uint64_t prep_algorithmic_bytes(const PairInput& p, bool fp16) {
 
 Engine::Engine(const EngineOptions& opt) : opt_(opt) {
     if (opt_.max_pairs < 1) VSHIP_THROW(E_BadPointer);
-    if (opt_.map_th != 48 && opt_.map_th != 64 && opt_.map_th != 112) opt_.map_th = 48;
+    if (opt_.map_th != 48 && opt_.map_th != 64 && opt_.map_th != 112 && opt_.map_th != kMapMfma) opt_.map_th = 48;
     HIP_OR_THROW(hipSetDevice(opt_.device), E_BadDeviceArgument);
     HIP_OR_THROW(hipDeviceGetAttribute(&num_cus_, hipDeviceAttributeMultiprocessorCount, opt_.device),
                  E_BadDeviceCode);
@@ -256,7 +256,8 @@ void Engine::plan(const PairInput* in, int n) {
                 if (((int64_t)sw * sh < kSmallSegment ? 1 : 0) != cls) continue;
                 if ((opt_.fp16 && s < kHalfScales) != half) continue;
                 const int th = cls ? kSmallTH : opt_.map_th;
-                const int64_t cnt = (int64_t)((sw + kMapTW - 1) / kMapTW) * ((sh + th - 1) / th);
+                const int tw = map_tile_w(th), tht = map_tile_h(th);
+                const int64_t cnt = (int64_t)((sw + tw - 1) / tw) * ((sh + tht - 1) / tht);
                 const int seg = i * kScales + s;
                 cls_bytes_[cls] += (uint64_t)sw * sh * 2 * 3 * scale_elem_bytes(s, opt_.fp16);
                 seg_start[seg] = (int32_t)mtiles;
@@ -322,9 +323,13 @@ void Engine::enqueue_body() {
             if (half && !opt_.fp16) continue;
             if (!graph && ntiles_grp_[g] == 0) continue;
             const int grid = grid_for(ntiles_grp_[g], cls == 0 ? fixed : std::max(1, fixed / 4));
-            HIP_OR_THROW(launch_map(half, cls, cls ? kSmallTH : opt_.map_th, grid, pd, order_seg, order_start, hdr,
-                                    arena_d_, partials_d_, taps_, stream_),
-                         E_BadDeviceCode);
+            if (cls == 0 && opt_.map_th == kMapMfma)
+                HIP_OR_THROW(launch_map_mx(half, grid, pd, order_seg, order_start, hdr, arena_d_, partials_d_, stream_),
+                             E_BadDeviceCode);
+            else
+                HIP_OR_THROW(launch_map(half, cls, cls ? kSmallTH : opt_.map_th, grid, pd, order_seg, order_start,
+                                        hdr, arena_d_, partials_d_, taps_, stream_),
+                             E_BadDeviceCode);
         }
         if (prof) HIP_OR_THROW(hipEventRecord(ev_[3 + cls], stream_), E_BadDeviceCode);
     }

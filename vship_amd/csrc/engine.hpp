@@ -64,7 +64,7 @@ struct EngineOptions {
     int64_t arena_bytes = 0;   // XYB arena capacity
     int64_t max_map_tiles = 0;  // 0: derived from the arena for the smallest tile count per element
     bool fp16 = false;
-    int map_th = 48;       // FP32-class map tile height: 48 or 112
+    int map_th = 48;       // FP32-class map tiles: 32 x (48, 64 or 112) on the vector ALUs, or kMapMfma
     int persistent_grid = 0;   // 0: grid = tile count
     bool use_graph = false;
     bool profile = false;

@@ -68,7 +68,9 @@ public:
         o.device = device;
         o.max_pairs = cfg_.max_pairs_per_step;
         o.fp16 = cfg_.fp16_planes != 0;
-        o.map_th = cfg_.map_tile_height == 64 || cfg_.map_tile_height == 112 ? cfg_.map_tile_height : 48;
+        o.map_th = cfg_.map_tile_height == 64 || cfg_.map_tile_height == 112 || cfg_.map_tile_height == kMapMfma
+                       ? cfg_.map_tile_height
+                       : default_map_th(cfg_.max_width, cfg_.max_height);
         // the budget covers the arenas of both in-flight steps
         o.arena_bytes = (int64_t)(cfg_.memory_budget / 2);
         o.max_map_tiles = (int64_t)cfg_.max_pairs_per_step * map_tiles(cfg_.max_width, cfg_.max_height, 32);

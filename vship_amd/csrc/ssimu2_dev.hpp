@@ -65,6 +65,12 @@ __host__ __device__ constexpr int map_group(int cls, bool half) { return 2 * cls
 
 constexpr int64_t kSmallSegment = 32768;
 constexpr int kSmallTH = 48;
+// Tile height value that selects the matrix-core map kernel for the FP32
+// class; its tiles are kMxTile x kMxTile.
+constexpr int kMapMfma = 1;
+constexpr int kMxTile = 64;
+__host__ __device__ constexpr int map_tile_w(int th) { return th == kMapMfma ? kMxTile : kMapTW; }
+__host__ __device__ constexpr int map_tile_h(int th) { return th == kMapMfma ? kMxTile : th; }
 
 // The 17-tap Gaussian (sigma 1.5, radius 8) of src/ssimu2/gaussianblur.hpp:7-18
 // and src/util/preprocessor.hpp:89-92, normalised to sum 1 and rounded to

@@ -101,7 +101,7 @@ Vship_Exception Vship_SSIMU2BatchTune(const Vship_SSIMU2Pair* pairs, int npairs,
             mh = std::max(mh, p.height);
             arena += pair_bytes(p.width, p.height, fp16_planes != 0);
         }
-        const int ths[3] = {48, 64, 112};
+        const int ths[4] = {kMapMfma, 48, 64, 112};
         const int grids[4] = {0, 2 * cus, 4 * cus, 8 * cus};
         std::memset(result, 0, sizeof(*result));
         result->best_ms = 1e30f;
@@ -117,7 +117,7 @@ Vship_Exception Vship_SSIMU2BatchTune(const Vship_SSIMU2Pair* pairs, int npairs,
                     o.fp16 = fp16_planes != 0;
                     o.map_th = th;
                     o.arena_bytes = arena;
-                    o.max_map_tiles = (int64_t)npairs * map_tiles(mw, mh, 32);
+                    o.max_map_tiles = (int64_t)npairs * map_tiles(mw, mh, 32);  // bound for every shape
                     o.persistent_grid = grids[gi];
                     o.use_graph = graph != 0;
                     Engine e(o);
@@ -181,7 +181,7 @@ Vship_Exception Vship_SSIMU2TuneLoad(const char* path) {
         TuneEntry e;
         if (std::sscanf(line, "%d %d %d %d %d %lf", &e.size_class, &e.fp16, &e.th, &e.grid, &e.graph,
                         &e.ms_per_pair) == 6) {
-            if (e.th != 48 && e.th != 64 && e.th != 112) continue;
+            if (e.th != kMapMfma && e.th != 48 && e.th != 64 && e.th != 112) continue;
             record_shape(e);
         }
     }

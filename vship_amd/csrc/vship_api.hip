@@ -105,8 +105,9 @@ FrameHandle::~FrameHandle() {
 }
 
 // A size-independent default keeps a pair's result independent of the batch
-// it is in (the tile decomposition, and so the per-tile centring, is fixed).
-int default_map_th(int, int) { return 48; }
+// it is in (the tile decomposition, and so the per-tile centring, is fixed):
+// the matrix-core map kernel.
+int default_map_th(int, int) { return kMapMfma; }
 
 double FrameHandle::compute(int fmt, const uint8_t* const p1[3], const uint8_t* const p2[3], int64_t s1, int64_t s2) {
     if (!p1 || !p2) VSHIP_THROW(E_BadPointer);
@@ -410,7 +411,8 @@ Vship_Exception Vship_SSIMU2BatchInit(Vship_SSIMU2BatchHandler* handler, const V
         o.device = dev;
         o.max_pairs = cfg->max_pairs;
         o.fp16 = cfg->fp16_planes != 0;
-        o.map_th = cfg->map_tile_height == 48 || cfg->map_tile_height == 64 || cfg->map_tile_height == 112
+        o.map_th = cfg->map_tile_height == 48 || cfg->map_tile_height == 64 || cfg->map_tile_height == 112 ||
+                           cfg->map_tile_height == kMapMfma
                        ? cfg->map_tile_height
                        : default_map_th(cfg->max_width, cfg->max_height);
         o.persistent_grid = cfg->persistent_grid;

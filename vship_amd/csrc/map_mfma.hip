@@ -131,12 +131,16 @@ __global__ __launch_bounds__(256, 2) void map_mx_kernel(const PairDev* __restric
                                                         double* __restrict__ partials) {
     constexpr int T = kMxTile;
     constexpr int grp = map_group(0, sizeof(ST) == 2);
-    // ring[wave][slot][moment][part][column][8 words = 16 bf16 rows]
+    // ring[wave][slot][moment][part][column][8 words = 16 bf16 rows]; the two
+    // 4-word halves of a column are swapped for columns with swz = 4, so that
+    // the 8- and 16-lane groups of one LDS access (columns j .. j+7 or j+15 at
+    // a stride of 8 words) fall on distinct banks
     __shared__ uint32_t ring[4][2][4][3][16][8];
     __shared__ double red[4][kFeat];
     __shared__ float cum[18];
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     const int g = lane >> 4, j = lane & 15;
+    const int swz = (((j >> 2) ^ (j >> 3)) & 1) << 2;
     if (tid == 0) {
         float a = 0.f;
         cum[0] = 0.f;
@@ -197,6 +201,13 @@ __global__ __launch_bounds__(256, 2) void map_mx_kernel(const PairDev* __restric
         const int xi = xs0 - 8 + 8 * g;       // first of this lane's eight input columns
         const bool xin = (xs0 - 8 >= 0) && (xs0 + 24 <= W);  // every input column of the strip in the image
         const float invh = __builtin_amdgcn_rcpf(tap_total(cum, xo, W));
+        // renormalisation of this lane's 16 output pixels (rows y0 + 16 ob + 4g + r), channel independent
+        float inv[4][4];
+#pragma unroll
+        for (int ob = 0; ob < 4; ob++)
+#pragma unroll
+            for (int r = 0; r < 4; r++)
+                inv[ob][r] = invh * __builtin_amdgcn_rcpf(tap_total(cum, min(y0 + 16 * ob + 4 * g + r, H - 1), H));
 
 #pragma unroll 1
         for (int c = 0; c < 3; c++) {
@@ -255,11 +266,11 @@ __global__ __launch_bounds__(256, 2) void map_mx_kernel(const PairDev* __restric
                     float h[4], md[4], l[4];
 #pragma unroll
                     for (int r = 0; r < 4; r++) split3(hr[m][r], h[r], md[r], l[r]);
-                    *reinterpret_cast<uint2*>(&ring[wave][slot][m][0][j][2 * g]) =
+                    *reinterpret_cast<uint2*>(&ring[wave][slot][m][0][j][(2 * g) ^ swz]) =
                         make_uint2(pack2(h[0], h[1]), pack2(h[2], h[3]));
-                    *reinterpret_cast<uint2*>(&ring[wave][slot][m][1][j][2 * g]) =
+                    *reinterpret_cast<uint2*>(&ring[wave][slot][m][1][j][(2 * g) ^ swz]) =
                         make_uint2(pack2(md[0], md[1]), pack2(md[2], md[3]));
-                    *reinterpret_cast<uint2*>(&ring[wave][slot][m][2][j][2 * g]) =
+                    *reinterpret_cast<uint2*>(&ring[wave][slot][m][2][j][(2 * g) ^ swz]) =
                         make_uint2(pack2(l[0], l[1]), pack2(l[2], l[3]));
                 }
                 wave_lds_sync();
@@ -267,7 +278,7 @@ __global__ __launch_bounds__(256, 2) void map_mx_kernel(const PairDev* __restric
 
                 // ---- column pass of output rows y0 + 16 ob .. +15 from ring blocks ob, ob + 1
                 const int ob = mb - 1;
-                const int rs = (ob + (g >> 1)) & 1, rw = 4 * (g & 1);
+                const int rs = (ob + (g >> 1)) & 1, rw = (4 * (g & 1)) ^ swz;
                 f32x4 o[4];
 #pragma unroll
                 for (int m = 0; m < 4; m++) {
@@ -279,11 +290,14 @@ __global__ __launch_bounds__(256, 2) void map_mx_kernel(const PairDev* __restric
                 }
                 wave_lds_sync();
                 // ---- maps of this lane's output pixels: rows y0 + 16 ob + 4g + r, column xo
+                float ivr[4];
+#pragma unroll
+                for (int r = 0; r < 4; r++) ivr[r] = ob == 0 ? inv[0][r] : ob == 1 ? inv[1][r] : ob == 2 ? inv[2][r] : inv[3][r];
 #pragma unroll
                 for (int r = 0; r < 4; r++) {
                     const int y = y0 + 16 * ob + 4 * g + r;
                     if (xo < W && y < H) {
-                        const float iv = invh * __builtin_amdgcn_rcpf(tap_total(cum, y, H));
+                        const float iv = ivr[r];
                         const float mu1 = o[0][r] * iv, mu2 = o[1][r] * iv;
                         const float E2 = o[2][r] * iv, Q = o[3][r] * iv;
                         const float md = mu1 - mu2;

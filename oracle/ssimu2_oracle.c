@@ -204,13 +204,15 @@ double so_final_score(const double f[108]) {
 /*     (-3C[i-1] + 29C[i] + 111C[i+1] - 9C[i+2]) / 128                    */
 /*  limited range: Y' = (Y - 16s)/(219s), C' = (C - 128s)/(224s),         */
 /*  s = 2^(bits-8); matrix: R = Y' + 2(1-Kr)Cr', B = Y' + 2(1-Kb)Cb',      */
-/*  G = (Y' - Kr R - Kb B)/(1 - Kr - Kb); no clamping.                     */
+/*  G = (Y' - Kr R - Kb B)/(1 - Kr - Kb); no clamping of the result.      */
+/*  16-bit-container samples above 2^bits - 1 are clamped to it.          */
 /* ---------------------------------------------------------------------- */
 static double so_sample(const uint8_t* p, int64_t stride, int bits, int x, int y) {
     if (bits <= 8) return (double)p[(int64_t)y * stride + x];
     uint16_t u;
     memcpy(&u, p + (int64_t)y * stride + 2 * (int64_t)x, 2);
-    return (double)u;
+    const unsigned maxcode = (1u << bits) - 1u;
+    return (double)(u > maxcode ? maxcode : u);
 }
 
 static void so_matrix_coefs(int matrix, double* kr, double* kb) {

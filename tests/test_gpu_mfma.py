@@ -91,3 +91,41 @@ def test_mfma_batch_grid_and_graph_are_bit_exact():
         for _ in range(3):
             assert np.array_equal(b.compute(dp.descs), alone)
     dp.free()
+
+
+def test_mfma_extreme_yuv_codes_stay_in_range():
+    # uniform noise over every 8-bit code in all three planes reaches the
+    # extremes of the XYB range the scaled fp16 split is sized for
+    rng = np.random.default_rng(77)
+    w, h = 320, 240
+    mk = lambda: [rng.integers(0, 256, (h, w)).astype(np.uint8),
+                  rng.integers(0, 256, (h // 2, w // 2)).astype(np.uint8),
+                  rng.integers(0, 256, (h // 2, w // 2)).astype(np.uint8)]
+    pairs = [(mk(), mk()) for _ in range(2)]
+    smooth = synth.video_clip(w, h, 8, 1, 3).frame(0)
+    pairs.append((pairs[0][0], smooth[1]))  # extreme reference against a natural frame
+    say("case: mfma map, 320x240 YUV420P8 with uniformly random codes (extreme XYB)")
+    s_m = gpu_batch(pairs, api.PF_YUV420_U8, th=MFMA, features=False)
+    s_v = gpu_batch(pairs, api.PF_YUV420_U8, th=48, features=False)
+    for k, (r, d) in enumerate(pairs):
+        s_o, _ = ol.score_yuv420(r, d, 8, ol.SO_MAT_BT709)
+        say(f"  oracle {s_o:.6f} mfma {s_m[k]:.6f} ({s_m[k] - s_o:+.1e}) valu {s_v[k]:.6f}")
+        assert np.isfinite(s_m[k])
+        assert abs(s_m[k] - s_o) <= SCORE_TOL
+        assert abs(s_m[k] - s_v[k]) <= 1e-4
+
+
+def test_mfma_yuv10_codes_above_depth_are_clamped():
+    clip = synth.video_clip(320, 240, 10, 1, 6)
+    r, d = clip.frame(0)
+    wild = [p.copy() for p in d]
+    wild[0][10:20, 10:20] = 65535
+    wild[1][5:9, 5:9] = 3000
+    clamped = [np.minimum(p, 1023) for p in wild]
+    say("case: mfma map, 320x240 YUV420P10 with codes above 1023")
+    s = gpu_batch([(r, wild), (r, clamped)], api.PF_YUV420_U16, bits=10, matrix=api.MATRIX_BT2020NCL, th=MFMA,
+                  features=False)
+    s_o, _ = ol.score_yuv420(r, wild, 10, ol.SO_MAT_BT2020NCL)
+    say(f"  wild {s[0]:.6f} clamped {s[1]:.6f} oracle {s_o:.6f}")
+    assert s[0] == s[1]
+    assert abs(s[0] - s_o) <= SCORE_TOL

@@ -415,3 +415,23 @@ def test_yuv_catmull_rom_random_interior():
                     rx = min(max(k0 - 1 + b, 0), cw - 1)
                     acc += wy[a] * wx[xx % 2][b] * u[ry, rx]
             assert cb[yy, xx] == pytest.approx(acc, abs=1e-9)
+
+
+def test_yuv16_samples_above_depth_are_clamped():
+    # 16-bit-container samples above the declared depth's largest code are
+    # clamped to it (the front end's definition; it bounds the XYB range)
+    rng = np.random.default_rng(5)
+    w, h = 24, 16
+    planes = [rng.integers(64, 941, (h, w)).astype(np.uint16),
+              rng.integers(64, 961, (h // 2, w // 2)).astype(np.uint16),
+              rng.integers(64, 961, (h // 2, w // 2)).astype(np.uint16)]
+    wild = [p.copy() for p in planes]
+    wild[0][3, 5] = 65535
+    wild[1][2, 2] = 4000
+    clamped = [p.copy() for p in wild]
+    for p in clamped:
+        np.minimum(p, 1023, out=p)
+    a = ol.yuv_to_rgb(wild, 10, ol.SO_MAT_BT2020NCL)
+    b = ol.yuv_to_rgb(clamped, 10, ol.SO_MAT_BT2020NCL)
+    assert np.array_equal(a, b)
+    assert not np.array_equal(a, ol.yuv_to_rgb(planes, 10, ol.SO_MAT_BT2020NCL))

@@ -247,12 +247,16 @@ void Engine::plan(const PairInput* in, int n) {
     int nseg_grp[kMapGroups] = {}, tiles_grp[kMapGroups] = {};
     for (int& c : ntiles_cls_) c = 0;
     for (uint64_t& b : cls_bytes_) b = 0;
+    const bool mfma = opt_.map_th == kMapMfma;
     for (int g = 0; g < kMapGroups; g++) {
-        const int cls = g / 2;
+        const int cls = g < 4 ? g / 2 : 0;
         const bool half = g & 1;
+        const bool exact = g >= 4;
         for (int i = 0; i < n; i++)
             for (int s = 0; s < kScales; s++) {
                 const int sw = pd[i].sw[s], sh = pd[i].sh[s];
+                const bool float_in = pd[i].fmt == IN_RGB_F16 || pd[i].fmt == IN_RGB_F32;
+                if ((mfma && cls == 0 && float_in) != exact) continue;
                 if (((int64_t)sw * sh < kSmallSegment ? 1 : 0) != cls) continue;
                 if ((opt_.fp16 && s < kHalfScales) != half) continue;
                 const int th = cls ? kSmallTH : opt_.map_th;
@@ -282,7 +286,7 @@ void Engine::plan(const PairInput* in, int n) {
         hdr->nseg[g] = nseg_grp[g];
         hdr->tiles[g] = tiles_grp[g];
         ntiles_grp_[g] = tiles_grp[g];
-        ntiles_cls_[g / 2] += tiles_grp[g];
+        ntiles_cls_[g < 4 ? g / 2 : 0] += tiles_grp[g];
     }
     npairs_ = n;
     ntiles_prep_ = (int)ptiles;
@@ -317,20 +321,25 @@ void Engine::enqueue_body() {
     if (prof) HIP_OR_THROW(hipEventRecord(ev_[2], stream_), E_BadDeviceCode);
     // one launch per map group; FP16-plane groups exist only with FP16 storage,
     // and an eager launch is skipped when its group is empty
+    const bool mfma = opt_.map_th == kMapMfma;
     for (int cls = 0; cls < 2; cls++) {
-        for (int half = 0; half < 2; half++) {
-            const int g = map_group(cls, half);
-            if (half && !opt_.fp16) continue;
-            if (!graph && ntiles_grp_[g] == 0) continue;
-            const int grid = grid_for(ntiles_grp_[g], cls == 0 ? fixed : std::max(1, fixed / 4));
-            if (cls == 0 && opt_.map_th == kMapMfma)
-                HIP_OR_THROW(launch_map_mx(half, grid, pd, order_seg, order_start, hdr, arena_d_, partials_d_, stream_),
-                             E_BadDeviceCode);
-            else
-                HIP_OR_THROW(launch_map(half, cls, cls ? kSmallTH : opt_.map_th, grid, pd, order_seg, order_start,
-                                        hdr, arena_d_, partials_d_, taps_, stream_),
-                             E_BadDeviceCode);
-        }
+        // on the matrix cores, class 0 also has the exact-split groups of float-input pairs
+        const int nexact = (cls == 0 && mfma) ? 2 : 1;
+        for (int exact = 0; exact < nexact; exact++)
+            for (int half = 0; half < 2; half++) {
+                const int g = map_group(cls, half, exact);
+                if (half && !opt_.fp16) continue;
+                if (!graph && ntiles_grp_[g] == 0) continue;
+                const int grid = grid_for(ntiles_grp_[g], cls == 0 ? fixed : std::max(1, fixed / 4));
+                if (cls == 0 && mfma)
+                    HIP_OR_THROW(launch_map_mx(half, exact, grid, pd, order_seg, order_start, hdr, arena_d_,
+                                               partials_d_, stream_),
+                                 E_BadDeviceCode);
+                else
+                    HIP_OR_THROW(launch_map(half, cls, cls ? kSmallTH : opt_.map_th, grid, pd, order_seg, order_start,
+                                            hdr, arena_d_, partials_d_, taps_, stream_),
+                                 E_BadDeviceCode);
+            }
         if (prof) HIP_OR_THROW(hipEventRecord(ev_[3 + cls], stream_), E_BadDeviceCode);
     }
     double* scores_d = out_d_;

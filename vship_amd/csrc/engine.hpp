@@ -81,7 +81,7 @@ struct PairLayout {
     int64_t img_bytes;
 };
 
-constexpr int kMapGroups = 4;  // (FP32 | FP64 moments) x (FP32 | FP16 planes)
+constexpr int kMapGroups = 6;  // (FP32 | FP64 moments) x (FP32 | FP16 planes), + exact-split FP32 moments
 inline int scale_elem_bytes(int s, bool fp16) { return fp16 && s < kHalfScales ? 2 : 4; }
 PairLayout pair_layout(int w, int h, bool fp16);
 int64_t pair_bytes(int w, int h, bool fp16);  // arena bytes of both images

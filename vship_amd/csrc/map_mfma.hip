@@ -4,26 +4,42 @@
 // the four moment planes a, b, (x1 - x2)^2, ab, then the SSIM, artifact and
 // detail-loss maps and their 1-/4-norm sums; src/ssimu2/gaussianblur.hpp:24-92,
 // src/ssimu2/score.hpp:95-196), with both blur passes done as matrix products
-// on v_mfma_f32_16x16x32_bf16:
+// on the 16x16x32 MFMA instructions:
 //
 //   row pass    Hr(16 rows x 16 cols) = In(16 rows x 32 input cols) . G(32 x 16)
 //   column pass Out(16 rows x 16 cols) = G'(16 x 32 input rows) . Hr(32 rows x 16 cols)
 //
 // where G[k][j] = g[k - j] (zero outside 0..16) is the band of taps; both
-// passes use the same per-lane tap operand. Every operand x is split exactly
-// into three bf16 parts x = h + m + l (truncation: h keeps the top 8
-// significant bits of x, m the next 8, l the last 8), and the six partial
-// products whose weight exceeds 2^-24 are accumulated in FP32:
-// hh + hm + mh + hl + mm + lh. That is FP32 accuracy with FP32 range, so the
-// maps are those of the FP32 vector kernel to rounding.
+// passes use the same per-lane tap operand. The operands are split into
+// 16-bit parts and the partial products accumulate in FP32, one of two ways:
+//
+//   SplitBf16x3 (planar float RGB input, whose values are unbounded): every
+//     operand x is split exactly into three bf16 parts x = h + m + l
+//     (truncation: h keeps the top 8 significant bits, m the next 8, l the
+//     last 8); the six products above 2^-24 are accumulated (hh + hm + mh +
+//     hl + mm + lh). FP32 accuracy, FP32 range.
+//   SplitF16x2 (8- to 16-bit RGB and YUV input): operands are scaled by a
+//     power of two (a, b: 2^11, (x1 - x2)^2: 2^6, ab: 2^6, taps: 2^14) and
+//     split into two fp16 parts h + l (22 significant bits); all four
+//     products hh + hl + lh + ll are accumulated, so each product of the two
+//     representations is exact. The taps are the values the fp16 pair
+//     represents exactly (within 2^-22 of the FP32 taps), and the
+//     renormalisation sums use the same values. The scales keep every value
+//     inside the fp16 range for any input of these formats: their R'G'B' lies
+//     in [-0.97, 1.98] (limited-range YUV with samples clamped to the
+//     declared depth), so XYB lies in [-11.7, 12.5] x [-0.15, 1.58] x
+//     [-1.18, 2.28], |a|, |b|, |x1 - x2| <= 24.2 and |ab| <= 586. A CPU
+//     emulation of this arithmetic on real frames puts the score within
+//     1e-5 of the FP64 maps (typically 1e-6), at about half the splitting
+//     work of the exact split and two thirds of its matrix products.
 //
 // Tile: 64 x 64 output pixels of one scale; each of the four waves owns a
 // 16-column strip (80 input rows x 32 input columns) and needs nothing from
 // the other waves until the tile sums. A wave streams five 16-row blocks
-// through the row pass; the row-pass results go, split into bf16 parts, to a
-// two-block LDS ring of the wave, from which the column pass of output block
-// ob reads blocks ob and ob + 1. Each output pixel's maps are formed in the
-// registers the column pass leaves them in.
+// through the row pass; the row-pass results go, split, to a two-block LDS
+// ring of the wave, from which the column pass of output block ob reads
+// blocks ob and ob + 1. Each output pixel's maps are formed in the registers
+// the column pass leaves them in.
 #include <hip/hip_fp16.h>
 #include <hip/hip_runtime.h>
 
@@ -35,10 +51,15 @@ namespace vship {
 namespace {
 
 typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
+typedef _Float16 f16x8 __attribute__((ext_vector_type(8)));
+typedef _Float16 f16x2 __attribute__((ext_vector_type(2)));
+typedef float f32x2 __attribute__((ext_vector_type(2)));
 typedef float f32x4 __attribute__((ext_vector_type(4)));
 typedef unsigned int u32x4 __attribute__((ext_vector_type(4)));
 
-// Exact split of x into three bf16 values h + m + l.
+// ---------------------------------------------------------------------------
+// exact three-way bf16 split
+// ---------------------------------------------------------------------------
 __device__ __forceinline__ void split3(float x, float& h, float& m, float& l) {
     h = __uint_as_float(__float_as_uint(x) & 0xffff0000u);
     const float r = x - h;
@@ -52,40 +73,89 @@ __device__ __forceinline__ uint32_t pack2(float first, float second) {
     return __builtin_amdgcn_perm(__float_as_uint(second), __float_as_uint(first), 0x07060302u);
 }
 
-// Eight values as three packed bf16 operands.
-struct Split8 {
-    u32x4 h, m, l;
-};
-
-__device__ __forceinline__ Split8 split8(const float v[8]) {
-    Split8 s;
-#pragma unroll
-    for (int k = 0; k < 4; k++) {
-        float h0, m0, l0, h1, m1, l1;
-        split3(v[2 * k], h0, m0, l0);
-        split3(v[2 * k + 1], h1, m1, l1);
-        s.h[k] = pack2(h0, h1);
-        s.m[k] = pack2(m0, m1);
-        s.l[k] = pack2(l0, l1);
-    }
-    return s;
-}
-
-__device__ __forceinline__ f32x4 mfma(u32x4 a, u32x4 b, f32x4 c) {
+__device__ __forceinline__ f32x4 mfma_bf16(u32x4 a, u32x4 b, f32x4 c) {
     return __builtin_amdgcn_mfma_f32_16x16x32_bf16(__builtin_bit_cast(bf16x8, a), __builtin_bit_cast(bf16x8, b), c,
                                                    0, 0, 0);
 }
 
-// c + a.b from the six partial products above 2^-24, smallest first.
-__device__ __forceinline__ f32x4 mfma6(const Split8& a, const Split8& b, f32x4 c) {
-    c = mfma(a.l, b.h, c);
-    c = mfma(a.m, b.m, c);
-    c = mfma(a.h, b.l, c);
-    c = mfma(a.m, b.h, c);
-    c = mfma(a.h, b.m, c);
-    c = mfma(a.h, b.h, c);
-    return c;
+struct SplitBf16x3 {
+    static constexpr int kParts = 3;
+    static constexpr float kScaleAB = 1.f, kScaleDD = 1.f, kScaleQ = 1.f, kTapScale = 1.f;
+    struct Op {
+        u32x4 p[3];  // h, m, l
+    };
+    // N consecutive values (8 for an operand, 4 for a row-pass result)
+    template <int N>
+    static __device__ __forceinline__ void split(const float* v, uint32_t (&w)[3][N / 2]) {
+#pragma unroll
+        for (int k = 0; k < N / 2; k++) {
+            float h0, m0, l0, h1, m1, l1;
+            split3(v[2 * k], h0, m0, l0);
+            split3(v[2 * k + 1], h1, m1, l1);
+            w[0][k] = pack2(h0, h1);
+            w[1][k] = pack2(m0, m1);
+            w[2][k] = pack2(l0, l1);
+        }
+    }
+    // c + a.b from the six partial products above 2^-24, smallest first
+    static __device__ __forceinline__ f32x4 mma(const Op& a, const Op& b, f32x4 c) {
+        c = mfma_bf16(a.p[2], b.p[0], c);
+        c = mfma_bf16(a.p[1], b.p[1], c);
+        c = mfma_bf16(a.p[0], b.p[2], c);
+        c = mfma_bf16(a.p[1], b.p[0], c);
+        c = mfma_bf16(a.p[0], b.p[1], c);
+        c = mfma_bf16(a.p[0], b.p[0], c);
+        return c;
+    }
+    static __device__ __forceinline__ float tap_value(int k) { return tap(k); }
+};
+
+// ---------------------------------------------------------------------------
+// scaled two-way fp16 split
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ f32x4 mfma_f16(u32x4 a, u32x4 b, f32x4 c) {
+    return __builtin_amdgcn_mfma_f32_16x16x32_f16(__builtin_bit_cast(f16x8, a), __builtin_bit_cast(f16x8, b), c, 0,
+                                                  0, 0);
 }
+
+struct SplitF16x2 {
+    static constexpr int kParts = 2;
+    static constexpr float kScaleAB = 2048.f;        // a, b: 2^11
+    static constexpr float kScaleDD = 8.f;           // (x1 - x2)^2 as (8 (x1 - x2))^2: 2^6
+    static constexpr float kScaleQ = 1.f / 65536.f;  // ab from the scaled a, b: 2^22 / 2^16 = 2^6
+    static constexpr float kTapScale = 16384.f;      // taps: 2^14
+    struct Op {
+        u32x4 p[2];  // h, l
+    };
+    template <int N>
+    static __device__ __forceinline__ void split(const float* v, uint32_t (&w)[2][N / 2]) {
+#pragma unroll
+        for (int k = 0; k < N / 2; k++) {
+            const f32x2 x = {v[2 * k], v[2 * k + 1]};
+            const f16x2 h = __builtin_convertvector(x, f16x2);
+            const f32x2 r = x - __builtin_convertvector(h, f32x2);
+            const f16x2 l = __builtin_convertvector(r, f16x2);
+            w[0][k] = __builtin_bit_cast(uint32_t, h);
+            w[1][k] = __builtin_bit_cast(uint32_t, l);
+        }
+    }
+    // c + a.b: all four products of the two-part representations
+    static __device__ __forceinline__ f32x4 mma(const Op& a, const Op& b, f32x4 c) {
+        c = mfma_f16(a.p[1], b.p[1], c);
+        c = mfma_f16(a.p[1], b.p[0], c);
+        c = mfma_f16(a.p[0], b.p[1], c);
+        c = mfma_f16(a.p[0], b.p[0], c);
+        return c;
+    }
+    // The tap the fp16 pair represents exactly (the operand split of
+    // tap(k) * 2^14 reproduces it), divided back by 2^14.
+    static __device__ __forceinline__ float tap_value(int k) {
+        const float x = tap(k) * kTapScale;
+        const _Float16 h = (_Float16)x;
+        const _Float16 l = (_Float16)(x - (float)h);
+        return ((float)h + (float)l) * (1.f / kTapScale);
+    }
+};
 
 // Orders this wave's LDS accesses before and after (the wave's lanes exchange
 // data through LDS; no other wave touches the region).
@@ -120,45 +190,198 @@ struct Load8<__half> {
     }
 };
 
+// Shared memory of one workgroup, for a split into NP parts.
+template <int NP>
+struct MxLds {
+    // ring[wave][slot][moment][part][column][8 words = 16 rows of 16-bit parts];
+    // the two 4-word halves of a column are swapped for columns with swz = 4,
+    // so that the 8- and 16-lane groups of one LDS access (columns j .. j+7 or
+    // j+15 at a stride of 8 words) fall on distinct banks
+    uint32_t ring[4][2][4][NP][16][8];
+    double red[4][kFeat];
+    float cum[18];  // cumulative taps, as the split represents them
+};
+
+// Geometry of one tile, the same for every channel.
+struct TileGeo {
+    int W, H, x0, y0, xo, g, j, wave, swz;
+    int xl;     // column of this lane's 8-element input load
+    int cmask;  // bit e: input column xi + e lies in the image
+    int64_t pitch, pe;
+};
+
+// Tap operand of a split (row pass B, column pass A): element e of lane
+// (g, j) is tap[8g + e - j] at the split's tap scale.
+template <typename SP>
+__device__ __forceinline__ typename SP::Op tap_operand(int g, int j) {
+    float tv[8];
+#pragma unroll
+    for (int e = 0; e < 8; e++) {
+        const int k = 8 * g + e - j;
+        tv[e] = (k >= 0 && k <= 16) ? SP::tap_value(k) * SP::kTapScale : 0.f;
+    }
+    uint32_t w[SP::kParts][4];
+    SP::template split<8>(tv, w);
+    typename SP::Op G;
+#pragma unroll
+    for (int p = 0; p < SP::kParts; p++) G.p[p] = u32x4{w[p][0], w[p][1], w[p][2], w[p][3]};
+    return G;
+}
+
+// The three channels of one tile; leaves the 18 wave sums in red[wave].
+template <typename SP, typename ST>
+__device__ __forceinline__ void mx_tile(MxLds<SP::kParts>& L, const TileGeo& T, const ST* base1, const ST* base2,
+                                        const float* cum, int lane) {
+    constexpr int NP = SP::kParts;
+    const int W = T.W, H = T.H, g = T.g, j = T.j, wave = T.wave, swz = T.swz;
+    const typename SP::Op G = tap_operand<SP>(g, j);
+    const int64_t pitch = T.pitch;
+    // renormalisation of this lane's output column, with the scale of the
+    // products folded in; the row factor is formed per output pixel, which
+    // keeps the registers under the 128 of four workgroups per CU
+    constexpr float kOutScale = 1.f / (SP::kScaleAB * SP::kTapScale);
+    const float invh = __builtin_amdgcn_rcpf(tap_total(cum, T.xo, W)) * kOutScale;
+    // E2 carries kScaleDD^2, Q carries kScaleAB^2 kScaleQ, where a, b carry kScaleAB
+    constexpr float kEOut = SP::kScaleAB / (SP::kScaleDD * SP::kScaleDD);
+    constexpr float kQOut = 1.f / (SP::kScaleAB * SP::kScaleQ);
+    // the row-pass results are re-split at the data scale
+    constexpr float kMid = 1.f / SP::kTapScale;
+
+#pragma unroll 1
+    for (int c = 0; c < 3; c++) {
+        const ST* p1 = base1 + c * T.pe;
+        const ST* p2 = base2 + c * T.pe;
+        const int cxp = min(T.x0 + kMxTile / 2, W - 1), cyp = min(T.y0 + kMxTile / 2, H - 1);
+        const float cc = ldf(p1 + (int64_t)cyp * pitch + cxp);
+        const float ccs = cc * SP::kScaleAB;
+        float acc[6] = {0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
+
+#pragma unroll 1
+        for (int mb = 0; mb < 5; mb++) {
+            // ---- row pass of input rows y0 - 8 + 16 mb .. +15; this lane: row j, columns xi .. xi + 7
+            const int yr = T.y0 - 8 + 16 * mb + j;
+            typename SP::Op A[4];
+            {
+                // one vector load per image for every lane: the row is clamped
+                // into the image and the columns outside it are masked to zero
+                float u1[8], u2[8];
+                const bool row_ok = yr >= 0 && yr < H;
+                const int64_t ro = (int64_t)min(max(yr, 0), H - 1) * pitch + T.xl;
+                Load8<ST>::run(p1 + ro, u1);
+                Load8<ST>::run(p2 + ro, u2);
+                bool in[8];
+#pragma unroll
+                for (int e = 0; e < 8; e++) in[e] = row_ok && ((T.cmask >> e) & 1);
+                float v[4][8];
+#pragma unroll
+                for (int e = 0; e < 8; e++) {
+                    const float a = in[e] ? fmaf(u1[e], SP::kScaleAB, -ccs) : 0.f;
+                    const float b = in[e] ? fmaf(u2[e], SP::kScaleAB, -ccs) : 0.f;
+                    const float dd = in[e] ? (u1[e] - u2[e]) * SP::kScaleDD : 0.f;
+                    v[0][e] = a;
+                    v[1][e] = b;
+                    v[2][e] = dd * dd;
+                    v[3][e] = (a * SP::kScaleQ) * b;
+                }
+#pragma unroll
+                for (int m = 0; m < 4; m++) {
+                    uint32_t w[NP][4];
+                    SP::template split<8>(v[m], w);
+#pragma unroll
+                    for (int p = 0; p < NP; p++) A[m].p[p] = u32x4{w[p][0], w[p][1], w[p][2], w[p][3]};
+                }
+            }
+            const f32x4 z = {0.f, 0.f, 0.f, 0.f};
+            const int slot = mb & 1;
+#pragma unroll
+            for (int m = 0; m < 4; m++) {
+                const f32x4 hr = SP::mma(A[m], G, z);
+                // lane (g, j) holds rows 4g .. 4g+3 of the block, column j: store the split parts
+                const float rv[4] = {hr[0] * kMid, hr[1] * kMid, hr[2] * kMid, hr[3] * kMid};
+                uint32_t w[NP][2];
+                SP::template split<4>(rv, w);
+#pragma unroll
+                for (int p = 0; p < NP; p++)
+                    *reinterpret_cast<uint2*>(&L.ring[wave][slot][m][p][j][(2 * g) ^ swz]) = make_uint2(w[p][0], w[p][1]);
+            }
+            wave_lds_sync();
+            if (mb == 0) continue;
+
+            // ---- column pass of output rows y0 + 16 ob .. +15 from ring blocks ob, ob + 1
+            const int ob = mb - 1;
+            const int rs = (ob + (g >> 1)) & 1, rw = (4 * (g & 1)) ^ swz;
+            f32x4 o[4];
+#pragma unroll
+            for (int m = 0; m < 4; m++) {
+                typename SP::Op B;
+#pragma unroll
+                for (int p = 0; p < NP; p++) B.p[p] = *reinterpret_cast<const u32x4*>(&L.ring[wave][rs][m][p][j][rw]);
+                o[m] = SP::mma(G, B, z);
+            }
+            wave_lds_sync();
+            // ---- maps of this lane's output pixels: rows y0 + 16 ob + 4g + r, column xo
+#pragma unroll
+            for (int r = 0; r < 4; r++) {
+                const int y = T.y0 + 16 * ob + 4 * g + r;
+                if (T.xo < W && y < H) {
+                    const float iv = invh * __builtin_amdgcn_rcpf(tap_total(cum, y, H));
+                    const float mu1 = o[0][r] * iv, mu2 = o[1][r] * iv;
+                    const float E2 = o[2][r] * (iv * kEOut), Q = o[3][r] * (iv * kQOut);
+                    const float md = mu1 - mu2;
+                    const float md2 = md * md;
+                    const float vd = E2 - md2;  // var(x1 - x2)
+                    const float num_s = fmaf(2.f, Q - mu1 * mu2, 0.0009f);
+                    float d = fmaf(md2, num_s, vd) * __builtin_amdgcn_rcpf(vd + num_s);
+                    d = d > 0.f ? d : 0.f;
+                    const float a = ldf(p1 + (int64_t)y * pitch + T.xo) - cc;
+                    const float b = ldf(p2 + (int64_t)y * pitch + T.xo) - cc;
+                    const float fa = fabsf(a - mu1) + 1.f, fb = fabsf(b - mu2) + 1.f;
+                    const float ev = (fb - fa) * __builtin_amdgcn_rcpf(fa);  // src/ssimu2/score.hpp:150-154
+                    const float art = ev > 0.f ? ev : 0.f, det = ev < 0.f ? -ev : 0.f;
+                    const float d2 = d * d, a2 = art * art, t2 = det * det;
+                    acc[0] += d;
+                    acc[1] += d2 * d2;
+                    acc[2] += art;
+                    acc[3] += a2 * a2;
+                    acc[4] += det;
+                    acc[5] += t2 * t2;
+                }
+            }
+        }
+        // wave sums of this channel in FP64, fixed order
+#pragma unroll
+        for (int k = 0; k < 6; k++) {
+            double x = (double)acc[k];
+#pragma unroll
+            for (int off = 32; off > 0; off >>= 1) x += __shfl_xor(x, off, 64);
+            if (lane == 0) L.red[wave][c * 6 + k] = x;
+        }
+    }
+}
+
 }  // namespace
 
-template <typename ST>
-__global__ __launch_bounds__(256, 2) void map_mx_kernel(const PairDev* __restrict__ pairs,
+// Occupancy: the two-part split fits 4 workgroups per CU (33 KB of LDS, at
+// most 128 VGPRs), the three-part split 3 (49 KB, about 135 VGPRs).
+template <typename ST, typename SP>
+__global__ __launch_bounds__(256, SP::kParts == 2 ? 4 : 2) void map_mx_kernel(const PairDev* __restrict__ pairs,
                                                         const int32_t* __restrict__ order_seg,
                                                         const int32_t* __restrict__ order_start,
                                                         const BatchHdr* __restrict__ hdr,
                                                         const uint8_t* __restrict__ arena,
                                                         double* __restrict__ partials) {
-    constexpr int T = kMxTile;
-    constexpr int grp = map_group(0, sizeof(ST) == 2);
-    // ring[wave][slot][moment][part][column][8 words = 16 bf16 rows]; the two
-    // 4-word halves of a column are swapped for columns with swz = 4, so that
-    // the 8- and 16-lane groups of one LDS access (columns j .. j+7 or j+15 at
-    // a stride of 8 words) fall on distinct banks
-    __shared__ uint32_t ring[4][2][4][3][16][8];
-    __shared__ double red[4][kFeat];
-    __shared__ float cum[18];
+    constexpr int TS = kMxTile;
+    constexpr int grp = map_group(0, sizeof(ST) == 2, SP::kParts == 3);
+    __shared__ MxLds<SP::kParts> L;
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     const int g = lane >> 4, j = lane & 15;
-    const int swz = (((j >> 2) ^ (j >> 3)) & 1) << 2;
     if (tid == 0) {
         float a = 0.f;
-        cum[0] = 0.f;
+        L.cum[0] = 0.f;
         for (int k = 0; k < 17; k++) {
-            a += tap(k);
-            cum[k + 1] = a;
+            a += SP::tap_value(k);
+            L.cum[k + 1] = a;
         }
-    }
-    // tap operand (row pass B, column pass A): element e of lane (g, j) is g[8g + e - j]
-    Split8 G;
-    {
-        float tv[8];
-#pragma unroll
-        for (int e = 0; e < 8; e++) {
-            const int k = 8 * g + e - j;
-            tv[e] = (k >= 0 && k <= 16) ? tap(k) : 0.f;
-        }
-        G = split8(tv);
     }
     __syncthreads();
 
@@ -186,165 +409,56 @@ __global__ __launch_bounds__(256, 2) void map_mx_kernel(const PairDev* __restric
         const int seg = order_seg[lo];
         const int pair = seg / kScales, s = seg - pair * kScales;
         const PairDev& P = pairs[pair];
-        const int W = P.sw[s], H = P.sh[s];
-        const int64_t pitch = P.pitch[s];
-        const int ntx = (W + T - 1) / T;
+        TileGeo T;
+        T.W = P.sw[s];
+        T.H = P.sh[s];
+        T.pitch = P.pitch[s];
+        T.pe = P.plane_elems[s];
+        const int ntx = (T.W + TS - 1) / TS;
         const int local = t - order_start[lo];
         const int ty = local / ntx, tx = local - ty * ntx;
-        const int x0 = tx * T, y0 = ty * T;
+        T.x0 = tx * TS;
+        T.y0 = ty * TS;
+        T.g = g;
+        T.j = j;
+        T.wave = wave;
+        T.swz = (((j >> 2) ^ (j >> 3)) & 1) << 2;
+        const int xs0 = T.x0 + 16 * wave;  // first output column of this wave's strip
+        T.xo = xs0 + j;                    // this lane's output column
+        // this lane's eight input columns xi .. xi + 7 (xi a multiple of 8): a
+        // group that starts in the image is read in place, past W up to the
+        // row's pitch (a multiple of 64 elements); a group wholly outside it
+        // is read at column 0 and masked
+        const int xi = xs0 - 8 + 8 * g;
+        const bool gin = xi >= 0 && xi < T.W;
+        T.xl = gin ? xi : 0;
+        T.cmask = gin ? (int)((1u << min(T.W - xi, 8)) - 1u) : 0;
         const ST* base1 = reinterpret_cast<const ST*>(arena + P.arena_off + P.scale_off[s]);
         const ST* base2 = reinterpret_cast<const ST*>(arena + P.arena_off + P.img_bytes + P.scale_off[s]);
-        const int64_t pe = P.plane_elems[s];
+        mx_tile<SP>(L, T, base1, base2, L.cum, lane);
 
-        const int xs0 = x0 + 16 * wave;       // first output column of this wave's strip
-        const int xo = xs0 + j;               // this lane's output column
-        const int xi = xs0 - 8 + 8 * g;       // first of this lane's eight input columns
-        const bool xin = (xs0 - 8 >= 0) && (xs0 + 24 <= W);  // every input column of the strip in the image
-        const float invh = __builtin_amdgcn_rcpf(tap_total(cum, xo, W));
-        // renormalisation of this lane's 16 output pixels (rows y0 + 16 ob + 4g + r), channel independent
-        float inv[4][4];
-#pragma unroll
-        for (int ob = 0; ob < 4; ob++)
-#pragma unroll
-            for (int r = 0; r < 4; r++)
-                inv[ob][r] = invh * __builtin_amdgcn_rcpf(tap_total(cum, min(y0 + 16 * ob + 4 * g + r, H - 1), H));
-
-#pragma unroll 1
-        for (int c = 0; c < 3; c++) {
-            const ST* p1 = base1 + c * pe;
-            const ST* p2 = base2 + c * pe;
-            const int cxp = min(x0 + T / 2, W - 1), cyp = min(y0 + T / 2, H - 1);
-            const float cc = ldf(p1 + (int64_t)cyp * pitch + cxp);
-            float acc[6] = {0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
-
-#pragma unroll 1
-            for (int mb = 0; mb < 5; mb++) {
-                // ---- row pass of input rows y0 - 8 + 16 mb .. +15; this lane: row j, columns xi .. xi + 7
-                const int yr = y0 - 8 + 16 * mb + j;
-                float va[8], vb2[8], ve[8], vq[8];
-                {
-                    float u1[8], u2[8];
-                    bool in[8];
-                    const bool row_ok = yr >= 0 && yr < H;
-                    if (row_ok && xin) {
-                        Load8<ST>::run(p1 + (int64_t)yr * pitch + xi, u1);
-                        Load8<ST>::run(p2 + (int64_t)yr * pitch + xi, u2);
-#pragma unroll
-                        for (int e = 0; e < 8; e++) in[e] = true;
-                    } else {
-                        const int yc = min(max(yr, 0), H - 1);
-#pragma unroll
-                        for (int e = 0; e < 8; e++) {
-                            const int x = xi + e;
-                            in[e] = row_ok && x >= 0 && x < W;
-                            const int xc = min(max(x, 0), W - 1);
-                            u1[e] = ldf(p1 + (int64_t)yc * pitch + xc);
-                            u2[e] = ldf(p2 + (int64_t)yc * pitch + xc);
-                        }
-                    }
-#pragma unroll
-                    for (int e = 0; e < 8; e++) {
-                        const float a = in[e] ? u1[e] - cc : 0.f;
-                        const float b = in[e] ? u2[e] - cc : 0.f;
-                        const float dd = in[e] ? u1[e] - u2[e] : 0.f;
-                        va[e] = a;
-                        vb2[e] = b;
-                        ve[e] = dd * dd;
-                        vq[e] = a * b;
-                    }
-                }
-                const f32x4 z = {0.f, 0.f, 0.f, 0.f};
-                f32x4 hr[4];
-                hr[0] = mfma6(split8(va), G, z);
-                hr[1] = mfma6(split8(vb2), G, z);
-                hr[2] = mfma6(split8(ve), G, z);
-                hr[3] = mfma6(split8(vq), G, z);
-                // lane (g, j) holds rows 4g .. 4g+3 of the block, column j: store the bf16 parts
-                const int slot = mb & 1;
-#pragma unroll
-                for (int m = 0; m < 4; m++) {
-                    float h[4], md[4], l[4];
-#pragma unroll
-                    for (int r = 0; r < 4; r++) split3(hr[m][r], h[r], md[r], l[r]);
-                    *reinterpret_cast<uint2*>(&ring[wave][slot][m][0][j][(2 * g) ^ swz]) =
-                        make_uint2(pack2(h[0], h[1]), pack2(h[2], h[3]));
-                    *reinterpret_cast<uint2*>(&ring[wave][slot][m][1][j][(2 * g) ^ swz]) =
-                        make_uint2(pack2(md[0], md[1]), pack2(md[2], md[3]));
-                    *reinterpret_cast<uint2*>(&ring[wave][slot][m][2][j][(2 * g) ^ swz]) =
-                        make_uint2(pack2(l[0], l[1]), pack2(l[2], l[3]));
-                }
-                wave_lds_sync();
-                if (mb == 0) continue;
-
-                // ---- column pass of output rows y0 + 16 ob .. +15 from ring blocks ob, ob + 1
-                const int ob = mb - 1;
-                const int rs = (ob + (g >> 1)) & 1, rw = (4 * (g & 1)) ^ swz;
-                f32x4 o[4];
-#pragma unroll
-                for (int m = 0; m < 4; m++) {
-                    Split8 B;
-                    B.h = *reinterpret_cast<const u32x4*>(&ring[wave][rs][m][0][j][rw]);
-                    B.m = *reinterpret_cast<const u32x4*>(&ring[wave][rs][m][1][j][rw]);
-                    B.l = *reinterpret_cast<const u32x4*>(&ring[wave][rs][m][2][j][rw]);
-                    o[m] = mfma6(G, B, z);
-                }
-                wave_lds_sync();
-                // ---- maps of this lane's output pixels: rows y0 + 16 ob + 4g + r, column xo
-                float ivr[4];
-#pragma unroll
-                for (int r = 0; r < 4; r++) ivr[r] = ob == 0 ? inv[0][r] : ob == 1 ? inv[1][r] : ob == 2 ? inv[2][r] : inv[3][r];
-#pragma unroll
-                for (int r = 0; r < 4; r++) {
-                    const int y = y0 + 16 * ob + 4 * g + r;
-                    if (xo < W && y < H) {
-                        const float iv = ivr[r];
-                        const float mu1 = o[0][r] * iv, mu2 = o[1][r] * iv;
-                        const float E2 = o[2][r] * iv, Q = o[3][r] * iv;
-                        const float md = mu1 - mu2;
-                        const float md2 = md * md;
-                        const float vd = E2 - md2;  // var(x1 - x2)
-                        const float num_s = fmaf(2.f, Q - mu1 * mu2, 0.0009f);
-                        float d = fmaf(md2, num_s, vd) * __builtin_amdgcn_rcpf(vd + num_s);
-                        d = d > 0.f ? d : 0.f;
-                        const float a = ldf(p1 + (int64_t)y * pitch + xo) - cc;
-                        const float b = ldf(p2 + (int64_t)y * pitch + xo) - cc;
-                        const float fa = fabsf(a - mu1) + 1.f, fb = fabsf(b - mu2) + 1.f;
-                        const float ev = (fb - fa) * __builtin_amdgcn_rcpf(fa);  // src/ssimu2/score.hpp:150-154
-                        const float art = ev > 0.f ? ev : 0.f, det = ev < 0.f ? -ev : 0.f;
-                        const float d2 = d * d, a2 = art * art, t2 = det * det;
-                        acc[0] += d;
-                        acc[1] += d2 * d2;
-                        acc[2] += art;
-                        acc[3] += a2 * a2;
-                        acc[4] += det;
-                        acc[5] += t2 * t2;
-                    }
-                }
-            }
-            // wave sums of this channel in FP64, fixed order
-#pragma unroll
-            for (int k = 0; k < 6; k++) {
-                double x = (double)acc[k];
-#pragma unroll
-                for (int off = 32; off > 0; off >>= 1) x += __shfl_xor(x, off, 64);
-                if (lane == 0) red[wave][c * 6 + k] = x;
-            }
-        }
         __syncthreads();
         if (tid < kFeat) {
-            const double sum = ((red[0][tid] + red[1][tid]) + red[2][tid]) + red[3][tid];
+            const double sum = ((L.red[0][tid] + L.red[1][tid]) + L.red[2][tid]) + L.red[3][tid];
             partials[(int64_t)t * kFeat + tid] = sum;
         }
         __syncthreads();
     }
 }
 
-hipError_t launch_map_mx(bool half, int grid, const PairDev* pairs, const int32_t* order_seg,
+hipError_t launch_map_mx(bool half, bool exact, int grid, const PairDev* pairs, const int32_t* order_seg,
                          const int32_t* order_start, const BatchHdr* hdr, const void* arena, double* partials,
                          hipStream_t st) {
     const uint8_t* a = reinterpret_cast<const uint8_t*>(arena);
-    if (half) map_mx_kernel<__half><<<grid, 256, 0, st>>>(pairs, order_seg, order_start, hdr, a, partials);
-    else map_mx_kernel<float><<<grid, 256, 0, st>>>(pairs, order_seg, order_start, hdr, a, partials);
+    // float RGB input may hold any value: the exact split; every other format
+    // gives bounded XYB values: the scaled fp16 split
+    if (exact) {
+        if (half) map_mx_kernel<__half, SplitBf16x3><<<grid, 256, 0, st>>>(pairs, order_seg, order_start, hdr, a, partials);
+        else map_mx_kernel<float, SplitBf16x3><<<grid, 256, 0, st>>>(pairs, order_seg, order_start, hdr, a, partials);
+    } else {
+        if (half) map_mx_kernel<__half, SplitF16x2><<<grid, 256, 0, st>>>(pairs, order_seg, order_start, hdr, a, partials);
+        else map_mx_kernel<float, SplitF16x2><<<grid, 256, 0, st>>>(pairs, order_seg, order_start, hdr, a, partials);
+    }
     return hipGetLastError();
 }
 

@@ -55,13 +55,18 @@ struct BatchHdr {
     // accumulates the moment blur in FP32, class 1 (segments of fewer than
     // kSmallSegment pixels, where FP32 rounding noise would not average out)
     // in FP64 with tiles of kSmallTH rows. Each class splits by plane storage
-    // into group 2*class (FP32 planes) and 2*class+1 (FP16 planes); segments
-    // and tiles are numbered group by group.
-    int32_t nseg[4];
-    int32_t tiles[4];
+    // into group 2*class (FP32 planes) and 2*class+1 (FP16 planes). On the
+    // matrix cores, class-0 segments of pairs with float RGB input go to
+    // groups 4 (FP32 planes) and 5 (FP16 planes) instead: they take the exact
+    // bf16 split, the others the scaled fp16 one. Segments and tiles are
+    // numbered group by group.
+    int32_t nseg[6];
+    int32_t tiles[6];
 };
 
-__host__ __device__ constexpr int map_group(int cls, bool half) { return 2 * cls + (half ? 1 : 0); }
+__host__ __device__ constexpr int map_group(int cls, bool half, bool exact = false) {
+    return exact ? 4 + (half ? 1 : 0) : 2 * cls + (half ? 1 : 0);
+}
 
 constexpr int64_t kSmallSegment = 32768;
 constexpr int kSmallTH = 48;

@@ -64,8 +64,11 @@ __device__ __forceinline__ void to_xyb(float r, float g, float b, float& X, floa
     Y = y + 0.01f;
 }
 
-__device__ __forceinline__ float sample_u(const uint8_t* plane, int64_t stride, int x, int y, bool wide) {
-    if (wide) return (float)reinterpret_cast<const uint16_t*>(plane + (int64_t)y * stride)[x];
+// One YUV sample; 16-bit-container samples above the declared depth's
+// largest code are clamped to it.
+__device__ __forceinline__ float sample_u(const uint8_t* plane, int64_t stride, int x, int y, bool wide,
+                                          int maxcode) {
+    if (wide) return (float)min((int)reinterpret_cast<const uint16_t*>(plane + (int64_t)y * stride)[x], maxcode);
     return (float)plane[(int64_t)y * stride + x];
 }
 
@@ -172,6 +175,7 @@ __global__ __launch_bounds__(256) void prep_kernel(const PairDev* __restrict__ p
         const int fmt = P.fmt;
         const bool yuv = fmt >= IN_YUV420_U8;
         const bool wide = (fmt == IN_YUV420_U16);
+        const int maxcode = wide ? (1 << P.bits) - 1 : 255;
 
         if (yuv) {
             // stage the 36x36 chroma neighbourhood (indices clamped to the plane)
@@ -181,7 +185,7 @@ __global__ __launch_bounds__(256) void prep_kernel(const PairDev* __restrict__ p
                 const int pl = idx / 1296, rem = idx - pl * 1296;
                 const int r = rem / 36, c = rem - r * 36;
                 const int gy = min(max(cy0 + r, 0), chh - 1), gx = min(max(cx0 + c, 0), cw - 1);
-                L.ch[pl][r][c] = sample_u(src[pl + 1], stride[pl + 1], gx, gy, wide);
+                L.ch[pl][r][c] = sample_u(src[pl + 1], stride[pl + 1], gx, gy, wide, maxcode);
             }
             __syncthreads();
         }
@@ -213,8 +217,10 @@ __global__ __launch_bounds__(256) void prep_kernel(const PairDev* __restrict__ p
                     rgb[3][p] = rgb_sample(src[p], stride[p], fmt, xb, yb);
                 }
             } else {
-                const float yv[4] = {sample_u(src[0], stride[0], xa, ya, wide), sample_u(src[0], stride[0], xa, yb, wide),
-                                     sample_u(src[0], stride[0], xb, ya, wide), sample_u(src[0], stride[0], xb, yb, wide)};
+                const float yv[4] = {sample_u(src[0], stride[0], xa, ya, wide, maxcode),
+                                     sample_u(src[0], stride[0], xa, yb, wide, maxcode),
+                                     sample_u(src[0], stride[0], xb, ya, wide, maxcode),
+                                     sample_u(src[0], stride[0], xb, yb, wide, maxcode)};
                 // chroma: local staged rows qy..qy+4, columns qx..qx+3
                 float cbcr[2][4];
 #pragma unroll

@@ -199,7 +199,8 @@ struct MxLds {
     // j+15 at a stride of 8 words) fall on distinct banks
     uint32_t ring[4][2][4][NP][16][8];
     double red[4][kFeat];
-    float cum[18];  // cumulative taps, as the split represents them
+    float rowf[4][64];  // per wave: renormalisation factor of the tile's 64 output rows
+    float cum[18];      // cumulative taps, as the split represents them
 };
 
 // Geometry of one tile, the same for every channel.
@@ -236,11 +237,13 @@ __device__ __forceinline__ void mx_tile(MxLds<SP::kParts>& L, const TileGeo& T, 
     const int W = T.W, H = T.H, g = T.g, j = T.j, wave = T.wave, swz = T.swz;
     const typename SP::Op G = tap_operand<SP>(g, j);
     const int64_t pitch = T.pitch;
-    // renormalisation of this lane's output column, with the scale of the
-    // products folded in; the row factor is formed per output pixel, which
-    // keeps the registers under the 128 of four workgroups per CU
+    // renormalisation: the factor of this lane's output column, with the scale
+    // of the products folded in, and a table of the tile's 64 row factors
+    // (lane l: row y0 + l) that the maps read four at a time
     constexpr float kOutScale = 1.f / (SP::kScaleAB * SP::kTapScale);
     const float invh = __builtin_amdgcn_rcpf(tap_total(cum, T.xo, W)) * kOutScale;
+    L.rowf[wave][lane] = __builtin_amdgcn_rcpf(tap_total(cum, min(T.y0 + lane, H - 1), H));
+    wave_lds_sync();
     // E2 carries kScaleDD^2, Q carries kScaleAB^2 kScaleQ, where a, b carry kScaleAB
     constexpr float kEOut = SP::kScaleAB / (SP::kScaleDD * SP::kScaleDD);
     constexpr float kQOut = 1.f / (SP::kScaleAB * SP::kScaleQ);
@@ -272,12 +275,16 @@ __device__ __forceinline__ void mx_tile(MxLds<SP::kParts>& L, const TileGeo& T, 
                 bool in[8];
 #pragma unroll
                 for (int e = 0; e < 8; e++) in[e] = row_ok && ((T.cmask >> e) & 1);
+                // an element outside the image takes the value cc in both
+                // images, so that all four of its moments are exactly zero
+                // (ccs = cc kScaleAB is exact: the scale is a power of two)
                 float v[4][8];
 #pragma unroll
                 for (int e = 0; e < 8; e++) {
-                    const float a = in[e] ? fmaf(u1[e], SP::kScaleAB, -ccs) : 0.f;
-                    const float b = in[e] ? fmaf(u2[e], SP::kScaleAB, -ccs) : 0.f;
-                    const float dd = in[e] ? (u1[e] - u2[e]) * SP::kScaleDD : 0.f;
+                    const float x1 = in[e] ? u1[e] : cc, x2 = in[e] ? u2[e] : cc;
+                    const float a = fmaf(x1, SP::kScaleAB, -ccs);
+                    const float b = fmaf(x2, SP::kScaleAB, -ccs);
+                    const float dd = (x1 - x2) * SP::kScaleDD;
                     v[0][e] = a;
                     v[1][e] = b;
                     v[2][e] = dd * dd;
@@ -320,11 +327,13 @@ __device__ __forceinline__ void mx_tile(MxLds<SP::kParts>& L, const TileGeo& T, 
             }
             wave_lds_sync();
             // ---- maps of this lane's output pixels: rows y0 + 16 ob + 4g + r, column xo
+            const float4 rf = *reinterpret_cast<const float4*>(&L.rowf[wave][16 * ob + 4 * g]);
+            const float rfv[4] = {rf.x, rf.y, rf.z, rf.w};
 #pragma unroll
             for (int r = 0; r < 4; r++) {
                 const int y = T.y0 + 16 * ob + 4 * g + r;
                 if (T.xo < W && y < H) {
-                    const float iv = invh * __builtin_amdgcn_rcpf(tap_total(cum, y, H));
+                    const float iv = invh * rfv[r];
                     const float mu1 = o[0][r] * iv, mu2 = o[1][r] * iv;
                     const float E2 = o[2][r] * (iv * kEOut), Q = o[3][r] * (iv * kQOut);
                     const float md = mu1 - mu2;

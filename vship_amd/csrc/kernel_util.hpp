@@ -7,6 +7,14 @@
 
 namespace vship {
 
+// Global-memory view of a device pointer that the compiler cannot place (one
+// read from a descriptor), so that loads through it are global loads rather
+// than flat ones, which also wait on LDS traffic.
+template <typename T>
+__device__ __forceinline__ const __attribute__((address_space(1))) T* as_global(const T* p) {
+    return (const __attribute__((address_space(1))) T*)p;
+}
+
 __device__ __forceinline__ float ldf(const float* p) { return *p; }
 __device__ __forceinline__ float ldf(const __half* p) { return __half2float(*p); }
 __device__ __forceinline__ void stf(float* p, float v) { *p = v; }

@@ -39,12 +39,16 @@ __device__ __forceinline__ float linearize(float v) {
     return copysignf(r, v);
 }
 
-// cube root of x >= 0: 2^(log2(x)/3) refined by one Newton step on y^3 = x.
+// cube root of x >= 0 without a division: r = 2^(-log2(x)/3) ~ x^(-1/3) on
+// the hardware units, y = x r^2, then one Newton step on y^3 = x with the
+// derivative's reciprocal 1/(3 y^2) taken as r^2/3 (within 0.75 ulp of the
+// exact cube root where the hardware units are within 2 ulp).
 __device__ __forceinline__ float cbrt_pos(float x) {
     if (!(x > 0.f)) return 0.f;
-    float y = __builtin_amdgcn_exp2f(__builtin_amdgcn_logf(x) * (1.0f / 3.0f));
-    y = y - (y * y * y - x) * __builtin_amdgcn_rcpf(3.0f * y * y);
-    return y;
+    const float r = __builtin_amdgcn_exp2f(__builtin_amdgcn_logf(x) * (-1.0f / 3.0f));
+    const float rr = r * r;
+    const float y = x * rr;
+    return fmaf(-fmaf(y * y, y, -x), rr * (1.0f / 3.0f), y);
 }
 
 // Linear RGB -> positive XYB, src/ssimu2/makeXYB.hpp:4-50.
@@ -68,8 +72,8 @@ __device__ __forceinline__ void to_xyb(float r, float g, float b, float& X, floa
 // largest code are clamped to it.
 __device__ __forceinline__ float sample_u(const uint8_t* plane, int64_t stride, int x, int y, bool wide,
                                           int maxcode) {
-    if (wide) return (float)min((int)reinterpret_cast<const uint16_t*>(plane + (int64_t)y * stride)[x], maxcode);
-    return (float)plane[(int64_t)y * stride + x];
+    if (wide) return (float)min((int)as_global(reinterpret_cast<const uint16_t*>(plane + (int64_t)y * stride))[x], maxcode);
+    return (float)as_global(plane)[(int64_t)y * stride + x];
 }
 
 // ---------------------------------------------------------------------------
@@ -133,19 +137,136 @@ __device__ __forceinline__ void pyr_level(const float* par, float* chi, uint8_t*
     }
 }
 
-// Reads sample x of row y of an input plane of the pair's format as float.
-__device__ __forceinline__ float rgb_sample(const uint8_t* plane, int64_t stride, int fmt, int x, int y) {
+// Reads sample x of row y of an input plane of RGB format FMT as float.
+template <int FMT>
+__device__ __forceinline__ float rgb_sample(const uint8_t* plane, int64_t stride, int x, int y) {
     const uint8_t* row = plane + (int64_t)y * stride;
-    switch (fmt) {
-    case IN_RGB_U8: return (float)row[x] * (1.0f / 255.0f);
-    case IN_RGB_U16: return (float)reinterpret_cast<const uint16_t*>(row)[x] * (1.0f / 65535.0f);
-    case IN_RGB_F16: return __half2float(reinterpret_cast<const __half*>(row)[x]);
-    default: return reinterpret_cast<const float*>(row)[x];
+    if constexpr (FMT == IN_RGB_U8) return (float)as_global(row)[x] * (1.0f / 255.0f);
+    else if constexpr (FMT == IN_RGB_U16) return (float)as_global(reinterpret_cast<const uint16_t*>(row))[x] * (1.0f / 65535.0f);
+    else if constexpr (FMT == IN_RGB_F16)
+        return __half2float(__ushort_as_half(as_global(reinterpret_cast<const unsigned short*>(row))[x]));
+    else return as_global(reinterpret_cast<const float*>(row))[x];
+}
+
+// Scales 0 and 1 of one 64x64 tile of an image of input format FMT: a thread
+// owns 2x2 pixel quads of scale 0, so the scale-1 child is formed in
+// registers and also left in LDS (l1) for the coarser levels.
+template <typename ST, int FMT>
+__device__ __forceinline__ void prep_quads(PrepLds& L, const PairDev& P, uint8_t* arena, int img, int tx, int ty) {
+    constexpr bool yuv = FMT >= IN_YUV420_U8;
+    constexpr bool wide = FMT == IN_YUV420_U16;
+    const int tid = threadIdx.x;
+    const int W = P.w, H = P.h;
+    const int x0 = tx * kPrepTile, y0 = ty * kPrepTile;
+    const uint8_t* const* src = P.src[img];
+    const int64_t* stride = P.stride[img];
+    const int maxcode = wide ? (1 << P.bits) - 1 : 255;
+
+    if constexpr (yuv) {
+        // stage the 36x36 chroma neighbourhood (indices clamped to the plane)
+        const int cw = (W + 1) >> 1, chh = (H + 1) >> 1;
+        const int cx0 = x0 / 2 - 1, cy0 = y0 / 2 - 2;
+        for (int idx = tid; idx < 2 * 36 * 36; idx += 256) {
+            const int pl = idx / 1296, rem = idx - pl * 1296;
+            const int r = rem / 36, c = rem - r * 36;
+            const int gy = min(max(cy0 + r, 0), chh - 1), gx = min(max(cx0 + c, 0), cw - 1);
+            L.ch[pl][r][c] = sample_u(src[pl + 1], stride[pl + 1], gx, gy, wide, maxcode);
+        }
+        __syncthreads();
+    }
+
+    // limited-range YUV -> R'G'B' constants (uniform)
+    const float sc = yuv ? (float)(1 << (P.bits - 8)) : 1.0f;
+    const float ymul = 1.0f / (219.0f * sc), yadd = -16.0f / 219.0f;
+    const float cmul = 1.0f / (128.0f * 224.0f * sc), cadd = -128.0f / 224.0f;
+    const float kr = P.kr, kb = P.kb, ikg = 1.0f / (1.0f - P.kr - P.kb);
+    const float rcr = 2.0f * (1.0f - kr), bcb = 2.0f * (1.0f - kb);
+
+    const int nv1x = min(32, P.sw[1] - tx * 32), nv1y = min(32, P.sh[1] - ty * 32);
+#pragma unroll 1
+    for (int i = 0; i < 4; i++) {
+        const int q = tid + 256 * i;
+        const int qy = q >> 5, qx = q & 31;
+        const int xa = x0 + 2 * qx, ya = y0 + 2 * qy;
+        if (xa >= W || ya >= H) continue;  // no pixel of this quad is in the image
+        const bool hx = xa + 1 < W, hy = ya + 1 < H;
+        const int xb = hx ? xa + 1 : xa, yb = hy ? ya + 1 : ya;  // clamped second column/row
+        float rgb[4][3];  // quad pixels: (xa,ya), (xa,yb), (xb,ya), (xb,yb)
+        if constexpr (!yuv) {
+#pragma unroll
+            for (int p = 0; p < 3; p++) {
+                rgb[0][p] = rgb_sample<FMT>(src[p], stride[p], xa, ya);
+                rgb[1][p] = rgb_sample<FMT>(src[p], stride[p], xa, yb);
+                rgb[2][p] = rgb_sample<FMT>(src[p], stride[p], xb, ya);
+                rgb[3][p] = rgb_sample<FMT>(src[p], stride[p], xb, yb);
+            }
+        } else {
+            const float yv[4] = {sample_u(src[0], stride[0], xa, ya, wide, maxcode),
+                                 sample_u(src[0], stride[0], xa, yb, wide, maxcode),
+                                 sample_u(src[0], stride[0], xb, ya, wide, maxcode),
+                                 sample_u(src[0], stride[0], xb, yb, wide, maxcode)};
+            // chroma: local staged rows qy..qy+4, columns qx..qx+3
+            float cbcr[2][4];
+#pragma unroll
+            for (int pl = 0; pl < 2; pl++) {
+                float he[5], ho[5];  // horizontal: even luma column (left-sited sample), odd (Catmull-Rom at 0.5)
+#pragma unroll
+                for (int r = 0; r < 5; r++) {
+                    const float* row = L.ch[pl][qy + r];
+                    he[r] = row[qx + 1];
+                    ho[r] = (9.0f * (row[qx + 1] + row[qx + 2]) - (row[qx] + row[qx + 3])) * (1.0f / 16.0f);
+                }
+                // vertical, centre-sited: even luma row t = 0.75 from rows 0..3, odd row t = 0.25 from rows 1..4
+                const float ve_e = -3.0f * he[0] + 29.0f * he[1] + 111.0f * he[2] - 9.0f * he[3];
+                const float vo_e = -9.0f * he[1] + 111.0f * he[2] + 29.0f * he[3] - 3.0f * he[4];
+                const float ve_o = -3.0f * ho[0] + 29.0f * ho[1] + 111.0f * ho[2] - 9.0f * ho[3];
+                const float vo_o = -9.0f * ho[1] + 111.0f * ho[2] + 29.0f * ho[3] - 3.0f * ho[4];
+                // (xa,ya): even col, even row; (xa,yb): even col, odd row; (xb,..): odd col.
+                // A clamped column/row repeats the first one.
+                cbcr[pl][0] = fmaf(ve_e, cmul, cadd);
+                cbcr[pl][1] = fmaf(hy ? vo_e : ve_e, cmul, cadd);
+                cbcr[pl][2] = fmaf(hx ? ve_o : ve_e, cmul, cadd);
+                cbcr[pl][3] = fmaf(hx ? (hy ? vo_o : ve_o) : (hy ? vo_e : ve_e), cmul, cadd);
+            }
+#pragma unroll
+            for (int k = 0; k < 4; k++) {
+                const float Yn = fmaf(yv[k], ymul, yadd);
+                const float R = fmaf(rcr, cbcr[1][k], Yn);
+                const float B = fmaf(bcb, cbcr[0][k], Yn);
+                const float G = (Yn - kr * R - kb * B) * ikg;
+                rgb[k][0] = R;
+                rgb[k][1] = G;
+                rgb[k][2] = B;
+            }
+        }
+#pragma unroll
+        for (int k = 0; k < 4; k++)
+#pragma unroll
+            for (int p = 0; p < 3; p++) rgb[k][p] = linearize(rgb[k][p]);
+        store_xyb<ST>(arena, P, img, 0, xa, ya, rgb[0][0], rgb[0][1], rgb[0][2]);
+        if (hy) store_xyb<ST>(arena, P, img, 0, xa, yb, rgb[1][0], rgb[1][1], rgb[1][2]);
+        if (hx) store_xyb<ST>(arena, P, img, 0, xb, ya, rgb[2][0], rgb[2][1], rgb[2][2]);
+        if (hx && hy) store_xyb<ST>(arena, P, img, 0, xb, yb, rgb[3][0], rgb[3][1], rgb[3][2]);
+        // scale-1 child (src/ssimu2/downsample.hpp:3-19 summation order)
+        if (qx < nv1x && qy < nv1y) {
+            float v[3];
+#pragma unroll
+            for (int p = 0; p < 3; p++) {
+                float a = rgb[0][p];
+                a += rgb[1][p];
+                a += rgb[2][p];
+                a += rgb[3][p];
+                v[p] = a * 0.25f;
+            }
+            store_xyb<ST>(arena, P, img, 1, tx * 32 + qx, ty * 32 + qy, v[0], v[1], v[2]);
+#pragma unroll
+            for (int p = 0; p < 3; p++) L.l1[p][qy][qx] = v[p];
+        }
     }
 }
 
-// Each workgroup converts one 64x64 scale-0 tile of one image. A thread owns
-// 2x2 pixel quads, so scale 1 is formed in registers; scales 2..5 go through
+// Each workgroup converts one 64x64 scale-0 tile of one image. Scales 0 and 1
+// come from prep_quads, instantiated per input format; scales 2..5 go through
 // LDS. Every XYB plane is written exactly once.
 template <typename ST>
 __global__ __launch_bounds__(256) void prep_kernel(const PairDev* __restrict__ pairs,
@@ -154,7 +275,6 @@ __global__ __launch_bounds__(256) void prep_kernel(const PairDev* __restrict__ p
     __shared__ PrepLds L;
     const int npairs = hdr->npairs;
     const int ntiles = hdr->prep_tiles;
-    const int tid = threadIdx.x;
     for (int t = blockIdx.x; t < ntiles; t += gridDim.x) {
         int lo = 0, hi = npairs - 1;
         while (lo < hi) {
@@ -169,117 +289,17 @@ __global__ __launch_bounds__(256) void prep_kernel(const PairDev* __restrict__ p
         const int img = local / (ntx * nty);
         local -= img * ntx * nty;
         const int ty = local / ntx, tx = local - ty * ntx;
-        const int x0 = tx * kPrepTile, y0 = ty * kPrepTile;
-        const uint8_t* const* src = P.src[img];
-        const int64_t* stride = P.stride[img];
-        const int fmt = P.fmt;
-        const bool yuv = fmt >= IN_YUV420_U8;
-        const bool wide = (fmt == IN_YUV420_U16);
-        const int maxcode = wide ? (1 << P.bits) - 1 : 255;
 
-        if (yuv) {
-            // stage the 36x36 chroma neighbourhood (indices clamped to the plane)
-            const int cw = (W + 1) >> 1, chh = (H + 1) >> 1;
-            const int cx0 = x0 / 2 - 1, cy0 = y0 / 2 - 2;
-            for (int idx = tid; idx < 2 * 36 * 36; idx += 256) {
-                const int pl = idx / 1296, rem = idx - pl * 1296;
-                const int r = rem / 36, c = rem - r * 36;
-                const int gy = min(max(cy0 + r, 0), chh - 1), gx = min(max(cx0 + c, 0), cw - 1);
-                L.ch[pl][r][c] = sample_u(src[pl + 1], stride[pl + 1], gx, gy, wide, maxcode);
-            }
-            __syncthreads();
-        }
-
-        // limited-range YUV -> R'G'B' constants (uniform)
-        const float sc = yuv ? (float)(1 << (P.bits - 8)) : 1.0f;
-        const float ymul = 1.0f / (219.0f * sc), yadd = -16.0f / 219.0f;
-        const float cmul = 1.0f / (128.0f * 224.0f * sc), cadd = -128.0f / 224.0f;
-        const float kr = P.kr, kb = P.kb, ikg = 1.0f / (1.0f - P.kr - P.kb);
-        const float rcr = 2.0f * (1.0f - kr), bcb = 2.0f * (1.0f - kb);
-
-        // scale 0 (quads) and scale 1 (in registers)
-        const int nv1x = min(32, P.sw[1] - tx * 32), nv1y = min(32, P.sh[1] - ty * 32);
-#pragma unroll 1
-        for (int i = 0; i < 4; i++) {
-            const int q = tid + 256 * i;
-            const int qy = q >> 5, qx = q & 31;
-            const int xa = x0 + 2 * qx, ya = y0 + 2 * qy;
-            if (xa >= W || ya >= H) continue;  // no pixel of this quad is in the image
-            const bool hx = xa + 1 < W, hy = ya + 1 < H;
-            const int xb = hx ? xa + 1 : xa, yb = hy ? ya + 1 : ya;  // clamped second column/row
-            float rgb[4][3];  // quad pixels: (xa,ya), (xa,yb), (xb,ya), (xb,yb)
-            if (!yuv) {
-#pragma unroll
-                for (int p = 0; p < 3; p++) {
-                    rgb[0][p] = rgb_sample(src[p], stride[p], fmt, xa, ya);
-                    rgb[1][p] = rgb_sample(src[p], stride[p], fmt, xa, yb);
-                    rgb[2][p] = rgb_sample(src[p], stride[p], fmt, xb, ya);
-                    rgb[3][p] = rgb_sample(src[p], stride[p], fmt, xb, yb);
-                }
-            } else {
-                const float yv[4] = {sample_u(src[0], stride[0], xa, ya, wide, maxcode),
-                                     sample_u(src[0], stride[0], xa, yb, wide, maxcode),
-                                     sample_u(src[0], stride[0], xb, ya, wide, maxcode),
-                                     sample_u(src[0], stride[0], xb, yb, wide, maxcode)};
-                // chroma: local staged rows qy..qy+4, columns qx..qx+3
-                float cbcr[2][4];
-#pragma unroll
-                for (int pl = 0; pl < 2; pl++) {
-                    float he[5], ho[5];  // horizontal: even luma column (left-sited sample), odd (Catmull-Rom at 0.5)
-#pragma unroll
-                    for (int r = 0; r < 5; r++) {
-                        const float* row = L.ch[pl][qy + r];
-                        he[r] = row[qx + 1];
-                        ho[r] = (9.0f * (row[qx + 1] + row[qx + 2]) - (row[qx] + row[qx + 3])) * (1.0f / 16.0f);
-                    }
-                    // vertical, centre-sited: even luma row t = 0.75 from rows 0..3, odd row t = 0.25 from rows 1..4
-                    const float ve_e = -3.0f * he[0] + 29.0f * he[1] + 111.0f * he[2] - 9.0f * he[3];
-                    const float vo_e = -9.0f * he[1] + 111.0f * he[2] + 29.0f * he[3] - 3.0f * he[4];
-                    const float ve_o = -3.0f * ho[0] + 29.0f * ho[1] + 111.0f * ho[2] - 9.0f * ho[3];
-                    const float vo_o = -9.0f * ho[1] + 111.0f * ho[2] + 29.0f * ho[3] - 3.0f * ho[4];
-                    // (xa,ya): even col, even row; (xa,yb): even col, odd row; (xb,..): odd col.
-                    // A clamped column/row repeats the first one.
-                    cbcr[pl][0] = fmaf(ve_e, cmul, cadd);
-                    cbcr[pl][1] = fmaf(hy ? vo_e : ve_e, cmul, cadd);
-                    cbcr[pl][2] = fmaf(hx ? ve_o : ve_e, cmul, cadd);
-                    cbcr[pl][3] = fmaf(hx ? (hy ? vo_o : ve_o) : (hy ? vo_e : ve_e), cmul, cadd);
-                }
-#pragma unroll
-                for (int k = 0; k < 4; k++) {
-                    const float Yn = fmaf(yv[k], ymul, yadd);
-                    const float R = fmaf(rcr, cbcr[1][k], Yn);
-                    const float B = fmaf(bcb, cbcr[0][k], Yn);
-                    const float G = (Yn - kr * R - kb * B) * ikg;
-                    rgb[k][0] = R;
-                    rgb[k][1] = G;
-                    rgb[k][2] = B;
-                }
-            }
-#pragma unroll
-            for (int k = 0; k < 4; k++)
-#pragma unroll
-                for (int p = 0; p < 3; p++) rgb[k][p] = linearize(rgb[k][p]);
-            store_xyb<ST>(arena, P, img, 0, xa, ya, rgb[0][0], rgb[0][1], rgb[0][2]);
-            if (hy) store_xyb<ST>(arena, P, img, 0, xa, yb, rgb[1][0], rgb[1][1], rgb[1][2]);
-            if (hx) store_xyb<ST>(arena, P, img, 0, xb, ya, rgb[2][0], rgb[2][1], rgb[2][2]);
-            if (hx && hy) store_xyb<ST>(arena, P, img, 0, xb, yb, rgb[3][0], rgb[3][1], rgb[3][2]);
-            // scale-1 child (src/ssimu2/downsample.hpp:3-19 summation order)
-            if (qx < nv1x && qy < nv1y) {
-                float v[3];
-#pragma unroll
-                for (int p = 0; p < 3; p++) {
-                    float a = rgb[0][p];
-                    a += rgb[1][p];
-                    a += rgb[2][p];
-                    a += rgb[3][p];
-                    v[p] = a * 0.25f;
-                }
-                store_xyb<ST>(arena, P, img, 1, tx * 32 + qx, ty * 32 + qy, v[0], v[1], v[2]);
-#pragma unroll
-                for (int p = 0; p < 3; p++) L.l1[p][qy][qx] = v[p];
-            }
+        switch (P.fmt) {
+        case IN_RGB_U8: prep_quads<ST, IN_RGB_U8>(L, P, arena, img, tx, ty); break;
+        case IN_RGB_U16: prep_quads<ST, IN_RGB_U16>(L, P, arena, img, tx, ty); break;
+        case IN_RGB_F16: prep_quads<ST, IN_RGB_F16>(L, P, arena, img, tx, ty); break;
+        case IN_RGB_F32: prep_quads<ST, IN_RGB_F32>(L, P, arena, img, tx, ty); break;
+        case IN_YUV420_U8: prep_quads<ST, IN_YUV420_U8>(L, P, arena, img, tx, ty); break;
+        default: prep_quads<ST, IN_YUV420_U16>(L, P, arena, img, tx, ty); break;
         }
         __syncthreads();
+        const int nv1x = min(32, P.sw[1] - tx * 32), nv1y = min(32, P.sh[1] - ty * 32);
 
         // scales 2..5 from LDS
         pyr_level<16, 33, 17, ST>(&L.l1[0][0][0], &L.l2[0][0][0], arena, P, img, 2, tx, ty, nv1x, nv1y);

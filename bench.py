@@ -124,14 +124,36 @@ def per_frame_rate(descs, n, w, h, fp16):
     return rate
 
 
-def roofline(st, map_ms, fp16):
+TRAFFIC_FILE = os.path.join(ROOT, "profiles", "hbm_traffic.json")
+
+
+def measured_traffic(config, kernel):
+    """HBM bytes per launch of `kernel` on workload `config` from the committed
+    record of the FETCH_SIZE/WRITE_SIZE runs (calibrated on a device copy), or
+    None where no run of this kernel on this workload is recorded."""
+    try:
+        with open(TRAFFIC_FILE) as f:
+            rec = json.load(f)["workloads"][config][kernel]
+    except (OSError, KeyError, ValueError):
+        return None
+    return rec["traffic_bytes"]
+
+
+def roofline(st, map_ms, fp16, config):
     achieved = st["map_bytes_fp32"] / (map_ms * 1e-3) / 1e9
     elem = "2 B (scales 0-1) or 4 B (scales 2-5)" if fp16 else "4 B"
+    st_name = "__half" if fp16 else "float"
+    if st["map_tile_height"] == 1:
+        kernel = f"map_mx_kernel<{st_name}, SplitF16x2>"
+        label = f"{kernel} (matrix cores, 64x64 tiles)"
+    else:
+        kernel = label = f"map_kernel<{st['map_tile_height']}, {st_name}, float>"
+    traffic = measured_traffic(config, kernel)
     return {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-            "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
-            "kernel": (f"map_mx_kernel<{'__half' if fp16 else 'float'}> (matrix cores, 64x64 tiles)"
-                       if st["map_tile_height"] == 1 else
-                       f"map_kernel<{st['map_tile_height']},{'__half' if fp16 else 'float'},float>"),
+            "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic,
+            "traffic_source": ("profiles/hbm_traffic.json: rocprofv3 FETCH_SIZE x 2.0 + WRITE_SIZE x 1.0 per launch"
+                               if traffic is not None else "no PMC run of this kernel on this workload"),
+            "kernel": label,
             "algorithmic_bytes_per_launch": st["map_bytes_fp32"],
             "note": f"2 images x 3 XYB planes x {elem} read per pixel of every scale in the FP32 class; "
                     "the kernel is bound by vector-ALU issue (operand splitting and the maps), not HBM"}
@@ -208,7 +230,7 @@ def run_batched(a, pairs, bits, fp16, workload, tune=False, graph=False):
         out["per_frame_path"] = {"value": round(pf, 2), "unit": "frame-pairs/s",
                                  "note": "one pair per launch sequence, eager, same device inputs",
                                  "batched_over_per_frame": round(value / pf, 2)}
-    out["roofline"] = roofline(st, map_ms, fp16)
+    out["roofline"] = roofline(st, map_ms, fp16, a.config)
     batch.close()
     dp.free()
     return out, scores

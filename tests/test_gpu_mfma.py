@@ -1,8 +1,9 @@
 """The matrix-core map kernel (map_tile_height = 1: 64x64 tiles, both blur
-passes on v_mfma_f32_16x16x32_bf16 with operands split exactly into three
-bf16 parts) against the FP64 oracle (|dscore| <= 1e-3, features as in the
-parity tests) and against the vector-ALU kernel (the same FP32 maps to
-rounding: |dscore| <= 1e-4)."""
+passes as 16x16x32 MFMA products of operands split into 16-bit parts: a
+scaled two-way fp16 split for the bounded 8- to 16-bit RGB and YUV formats,
+an exact three-way bf16 split for float RGB) against the FP64 oracle
+(|dscore| <= 1e-3, features as in the parity tests) and against the
+vector-ALU kernel (the same FP32 maps to rounding: |dscore| <= 1e-4)."""
 import numpy as np
 import pytest
 

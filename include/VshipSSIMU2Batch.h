@@ -16,18 +16,48 @@ extern "C" {
 
 /* Sample formats. RGB formats are the reference's InputMemType
  * (src/util/preprocessor.hpp:94): planar R, G, B, gamma-encoded (BT.709
- * transfer), u8/u16 full range or f16/f32 in [0,1]. The YUV 4:2:0 formats
- * are limited range; chroma is upsampled with Catmull-Rom cubic weights
- * (left-sited horizontally, centre-sited vertically, edge samples clamped)
- * and converted with the given matrix before the same pipeline runs. */
+ * transfer), u8/u16 full range or f16/f32 in [0,1].
+ *
+ * The YUV formats are planar, limited range, in the chroma layouts the
+ * reference's FFVship front end accepts
+ * (src/ffvship_utility/gpuColorToLinear/vshipColor.hpp:12-128): chroma planes
+ * of ceil(width / 2^sw) x ceil(height / 2^sh) samples with (sw, sh) = 4:2:0
+ * (1, 1), 4:2:2 (1, 0), 4:4:4 (0, 0), 4:4:0 (0, 1), 4:1:1 (2, 0), 4:1:0
+ * (2, 1). Even codes hold 8-bit samples, odd codes 9..16-bit samples (the
+ * pair's bits) in little-endian 16-bit words; samples above 2^bits - 1 are
+ * read as 2^bits - 1. Chroma is upsampled with Catmull-Rom cubic weights at
+ * the positions the pair's chroma_location gives, with edge samples clamped
+ * (src/ffvship_utility/gpuColorToLinear/chromaUpsample.hpp:5-205), and
+ * converted with the pair's matrix before the same pipeline runs. */
 typedef enum {
     Vship_PF_RGB_U8 = 0,
     Vship_PF_RGB_U16 = 1,
     Vship_PF_RGB_F16 = 2,
     Vship_PF_RGB_F32 = 3,
     Vship_PF_YUV420_U8 = 4,
-    Vship_PF_YUV420_U16 = 5, /* 9..16-bit samples in little-endian 16-bit words */
+    Vship_PF_YUV420_U16 = 5,
+    Vship_PF_YUV422_U8 = 6,
+    Vship_PF_YUV422_U16 = 7,
+    Vship_PF_YUV444_U8 = 8,
+    Vship_PF_YUV444_U16 = 9,
+    Vship_PF_YUV440_U8 = 10,
+    Vship_PF_YUV440_U16 = 11,
+    Vship_PF_YUV411_U8 = 12,
+    Vship_PF_YUV411_U16 = 13,
+    Vship_PF_YUV410_U8 = 14,
+    Vship_PF_YUV410_U16 = 15,
 } Vship_PixelFormat;
+
+/* Position of the chroma samples relative to the luma grid, as
+ * AVChromaLocation (chromaUpsample.hpp:131-205): co-sited ("left", "top")
+ * puts chroma sample i at luma position 2^s * i, centred at
+ * 2^s * i + (2^s - 1) / 2. */
+typedef enum {
+    Vship_Chroma_Left = 0,    /* horizontally co-sited, vertically centred (MPEG-2, H.264 4:2:0) */
+    Vship_Chroma_Center = 1,  /* centred both ways (JPEG, MPEG-1) */
+    Vship_Chroma_TopLeft = 2, /* co-sited both ways */
+    Vship_Chroma_Top = 3,     /* horizontally centred, vertically co-sited */
+} Vship_ChromaLocation;
 
 typedef enum {
     Vship_Matrix_BT709 = 0,
@@ -44,9 +74,9 @@ typedef struct {
     int width;
     int height;
     int format; /* Vship_PixelFormat */
-    int bits;   /* sample depth for Vship_PF_YUV420_U16 (9..16); ignored otherwise */
+    int bits;   /* sample depth of the 16-bit-container YUV formats (9..16); ignored otherwise */
     int matrix; /* Vship_Matrix for the YUV formats */
-    int reserved;
+    int chroma_location; /* Vship_ChromaLocation for the subsampled YUV formats; 0 for the others */
 } Vship_SSIMU2Pair;
 
 typedef struct {

@@ -225,6 +225,90 @@ static void so_matrix_coefs(int matrix, double* kr, double* kb) {
 
 static int clampi(int v, int lo, int hi) { return v < lo ? lo : (v > hi ? hi : v); }
 
+/* ---------------------------------------------------------------------- */
+/* Any planar YUV layout (definition shared with the GPU front end)       */
+/*                                                                        */
+/*  chroma planes are ceil(w / 2^sw) x ceil(h / 2^sh); chroma sample i of */
+/*  an axis subsampled by f = 2^s sits at luma position f*i (co-sited) or */
+/*  f*i + (f-1)/2 (centred), as the reference's upsampling kernels place  */
+/*  it (src/ffvship_utility/gpuColorToLinear/chromaUpsample.hpp:44-128:   */
+/*  left/top write t = 0 and 1/2 (x2) or k/4 (x4) at luma f*i + k,        */
+/*  centre writes t = 1/4, 3/4 (x2) or 1/8 .. 7/8 (x4) across [i, i+1]).  */
+/*  Luma position x is thus at chroma coordinate u; the value is the      */
+/*  Catmull-Rom interpolant of the four samples floor(u)-1 .. floor(u)+2  */
+/*  (clamped to the plane), horizontally then vertically (:131-205).      */
+/*  Locations (AVChromaLocation): left = co-sited horizontally, centred   */
+/*  vertically; center = centred both ways; topleft = co-sited both ways; */
+/*  top = centred horizontally, co-sited vertically.                       */
+/* ---------------------------------------------------------------------- */
+static void so_chroma_pos(int x, int s, int centred, int* c, double* t) {
+    const double f = (double)(1 << s);
+    const double u = centred ? ((double)x - (f - 1.0) / 2.0) / f : (double)x / f;
+    const double fl = floor(u);
+    *c = (int)fl;
+    *t = u - fl;
+}
+
+/* Catmull-Rom weights from the cubic Hermite basis: value at t of the
+   segment [p1, p2] with tangents (p2 - p0)/2 and (p3 - p1)/2. */
+static void so_cr_weights(double t, double w[4]) {
+    const double t2 = t * t, t3 = t2 * t;
+    const double h00 = 2.0 * t3 - 3.0 * t2 + 1.0, h10 = t3 - 2.0 * t2 + t;
+    const double h01 = -2.0 * t3 + 3.0 * t2, h11 = t3 - t2;
+    w[0] = -0.5 * h10;
+    w[1] = h00 - 0.5 * h11;
+    w[2] = h01 + 0.5 * h10;
+    w[3] = 0.5 * h11;
+}
+
+double so_chroma_sample(const uint8_t* plane, int64_t stride, int w, int h, int bits, int sw, int sh, int loc,
+                        int x, int y) {
+    const int cw = (w + (1 << sw) - 1) >> sw, ch = (h + (1 << sh) - 1) >> sh;
+    const int hc = loc == SO_LOC_CENTER || loc == SO_LOC_TOP;
+    const int vc = loc == SO_LOC_LEFT || loc == SO_LOC_CENTER;
+    int cx, cy;
+    double tx, ty, wx[4], wy[4];
+    so_chroma_pos(x, sw, hc, &cx, &tx);
+    so_chroma_pos(y, sh, vc, &cy, &ty);
+    so_cr_weights(tx, wx);
+    so_cr_weights(ty, wy);
+    double acc = 0.0;
+    for (int a = 0; a < 4; a++) {
+        const int yy = clampi(cy - 1 + a, 0, ch - 1);
+        double hv = 0.0;
+        for (int b = 0; b < 4; b++) hv += wx[b] * so_sample(plane, stride, bits, clampi(cx - 1 + b, 0, cw - 1), yy);
+        acc += wy[a] * hv;
+    }
+    return acc;
+}
+
+int so_yuv_to_rgb_f64(const uint8_t* const p[3], const int64_t stride[3], int w, int h, int bits, int matrix,
+                      int sw, int sh, int loc, double* rgb) {
+    if (w <= 0 || h <= 0 || bits < 8 || bits > 16 || sw < 0 || sw > 2 || sh < 0 || sh > 1 || loc < 0 || loc > 3)
+        return -1;
+    const double sc = (double)(1 << (bits - 8));
+    double kr, kb;
+    so_matrix_coefs(matrix, &kr, &kb);
+    const double kg = 1.0 - kr - kb;
+    const int64_t n = (int64_t)w * h;
+    for (int y = 0; y < h; y++)
+        for (int x = 0; x < w; x++) {
+            const double cbv = so_chroma_sample(p[1], stride[1], w, h, bits, sw, sh, loc, x, y);
+            const double crv = so_chroma_sample(p[2], stride[2], w, h, bits, sw, sh, loc, x, y);
+            const double Y = (so_sample(p[0], stride[0], bits, x, y) - 16.0 * sc) / (219.0 * sc);
+            const double cb = (cbv - 128.0 * sc) / (224.0 * sc);
+            const double cr = (crv - 128.0 * sc) / (224.0 * sc);
+            const double R = Y + 2.0 * (1.0 - kr) * cr;
+            const double B = Y + 2.0 * (1.0 - kb) * cb;
+            const double G = (Y - kr * R - kb * B) / kg;
+            const int64_t o = (int64_t)y * w + x;
+            rgb[o] = R;
+            rgb[n + o] = G;
+            rgb[2 * n + o] = B;
+        }
+    return 0;
+}
+
 int so_yuv420_to_rgb_f64(const uint8_t* const p[3], const int64_t stride[3], int w, int h, int bits, int matrix,
                          double* rgb) {
     if (w <= 0 || h <= 0 || bits < 8 || bits > 16) return -1;

@@ -23,17 +23,21 @@
  *   feature order       src/ssimu2/main.hpp:82-95
  *   weights + score     src/ssimu2/score.hpp:272-401
  *
- * The YUV 4:2:0 front end has no counterpart inside the reference's path
- * (the reference receives R'G'B' from VapourSynth resize / zimg); the
- * definition used here is stated in so_yuv420_to_rgb_* below and is shared
- * with the GPU front end.
+ * The YUV front end has no counterpart inside the reference's SSIMULACRA2
+ * path (the reference receives R'G'B' from VapourSynth resize / zimg); the
+ * definition used here is stated in so_yuv420_to_rgb_* and so_yuv_to_rgb_*
+ * and is shared with the GPU front end. Its chroma positions and
+ * Catmull-Rom weights follow the reference's FFVship upsampling kernels
+ * (src/ffvship_utility/gpuColorToLinear/chromaUpsample.hpp).
  *
  * Pinning: every function is checked by tests/test_oracle.py against
  * closed forms, brute force and the reference's own known answer
  * (identical inputs, any row padding => exactly 100,
  * test/basic_metric_math_test.cpp:40-58). The YUV front end is pinned only
- * by its own closed forms (constant planes, exact Catmull-Rom weights); its
- * equality with zimg / VapourSynth resize is "parity unpinned".
+ * by its own closed forms (constant planes, exact Catmull-Rom weights,
+ * reproduction of linear chroma ramps at the stated sample positions for
+ * every layout and location); its equality with zimg / VapourSynth resize
+ * is "parity unpinned".
  */
 #ifndef SSIMU2_ORACLE_H
 #define SSIMU2_ORACLE_H
@@ -46,8 +50,10 @@ extern "C" {
 
 /* RGB sample formats of the reference's InputMemType (src/util/preprocessor.hpp:94) */
 enum { SO_RGB_U8 = 0, SO_RGB_U16 = 1, SO_RGB_F16 = 2, SO_RGB_F32 = 3 };
-/* YUV matrices for the 4:2:0 front end */
+/* YUV matrices of the front end */
 enum { SO_MAT_BT709 = 0, SO_MAT_BT601 = 1, SO_MAT_BT2020NCL = 2 };
+/* chroma sample locations (AVChromaLocation subset) */
+enum { SO_LOC_LEFT = 0, SO_LOC_CENTER = 1, SO_LOC_TOPLEFT = 2, SO_LOC_TOP = 3 };
 
 /* ---- stage functions (double precision) ------------------------------ */
 int64_t so_total_scale_size(int w, int h);
@@ -81,6 +87,19 @@ int so_score_yuv420_f64(const uint8_t* const p1[3], const int64_t s1[3],
 int so_score_yuv420_f32(const uint8_t* const p1[3], const int64_t s1[3],
                         const uint8_t* const p2[3], const int64_t s2[3], int w, int h,
                         int bits, int matrix, double* score, double* feat108);
+
+/* ---- any planar YUV layout: log2 chroma subsampling sw (0..2), sh (0..1),
+   chroma location loc; 4:2:0 with SO_LOC_LEFT is so_yuv420_* ------------- */
+double so_chroma_sample(const uint8_t* plane, int64_t stride, int w, int h, int bits, int sw, int sh, int loc,
+                        int x, int y);
+int so_yuv_to_rgb_f64(const uint8_t* const p[3], const int64_t stride[3], int w, int h, int bits, int matrix,
+                      int sw, int sh, int loc, double* rgb /* 3*w*h planar */);
+int so_score_yuv_f64(const uint8_t* const p1[3], const int64_t s1[3], const uint8_t* const p2[3],
+                     const int64_t s2[3], int w, int h, int bits, int matrix, int sw, int sh, int loc,
+                     double* score, double* feat108);
+int so_score_yuv_f32(const uint8_t* const p1[3], const int64_t s1[3], const uint8_t* const p2[3],
+                     const int64_t s2[3], int w, int h, int bits, int matrix, int sw, int sh, int loc,
+                     double* score, double* feat108);
 
 /* Score from already linear-RGB planes at scale 0 (3*w*h each, planar). */
 int so_score_linear_f64(const double* lin1, const double* lin2, int w, int h,

@@ -215,6 +215,26 @@ int FN(so_score_rgb)(int fmt, const uint8_t* const p1[3], int64_t stride1,
     return rc;
 }
 
+int FN(so_score_yuv)(const uint8_t* const p1[3], const int64_t s1[3],
+                     const uint8_t* const p2[3], const int64_t s2[3], int w, int h,
+                     int bits, int matrix, int sw, int sh, int loc, double* score, double* feat108) {
+    if (w <= 0 || h <= 0) return -1;
+    const int64_t n = (int64_t)w * h;
+    double* rgb = (double*)malloc(sizeof(double) * n * 3);
+    REAL* a = (REAL*)malloc(sizeof(REAL) * n * 3);
+    REAL* b = (REAL*)malloc(sizeof(REAL) * n * 3);
+    int rc = -1;
+    if (!rgb || !a || !b) goto done;
+    if (so_yuv_to_rgb_f64(p1, s1, w, h, bits, matrix, sw, sh, loc, rgb)) goto done;
+    for (int64_t i = 0; i < 3 * n; i++) a[i] = (REAL)so_linearize(rgb[i]);
+    if (so_yuv_to_rgb_f64(p2, s2, w, h, bits, matrix, sw, sh, loc, rgb)) goto done;
+    for (int64_t i = 0; i < 3 * n; i++) b[i] = (REAL)so_linearize(rgb[i]);
+    rc = FN(score_from_linear)(a, b, w, h, score, feat108);
+done:
+    free(rgb); free(a); free(b);
+    return rc;
+}
+
 int FN(so_score_yuv420)(const uint8_t* const p1[3], const int64_t s1[3],
                         const uint8_t* const p2[3], const int64_t s2[3], int w, int h,
                         int bits, int matrix, double* score, double* feat108) {

@@ -53,9 +53,9 @@ def oracle_rgb(ref, dist, fmt):
 
 
 def gpu_batch(pairs, fmt, bits=8, matrix=api.MATRIX_BT709, fp16=False, th=0, graph=False, features=True,
-              persistent_grid=0):
+              persistent_grid=0, chroma_location=api.CHROMA_LEFT):
     """Scores (and features) of a list of (ref, dist) host pairs through the batched C ABI."""
-    dp = api.DevicePairs(pairs, fmt, bits, matrix)
+    dp = api.DevicePairs(pairs, fmt, bits, matrix, chroma_location)
     mw = max(p[0][0].shape[1] for p in pairs)
     mh = max(p[0][0].shape[0] for p in pairs)
     with api.SSIMU2Batch(len(pairs), mw, mh, fp16=fp16, map_tile_height=th, use_graph=graph,

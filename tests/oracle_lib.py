@@ -47,6 +47,14 @@ def lib():
             fn = getattr(L, name)
             fn.restype = ctypes.c_int
             fn.argtypes = [P3, S3, P3, S3, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int, D, D]
+        for name in ("so_score_yuv_f64", "so_score_yuv_f32"):
+            fn = getattr(L, name)
+            fn.restype = ctypes.c_int
+            fn.argtypes = [P3, S3, P3, S3] + [ctypes.c_int] * 7 + [D, D]
+        L.so_yuv_to_rgb_f64.restype = ctypes.c_int
+        L.so_yuv_to_rgb_f64.argtypes = [P3, S3] + [ctypes.c_int] * 7 + [D]
+        L.so_chroma_sample.restype = ctypes.c_double
+        L.so_chroma_sample.argtypes = [ctypes.c_void_p, ctypes.c_int64] + [ctypes.c_int] * 8
         L.so_yuv420_to_rgb_f64.restype = ctypes.c_int
         L.so_yuv420_to_rgb_f64.argtypes = [P3, S3, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int, D]
         L.so_score_linear_f64.restype = ctypes.c_int
@@ -177,6 +185,40 @@ def score_yuv420(ref, dist, bits, matrix, precision="f64"):
     fn = lib().so_score_yuv420_f64 if precision == "f64" else lib().so_score_yuv420_f32
     rc = fn(_planes(p1), S3(*[x.strides[0] for x in p1]), _planes(p2), S3(*[x.strides[0] for x in p2]),
             w, h, bits, matrix, ctypes.byref(score), _dp(feat))
+    assert rc == 0
+    return score.value, feat
+
+
+# YUV layouts: name -> (log2 horizontal, log2 vertical chroma subsampling),
+# and chroma locations (include/VshipSSIMU2Batch.h)
+LAYOUTS = {"420": (1, 1), "422": (1, 0), "444": (0, 0), "440": (0, 1), "411": (2, 0), "410": (2, 1)}
+SO_LOC_LEFT, SO_LOC_CENTER, SO_LOC_TOPLEFT, SO_LOC_TOP = range(4)
+
+
+def chroma_sample(plane, w, h, bits, sw, sh, loc, x, y):
+    p = np.ascontiguousarray(plane)
+    return lib().so_chroma_sample(p.ctypes.data, p.strides[0], w, h, bits, sw, sh, loc, x, y)
+
+
+def yuv_to_rgb_any(planes, bits, matrix, sw, sh, loc):
+    p = [np.ascontiguousarray(x) for x in planes]
+    h, w = p[0].shape
+    out = np.zeros((3, h, w))
+    rc = lib().so_yuv_to_rgb_f64(_planes(p), S3(*[x.strides[0] for x in p]), w, h, bits, matrix, sw, sh, loc,
+                                 _dp(out))
+    assert rc == 0
+    return out
+
+
+def score_yuv(ref, dist, bits, matrix, sw, sh, loc, precision="f64"):
+    p1 = [np.ascontiguousarray(x) for x in ref]
+    p2 = [np.ascontiguousarray(x) for x in dist]
+    h, w = p1[0].shape
+    score = ctypes.c_double()
+    feat = np.zeros(108)
+    fn = lib().so_score_yuv_f64 if precision == "f64" else lib().so_score_yuv_f32
+    rc = fn(_planes(p1), S3(*[x.strides[0] for x in p1]), _planes(p2), S3(*[x.strides[0] for x in p2]),
+            w, h, bits, matrix, sw, sh, loc, ctypes.byref(score), _dp(feat))
     assert rc == 0
     return score.value, feat
 

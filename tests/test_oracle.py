@@ -435,3 +435,110 @@ def test_yuv16_samples_above_depth_are_clamped():
     b = ol.yuv_to_rgb(clamped, 10, ol.SO_MAT_BT2020NCL)
     assert np.array_equal(a, b)
     assert not np.array_equal(a, ol.yuv_to_rgb(planes, 10, ol.SO_MAT_BT2020NCL))
+
+
+# ------------------------------------------------- YUV front end: every layout
+def _ref_chroma_coords(f, centred, n_chroma):
+    """Luma index -> chroma coordinate, enumerated from the writes of the
+    reference's upsampling kernels (src/ffvship_utility/gpuColorToLinear/
+    chromaUpsample.hpp): left/top x2 write dst[2x] <- t=0, dst[2x+1] <- t=.5
+    of interval x (:58-68, :116-128); left x4 write dst[4x+k] <- t=k/4
+    (:86-98); centre x2 write dst[2x+1] <- t=.25, dst[2x+2] <- t=.75 (:44-56,
+    :100-114); centre x4 write dst[4x+2..4x+5] <- t=1/8, 3/8, 5/8, 7/8
+    (:70-84). No subsampling: luma x is chroma x."""
+    coords = {}
+    for x in range(-1, n_chroma):
+        if f == 1:
+            coords[x] = float(x)
+        elif not centred:
+            for k in range(f):
+                coords[f * x + k] = x + k / f
+        elif f == 2:
+            coords[2 * x + 1], coords[2 * x + 2] = x + 0.25, x + 0.75
+        else:
+            for k, t in zip(range(2, 6), (0.125, 0.375, 0.625, 0.875)):
+                coords[4 * x + k] = x + t
+    return coords
+
+
+@pytest.mark.parametrize("layout", list(ol.LAYOUTS))
+@pytest.mark.parametrize("loc", [ol.SO_LOC_LEFT, ol.SO_LOC_CENTER, ol.SO_LOC_TOPLEFT, ol.SO_LOC_TOP])
+def test_yuv_any_layout_quadratic_chroma_lands_at_reference_positions(layout, loc):
+    # Catmull-Rom reproduces quadratics (central differences are exact for
+    # them), so away from the clamped edges a quadratic chroma field comes
+    # back exactly at the chroma coordinate of each luma sample
+    sw, sh = ol.LAYOUTS[layout]
+    fx, fy = 1 << sw, 1 << sh
+    w, h = 48 + 3, 40 + 1
+    cw, ch = -(-w // fx), -(-h // fy)
+    hc = loc in (ol.SO_LOC_CENTER, ol.SO_LOC_TOP)
+    vc = loc in (ol.SO_LOC_LEFT, ol.SO_LOC_CENTER)
+    q = lambda i, j: 300.0 + 7.0 * i - 5.0 * j + 0.25 * i * i + 0.5 * j * j
+    jj, ii = np.meshgrid(np.arange(cw), np.arange(ch))
+    u = (q(jj, ii) * 16).astype(np.uint16)  # integer codes below 2^16; read as bits = 16
+    assert u.max() < 65535 and np.array_equal(u, q(jj, ii) * 16)
+    cx = _ref_chroma_coords(fx, hc, cw)
+    cy = _ref_chroma_coords(fy, vc, ch)
+    checked = 0
+    for y in range(h):
+        for x in range(w):
+            px, py = cx[x], cy[y]
+            if not (1 <= px <= cw - 3 and 1 <= py <= ch - 3):
+                continue  # a tap would be clamped
+            got = ol.chroma_sample(u, w, h, 16, sw, sh, loc, x, y)
+            assert got == pytest.approx(q(px, py) * 16, abs=1e-7), (x, y, px, py)
+            checked += 1
+    assert checked > 200
+
+
+@pytest.mark.parametrize("layout", list(ol.LAYOUTS))
+def test_yuv_any_layout_constant_planes(layout):
+    sw, sh = ol.LAYOUTS[layout]
+    w, h = 17, 11
+    cw, ch = -(-w // (1 << sw)), -(-h // (1 << sh))
+    for loc in range(4):
+        y = np.full((h, w), 150, np.uint8)
+        u = np.full((ch, cw), 60, np.uint8)
+        v = np.full((ch, cw), 200, np.uint8)
+        rgb = ol.yuv_to_rgb_any([y, u, v], 8, 0, sw, sh, loc)
+        kr, kb = _kr_kb(0)
+        Y, cb, cr = (150 - 16) / 219, (60 - 128) / 224, (200 - 128) / 224
+        R, B = Y + 2 * (1 - kr) * cr, Y + 2 * (1 - kb) * cb
+        assert np.allclose(rgb[0], R, atol=1e-13) and np.allclose(rgb[2], B, atol=1e-13)
+        assert np.allclose(rgb[1], (Y - kr * R - kb * B) / (1 - kr - kb), atol=1e-13)
+
+
+def test_yuv444_takes_chroma_as_is():
+    rng = np.random.default_rng(8)
+    w, h = 19, 7
+    y, u, v = (rng.integers(16, 241, (h, w), dtype=np.uint8) for _ in range(3))
+    for loc in range(4):
+        rgb = ol.yuv_to_rgb_any([y, u, v], 8, 1, 0, 0, loc)
+        kr, kb = _kr_kb(1)
+        Y = (y - 16.0) / 219
+        assert np.allclose(rgb[0], Y + 2 * (1 - kr) * (v - 128.0) / 224, atol=1e-13)
+        assert np.allclose(rgb[2], Y + 2 * (1 - kb) * (u - 128.0) / 224, atol=1e-13)
+
+
+def test_yuv_any_layout_420_left_is_the_420_front_end():
+    rng = np.random.default_rng(31)
+    for w, h in [(21, 13), (32, 18), (1, 1), (2, 3)]:
+        cw, ch = (w + 1) // 2, (h + 1) // 2
+        planes = [rng.integers(0, 1024, (h, w)).astype(np.uint16),
+                  rng.integers(0, 1024, (ch, cw)).astype(np.uint16),
+                  rng.integers(0, 1024, (ch, cw)).astype(np.uint16)]
+        assert np.array_equal(ol.yuv_to_rgb_any(planes, 10, 2, 1, 1, ol.SO_LOC_LEFT), ol.yuv_to_rgb(planes, 10, 2))
+
+
+def test_yuv411_centre_random_interior_brute_force():
+    # 4:1:1 with centred chroma: fractions 1/8 .. 7/8 of the Hermite basis
+    rng = np.random.default_rng(12)
+    w, h = 40, 5
+    cw = w // 4
+    u = rng.integers(16, 241, (h, cw), dtype=np.uint8)
+    for x in range(w):
+        c = (x - 1.5) / 4
+        k0 = int(np.floor(c))
+        wts = _hermite_weights(c - k0)
+        exp = sum(wts[b] * u[3, min(max(k0 - 1 + b, 0), cw - 1)] for b in range(4))
+        assert ol.chroma_sample(u, w, h, 8, 2, 0, ol.SO_LOC_CENTER, x, 3) == pytest.approx(exp, abs=1e-9)

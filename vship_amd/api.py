@@ -17,9 +17,16 @@ ERRORS = [
     "NoDeviceDetected", "BadDeviceArgument", "BadDeviceCode", "BadHandler", "BadPointer", "BadErrorType",
 ]
 
-# Vship_PixelFormat / Vship_Matrix (include/VshipSSIMU2Batch.h)
-PF_RGB_U8, PF_RGB_U16, PF_RGB_F16, PF_RGB_F32, PF_YUV420_U8, PF_YUV420_U16 = range(6)
+# Vship_PixelFormat / Vship_Matrix / Vship_ChromaLocation (include/VshipSSIMU2Batch.h)
+(PF_RGB_U8, PF_RGB_U16, PF_RGB_F16, PF_RGB_F32, PF_YUV420_U8, PF_YUV420_U16, PF_YUV422_U8, PF_YUV422_U16,
+ PF_YUV444_U8, PF_YUV444_U16, PF_YUV440_U8, PF_YUV440_U16, PF_YUV411_U8, PF_YUV411_U16, PF_YUV410_U8,
+ PF_YUV410_U16) = range(16)
 MATRIX_BT709, MATRIX_BT601, MATRIX_BT2020NCL = range(3)
+CHROMA_LEFT, CHROMA_CENTER, CHROMA_TOPLEFT, CHROMA_TOP = range(4)
+# YUV layout name -> (8-bit format, 16-bit-container format, log2 horizontal, log2 vertical subsampling)
+YUV_LAYOUTS = {"420": (PF_YUV420_U8, PF_YUV420_U16, 1, 1), "422": (PF_YUV422_U8, PF_YUV422_U16, 1, 0),
+               "444": (PF_YUV444_U8, PF_YUV444_U16, 0, 0), "440": (PF_YUV440_U8, PF_YUV440_U16, 0, 1),
+               "411": (PF_YUV411_U8, PF_YUV411_U16, 2, 0), "410": (PF_YUV410_U8, PF_YUV410_U16, 2, 1)}
 
 
 class VshipNotBuilt(RuntimeError):
@@ -51,7 +58,7 @@ class Pair(ctypes.Structure):
     _fields_ = [("ref", ctypes.c_void_p * 3), ("dist", ctypes.c_void_p * 3),
                 ("ref_stride", ctypes.c_int64 * 3), ("dist_stride", ctypes.c_int64 * 3),
                 ("width", ctypes.c_int), ("height", ctypes.c_int), ("format", ctypes.c_int),
-                ("bits", ctypes.c_int), ("matrix", ctypes.c_int), ("reserved", ctypes.c_int)]
+                ("bits", ctypes.c_int), ("matrix", ctypes.c_int), ("chroma_location", ctypes.c_int)]
 
 
 class BatchConfig(ctypes.Structure):
@@ -342,7 +349,7 @@ class DevicePairs:
     """Host frame pairs copied into one device allocation, with the
     Vship_SSIMU2Pair descriptors that point into it."""
 
-    def __init__(self, pairs, fmt, bits=8, matrix=MATRIX_BT709):
+    def __init__(self, pairs, fmt, bits=8, matrix=MATRIX_BT709, chroma_location=CHROMA_LEFT):
         """pairs: list of (ref_planes, dist_planes), each three 2-D numpy arrays."""
         offs, total = [], 0
         for ref, dist in pairs:
@@ -367,6 +374,7 @@ class DevicePairs:
             d.format = fmt
             d.bits = bits
             d.matrix = matrix
+            d.chroma_location = chroma_location
         self.n = len(pairs)
         self.nbytes = total
 

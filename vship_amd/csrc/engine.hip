@@ -14,13 +14,18 @@ namespace {
 int64_t round_up(int64_t v, int64_t a) { return (v + a - 1) / a * a; }
 
 int input_bytes_per_sample(int fmt) {
+    if (fmt_yuv(fmt)) return fmt_wide(fmt) ? 2 : 1;
     switch (fmt) {
-    case IN_RGB_U8:
-    case IN_YUV420_U8: return 1;
+    case IN_RGB_U8: return 1;
     case IN_RGB_F32: return 4;
     default: return 2;
     }
 }
+
+// Samples per row and rows of plane pl (chroma planes of a YUV format are
+// ceil(w / 2^sw) x ceil(h / 2^sh)).
+int plane_w(const PairInput& p, int pl) { return pl > 0 ? (p.w + (1 << fmt_csw(p.fmt)) - 1) >> fmt_csw(p.fmt) : p.w; }
+int plane_h(const PairInput& p, int pl) { return pl > 0 ? (p.h + (1 << fmt_csh(p.fmt)) - 1) >> fmt_csh(p.fmt) : p.h; }
 
 // Normalised 17-tap Gaussian, sigma 1.5, radius 8 (src/ssimu2/gaussianblur.hpp:7-18,
 // src/util/preprocessor.hpp:89-92), recomputed in double and checked against
@@ -52,11 +57,10 @@ void matrix_coefs(int matrix, float& kr, float& kb) {
 // image: a bad pointer or stride becomes an error here instead of a GPU fault.
 void validate_planes(const PairInput& p) {
     const int bps = input_bytes_per_sample(p.fmt);
-    const bool yuv = p.fmt >= IN_YUV420_U8;
     for (int im = 0; im < 2; im++)
         for (int pl = 0; pl < 3; pl++) {
-            const int pw = (yuv && pl > 0) ? (p.w + 1) / 2 : p.w;
-            const int ph = (yuv && pl > 0) ? (p.h + 1) / 2 : p.h;
+            const int pw = plane_w(p, pl);
+            const int ph = plane_h(p, pl);
             const uint8_t* ptr = p.src[im][pl];
             const int64_t row = (int64_t)pw * bps;
             if (!ptr || p.stride[im][pl] < row) VSHIP_THROW(E_BadPointer);
@@ -119,11 +123,8 @@ uint64_t map_algorithmic_bytes(int w, int h, bool fp16) {
 
 uint64_t prep_algorithmic_bytes(const PairInput& p, bool fp16) {
     const uint64_t bps = input_bytes_per_sample(p.fmt);
-    uint64_t in;
-    if (p.fmt >= IN_YUV420_U8)
-        in = bps * ((uint64_t)p.w * p.h + 2ull * ((p.w + 1) / 2) * ((p.h + 1) / 2));
-    else
-        in = bps * 3ull * p.w * p.h;
+    uint64_t in = 0;
+    for (int pl = 0; pl < 3; pl++) in += bps * (uint64_t)plane_w(p, pl) * plane_h(p, pl);
     return 2 * in + map_algorithmic_bytes(p.w, p.h, fp16);  // read inputs, write every XYB plane once
 }
 
@@ -208,8 +209,9 @@ void Engine::plan(const PairInput* in, int n) {
     uint64_t mbytes = 0, pbytes = 0;
     for (int i = 0; i < n; i++) {
         const PairInput& p = in[i];
-        if (p.w < 1 || p.h < 1 || p.fmt < IN_RGB_U8 || p.fmt > IN_YUV420_U16) VSHIP_THROW(E_BadPointer);
-        if (p.fmt == IN_YUV420_U16 && (p.bits < 9 || p.bits > 16)) VSHIP_THROW(E_BadPointer);
+        if (p.w < 1 || p.h < 1 || p.fmt < IN_RGB_U8 || p.fmt >= IN_NUM_FORMATS) VSHIP_THROW(E_BadPointer);
+        if (fmt_wide(p.fmt) && (p.bits < 9 || p.bits > 16)) VSHIP_THROW(E_BadPointer);
+        if (p.chroma_loc < CLOC_LEFT || p.chroma_loc > CLOC_TOP) VSHIP_THROW(E_BadPointer);
         validate_planes(p);
         const PairLayout L = pair_layout(p.w, p.h, opt_.fp16);
         PairDev& d = pd[i];
@@ -220,7 +222,10 @@ void Engine::plan(const PairInput* in, int n) {
                 d.stride[im][pl] = p.stride[im][pl];
             }
         d.fmt = p.fmt;
-        d.bits = p.fmt == IN_YUV420_U8 ? 8 : (p.fmt == IN_YUV420_U16 ? p.bits : 0);
+        d.bits = fmt_yuv(p.fmt) ? (fmt_wide(p.fmt) ? p.bits : 8) : 0;
+        d.csw = fmt_csw(p.fmt);
+        d.csh = fmt_csh(p.fmt);
+        d.cloc = fmt_yuv(p.fmt) ? p.chroma_loc : CLOC_LEFT;
         matrix_coefs(p.matrix, d.kr, d.kb);
         d.w = p.w;
         d.h = p.h;
@@ -278,8 +283,12 @@ void Engine::plan(const PairInput* in, int n) {
     order_start[k] = (int32_t)mtiles;
     if (ptiles > INT32_MAX) VSHIP_THROW(E_OutOfVRAM);
     prep_start[n] = (int32_t)ptiles;
+    int ngen = 0;
+    for (int i = 0; i < n; i++) ngen += prep_generic(pd[i].fmt, pd[i].csw, pd[i].csh, pd[i].cloc) ? 1 : 0;
     hdr->npairs = n;
     hdr->prep_tiles = (int32_t)ptiles;
+    hdr->prep_generic_pairs = ngen;
+    nprep_generic_ = ngen;
     hdr->map_tiles = (int32_t)mtiles;
     hdr->map_th = opt_.map_th;
     for (int g = 0; g < kMapGroups; g++) {
@@ -317,7 +326,11 @@ void Engine::enqueue_body() {
     if (prof) HIP_OR_THROW(hipEventRecord(ev_[0], stream_), E_BadDeviceCode);
     HIP_OR_THROW(hipMemcpyAsync(desc_d_, desc_h_, copy_bytes, hipMemcpyHostToDevice, stream_), E_BadDeviceCode);
     if (prof) HIP_OR_THROW(hipEventRecord(ev_[1], stream_), E_BadDeviceCode);
-    HIP_OR_THROW(launch_prep(opt_.fp16, g_prep, pd, prep_start, hdr, arena_d_, stream_), E_BadDeviceCode);
+    HIP_OR_THROW(launch_prep(opt_.fp16, false, g_prep, pd, prep_start, hdr, arena_d_, stream_), E_BadDeviceCode);
+    // pairs on the generic chroma path have their own prep kernel; a captured
+    // graph always holds it (it returns at once when the batch has none)
+    if (graph || nprep_generic_ > 0)
+        HIP_OR_THROW(launch_prep(opt_.fp16, true, g_prep, pd, prep_start, hdr, arena_d_, stream_), E_BadDeviceCode);
     if (prof) HIP_OR_THROW(hipEventRecord(ev_[2], stream_), E_BadDeviceCode);
     // one launch per map group; FP16-plane groups exist only with FP16 storage,
     // and an eager launch is skipped when its group is empty

@@ -56,6 +56,7 @@ struct PairInput {
     int fmt;     // InFmt
     int bits;
     int matrix;  // 0 BT.709, 1 BT.601, 2 BT.2020 NCL
+    int chroma_loc = 0;  // ChromaLoc of the subsampled YUV formats
 };
 
 struct EngineOptions {
@@ -140,6 +141,7 @@ private:
     hipGraph_t graph_ = nullptr;
     hipGraphExec_t graph_exec_ = nullptr;
 
+    int nprep_generic_ = 0;  // pairs of the batch on the generic chroma path
     int npairs_ = 0, ntiles_prep_ = 0, ntiles_map_ = 0, ntiles_cls_[2] = {0, 0}, ntiles_grp_[kMapGroups] = {};
     uint64_t map_bytes_ = 0, prep_bytes_ = 0, cls_bytes_[2] = {0, 0};
     bool pending_ = false;

@@ -109,6 +109,7 @@ public:
             in.fmt = p.format;
             in.bits = p.bits;
             in.matrix = p.matrix;
+            in.chroma_loc = p.chroma_location;
             if (p.width < 1 || p.height < 1 || p.width > cfg_.max_width || p.height > cfg_.max_height)
                 VSHIP_THROW(E_BadPointer);
             c->footprint = std::max<int64_t>(c->footprint, pair_bytes(p.width, p.height, fp16_));

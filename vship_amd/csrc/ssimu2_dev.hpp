@@ -21,7 +21,9 @@ constexpr int kFeat = 18;       // partial sums per map tile: 3 channels x 3 map
 constexpr int kHalfScales = 2;  // scales stored in FP16 when FP16 storage is on
 
 // Input sample formats. RGB formats are the reference's InputMemType
-// (src/util/preprocessor.hpp:94); YUV 4:2:0 is the fused front end.
+// (src/util/preprocessor.hpp:94); the planar YUV formats are the fused front
+// end, in the chroma layouts of src/ffvship_utility/gpuColorToLinear/
+// vshipColor.hpp:12-128. YUV codes alternate 8-bit and 16-bit containers.
 enum InFmt : int32_t {
     IN_RGB_U8 = 0,
     IN_RGB_U16 = 1,
@@ -29,13 +31,52 @@ enum InFmt : int32_t {
     IN_RGB_F32 = 3,
     IN_YUV420_U8 = 4,
     IN_YUV420_U16 = 5,  // 9..16-bit samples in little-endian 16-bit containers
+    IN_YUV422_U8 = 6,
+    IN_YUV422_U16 = 7,
+    IN_YUV444_U8 = 8,
+    IN_YUV444_U16 = 9,
+    IN_YUV440_U8 = 10,
+    IN_YUV440_U16 = 11,
+    IN_YUV411_U8 = 12,
+    IN_YUV411_U16 = 13,
+    IN_YUV410_U8 = 14,
+    IN_YUV410_U16 = 15,
+    IN_NUM_FORMATS = 16,
 };
+
+__host__ __device__ constexpr bool fmt_yuv(int f) { return f >= IN_YUV420_U8; }
+// 16-bit container (YUV codes are odd for 16-bit, even for 8-bit)
+__host__ __device__ constexpr bool fmt_wide(int f) { return fmt_yuv(f) && (f & 1); }
+// log2 of the horizontal / vertical chroma subsampling of a YUV format, by
+// layout: 4:2:0, 4:2:2, 4:4:4, 4:4:0, 4:1:1, 4:1:0
+__host__ __device__ constexpr int fmt_csw(int f) {
+    constexpr int8_t kW[6] = {1, 1, 0, 0, 2, 2};
+    return fmt_yuv(f) ? kW[(f - IN_YUV420_U8) >> 1] : 0;
+}
+__host__ __device__ constexpr int fmt_csh(int f) {
+    constexpr int8_t kH[6] = {1, 0, 0, 1, 0, 1};
+    return fmt_yuv(f) ? kH[(f - IN_YUV420_U8) >> 1] : 0;
+}
+
+// Chroma sample positions (AVChromaLocation,
+// src/ffvship_utility/gpuColorToLinear/chromaUpsample.hpp:131-205)
+enum ChromaLoc : int32_t { CLOC_LEFT = 0, CLOC_CENTER = 1, CLOC_TOPLEFT = 2, CLOC_TOP = 3 };
+__host__ __device__ constexpr bool cloc_hcentred(int c) { return c == CLOC_CENTER || c == CLOC_TOP; }
+__host__ __device__ constexpr bool cloc_vcentred(int c) { return c == CLOC_LEFT || c == CLOC_CENTER; }
+
+// A YUV pair whose chroma is not 4:2:0 with left-located samples goes through
+// the generic chroma path, in its own prep kernel instantiation.
+__host__ __device__ constexpr bool prep_generic(int fmt, int csw, int csh, int cloc) {
+    return fmt_yuv(fmt) && !(csw == 1 && csh == 1 && cloc == CLOC_LEFT);
+}
 
 struct PairDev {
     const uint8_t* src[2][3];   // [image][plane] device pointers
     int64_t stride[2][3];       // bytes per row
     int32_t fmt;
     int32_t bits;               // YUV sample depth
+    int32_t csw, csh;           // log2 chroma subsampling (YUV)
+    int32_t cloc;               // ChromaLoc (YUV)
     float kr, kb;               // YUV matrix coefficients
     int32_t w, h;               // scale-0 size
     int64_t arena_off;          // byte offset of the pair's region
@@ -49,6 +90,7 @@ struct PairDev {
 struct BatchHdr {
     int32_t npairs;
     int32_t prep_tiles;  // total convert/pyramid tiles
+    int32_t prep_generic_pairs;  // pairs on the generic chroma path (prep_generic)
     int32_t map_tiles;   // total map tiles of all groups
     int32_t map_th;      // map tile height of the FP32 groups
     // Map segments (one per pair and scale) fall in two classes: class 0

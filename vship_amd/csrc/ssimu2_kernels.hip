@@ -148,10 +148,74 @@ __device__ __forceinline__ float rgb_sample(const uint8_t* plane, int64_t stride
     else return as_global(reinterpret_cast<const float*>(row))[x];
 }
 
+// Catmull-Rom weights of samples c-1 .. c+2 at fraction t of [c, c+1]: the
+// cubic Hermite interpolant with central-difference tangents
+// (src/ffvship_utility/gpuColorToLinear/chromaUpsample.hpp:5-24). Exact in
+// FP32 for the fractions k/8 that occur here.
+__device__ __forceinline__ void cr_weights(float t, float w[4]) {
+    w[0] = 0.5f * t * ((2.0f - t) * t - 1.0f);
+    w[1] = 0.5f * ((3.0f * t - 5.0f) * t * t + 2.0f);
+    w[2] = 0.5f * t * ((4.0f - 3.0f * t) * t + 1.0f);
+    w[3] = 0.5f * t * t * (t - 1.0f);
+}
+
+// Chroma interval c and fraction t of luma position x under subsampling 2^s:
+// chroma sample i sits at luma position 2^s i (co-sited) or
+// 2^s i + (2^s - 1) / 2 (centred), so x maps to u = x / 2^s or
+// (2x + 1 - 2^s) / 2^(s+1), c = floor(u), t = u - c
+// (chromaUpsample.hpp:44-128: left/top t = 0, 1/2 or k/4; centre 1/4, 3/4 or
+// 1/8, 3/8, 5/8, 7/8).
+__device__ __forceinline__ void chroma_pos(int x, int s, bool centred, int& c, float& t) {
+    if (!centred) {
+        c = x >> s;
+        t = (float)(x & ((1 << s) - 1)) / (float)(1 << s);
+    } else {
+        const int n = 2 * x + 1 - (1 << s);
+        c = n >> (s + 1);  // floor, n may be negative
+        t = (float)(n & ((2 << s) - 1)) / (float)(2 << s);
+    }
+}
+
+// One row of chroma at interval cx: the sample itself without horizontal
+// subsampling, else the four-tap horizontal interpolation (indices clamped).
+template <bool WIDE>
+__device__ __forceinline__ float chroma_row(const uint8_t* plane, int64_t stride, int cw, int y, int cx,
+                                            const float wx[4], bool direct, int maxcode) {
+    if (direct) return sample_u(plane, stride, cx, y, WIDE, maxcode);
+    float a = 0.f;
+#pragma unroll
+    for (int b = 0; b < 4; b++) a = fmaf(wx[b], sample_u(plane, stride, min(max(cx - 1 + b, 0), cw - 1), y, WIDE, maxcode), a);
+    return a;
+}
+
+// Chroma sample value at luma pixel (x, y) for any layout and chroma location:
+// horizontal interpolation of up to four rows, then vertical
+// (chromaUpsample.hpp:131-205 order).
+template <bool WIDE>
+__device__ __forceinline__ float chroma_at(const uint8_t* plane, int64_t stride, int cw, int chh, int x, int y,
+                                           int csw, int csh, bool hc, bool vc, int maxcode) {
+    int cx, cy;
+    float tx, ty, wx[4], wy[4];
+    chroma_pos(x, csw, hc, cx, tx);
+    chroma_pos(y, csh, vc, cy, ty);
+    cr_weights(tx, wx);
+    cr_weights(ty, wy);
+    if (csh == 0) return chroma_row<WIDE>(plane, stride, cw, cy, cx, wx, csw == 0, maxcode);
+    float a = 0.f;
+#pragma unroll
+    for (int r = 0; r < 4; r++)
+        a = fmaf(wy[r], chroma_row<WIDE>(plane, stride, cw, min(max(cy - 1 + r, 0), chh - 1), cx, wx, csw == 0, maxcode),
+                 a);
+    return a;
+}
+
 // Scales 0 and 1 of one 64x64 tile of an image of input format FMT: a thread
 // owns 2x2 pixel quads of scale 0, so the scale-1 child is formed in
-// registers and also left in LDS (l1) for the coarser levels.
-template <typename ST, int FMT>
+// registers and also left in LDS (l1) for the coarser levels. For YUV, FMT is
+// IN_YUV420_U8 or IN_YUV420_U16 (the container), and GEN selects the generic
+// chroma path (any layout and location) over the LDS-staged 4:2:0 one
+// (left-located chroma).
+template <typename ST, int FMT, bool GEN = false>
 __device__ __forceinline__ void prep_quads(PrepLds& L, const PairDev& P, uint8_t* arena, int img, int tx, int ty) {
     constexpr bool yuv = FMT >= IN_YUV420_U8;
     constexpr bool wide = FMT == IN_YUV420_U16;
@@ -162,7 +226,7 @@ __device__ __forceinline__ void prep_quads(PrepLds& L, const PairDev& P, uint8_t
     const int64_t* stride = P.stride[img];
     const int maxcode = wide ? (1 << P.bits) - 1 : 255;
 
-    if constexpr (yuv) {
+    if constexpr (yuv && !GEN) {
         // stage the 36x36 chroma neighbourhood (indices clamped to the plane)
         const int cw = (W + 1) >> 1, chh = (H + 1) >> 1;
         const int cx0 = x0 / 2 - 1, cy0 = y0 / 2 - 2;
@@ -178,7 +242,10 @@ __device__ __forceinline__ void prep_quads(PrepLds& L, const PairDev& P, uint8_t
     // limited-range YUV -> R'G'B' constants (uniform)
     const float sc = yuv ? (float)(1 << (P.bits - 8)) : 1.0f;
     const float ymul = 1.0f / (219.0f * sc), yadd = -16.0f / 219.0f;
-    const float cmul = 1.0f / (128.0f * 224.0f * sc), cadd = -128.0f / 224.0f;
+    // the staged 4:2:0 path sums its vertical taps with integer weights of
+    // total 128, so its chroma carries a factor 128 that cmul takes out;
+    // the generic path's weights are normalised (cmul1)
+    const float cmul = 1.0f / (128.0f * 224.0f * sc), cmul1 = 1.0f / (224.0f * sc), cadd = -128.0f / 224.0f;
     const float kr = P.kr, kb = P.kb, ikg = 1.0f / (1.0f - P.kr - P.kb);
     const float rcr = 2.0f * (1.0f - kr), bcb = 2.0f * (1.0f - kb);
 
@@ -205,28 +272,43 @@ __device__ __forceinline__ void prep_quads(PrepLds& L, const PairDev& P, uint8_t
                                  sample_u(src[0], stride[0], xa, yb, wide, maxcode),
                                  sample_u(src[0], stride[0], xb, ya, wide, maxcode),
                                  sample_u(src[0], stride[0], xb, yb, wide, maxcode)};
-            // chroma: local staged rows qy..qy+4, columns qx..qx+3
             float cbcr[2][4];
+            if constexpr (GEN) {
+                const int csw = P.csw, csh = P.csh;
+                const bool hc = cloc_hcentred(P.cloc), vc = cloc_vcentred(P.cloc);
+                const int cw = (W + (1 << csw) - 1) >> csw, chh = (H + (1 << csh) - 1) >> csh;
 #pragma unroll
-            for (int pl = 0; pl < 2; pl++) {
-                float he[5], ho[5];  // horizontal: even luma column (left-sited sample), odd (Catmull-Rom at 0.5)
+                for (int k = 0; k < 4; k++) {
+                    const int px = (k & 2) ? xb : xa, py = (k & 1) ? yb : ya;
 #pragma unroll
-                for (int r = 0; r < 5; r++) {
-                    const float* row = L.ch[pl][qy + r];
-                    he[r] = row[qx + 1];
-                    ho[r] = (9.0f * (row[qx + 1] + row[qx + 2]) - (row[qx] + row[qx + 3])) * (1.0f / 16.0f);
+                    for (int pl = 0; pl < 2; pl++)
+                        cbcr[pl][k] = fmaf(chroma_at<wide>(src[pl + 1], stride[pl + 1], cw, chh, px, py, csw, csh, hc,
+                                                           vc, maxcode),
+                                           cmul1, cadd);
                 }
-                // vertical, centre-sited: even luma row t = 0.75 from rows 0..3, odd row t = 0.25 from rows 1..4
-                const float ve_e = -3.0f * he[0] + 29.0f * he[1] + 111.0f * he[2] - 9.0f * he[3];
-                const float vo_e = -9.0f * he[1] + 111.0f * he[2] + 29.0f * he[3] - 3.0f * he[4];
-                const float ve_o = -3.0f * ho[0] + 29.0f * ho[1] + 111.0f * ho[2] - 9.0f * ho[3];
-                const float vo_o = -9.0f * ho[1] + 111.0f * ho[2] + 29.0f * ho[3] - 3.0f * ho[4];
-                // (xa,ya): even col, even row; (xa,yb): even col, odd row; (xb,..): odd col.
-                // A clamped column/row repeats the first one.
-                cbcr[pl][0] = fmaf(ve_e, cmul, cadd);
-                cbcr[pl][1] = fmaf(hy ? vo_e : ve_e, cmul, cadd);
-                cbcr[pl][2] = fmaf(hx ? ve_o : ve_e, cmul, cadd);
-                cbcr[pl][3] = fmaf(hx ? (hy ? vo_o : ve_o) : (hy ? vo_e : ve_e), cmul, cadd);
+            } else {
+                // chroma: local staged rows qy..qy+4, columns qx..qx+3
+#pragma unroll
+                for (int pl = 0; pl < 2; pl++) {
+                    float he[5], ho[5];  // horizontal: even luma column (left-sited sample), odd (Catmull-Rom at 0.5)
+#pragma unroll
+                    for (int r = 0; r < 5; r++) {
+                        const float* row = L.ch[pl][qy + r];
+                        he[r] = row[qx + 1];
+                        ho[r] = (9.0f * (row[qx + 1] + row[qx + 2]) - (row[qx] + row[qx + 3])) * (1.0f / 16.0f);
+                    }
+                    // vertical, centre-sited: even luma row t = 0.75 from rows 0..3, odd row t = 0.25 from rows 1..4
+                    const float ve_e = -3.0f * he[0] + 29.0f * he[1] + 111.0f * he[2] - 9.0f * he[3];
+                    const float vo_e = -9.0f * he[1] + 111.0f * he[2] + 29.0f * he[3] - 3.0f * he[4];
+                    const float ve_o = -3.0f * ho[0] + 29.0f * ho[1] + 111.0f * ho[2] - 9.0f * ho[3];
+                    const float vo_o = -9.0f * ho[1] + 111.0f * ho[2] + 29.0f * ho[3] - 3.0f * ho[4];
+                    // (xa,ya): even col, even row; (xa,yb): even col, odd row; (xb,..): odd col.
+                    // A clamped column/row repeats the first one.
+                    cbcr[pl][0] = fmaf(ve_e, cmul, cadd);
+                    cbcr[pl][1] = fmaf(hy ? vo_e : ve_e, cmul, cadd);
+                    cbcr[pl][2] = fmaf(hx ? ve_o : ve_e, cmul, cadd);
+                    cbcr[pl][3] = fmaf(hx ? (hy ? vo_o : ve_o) : (hy ? vo_e : ve_e), cmul, cadd);
+                }
             }
 #pragma unroll
             for (int k = 0; k < 4; k++) {
@@ -267,12 +349,17 @@ __device__ __forceinline__ void prep_quads(PrepLds& L, const PairDev& P, uint8_t
 
 // Each workgroup converts one 64x64 scale-0 tile of one image. Scales 0 and 1
 // come from prep_quads, instantiated per input format; scales 2..5 go through
-// LDS. Every XYB plane is written exactly once.
-template <typename ST>
+// LDS. Every XYB plane is written exactly once. GEN = false takes the RGB and
+// staged 4:2:0 pairs, GEN = true the pairs of the generic chroma path; each
+// skips the other's tiles. They are separate kernels so that the generic
+// path's registers (110 VGPRs) do not set the occupancy of the common formats
+// (91).
+template <typename ST, bool GEN>
 __global__ __launch_bounds__(256) void prep_kernel(const PairDev* __restrict__ pairs,
                                                    const int32_t* __restrict__ prep_start,
                                                    const BatchHdr* __restrict__ hdr, uint8_t* __restrict__ arena) {
     __shared__ PrepLds L;
+    if (GEN && hdr->prep_generic_pairs == 0) return;
     const int npairs = hdr->npairs;
     const int ntiles = hdr->prep_tiles;
     for (int t = blockIdx.x; t < ntiles; t += gridDim.x) {
@@ -289,14 +376,22 @@ __global__ __launch_bounds__(256) void prep_kernel(const PairDev* __restrict__ p
         const int img = local / (ntx * nty);
         local -= img * ntx * nty;
         const int ty = local / ntx, tx = local - ty * ntx;
+        if (prep_generic(P.fmt, P.csw, P.csh, P.cloc) != GEN) continue;  // uniform over the workgroup
 
-        switch (P.fmt) {
-        case IN_RGB_U8: prep_quads<ST, IN_RGB_U8>(L, P, arena, img, tx, ty); break;
-        case IN_RGB_U16: prep_quads<ST, IN_RGB_U16>(L, P, arena, img, tx, ty); break;
-        case IN_RGB_F16: prep_quads<ST, IN_RGB_F16>(L, P, arena, img, tx, ty); break;
-        case IN_RGB_F32: prep_quads<ST, IN_RGB_F32>(L, P, arena, img, tx, ty); break;
-        case IN_YUV420_U8: prep_quads<ST, IN_YUV420_U8>(L, P, arena, img, tx, ty); break;
-        default: prep_quads<ST, IN_YUV420_U16>(L, P, arena, img, tx, ty); break;
+        if constexpr (GEN) {
+            if (fmt_wide(P.fmt)) prep_quads<ST, IN_YUV420_U16, true>(L, P, arena, img, tx, ty);
+            else prep_quads<ST, IN_YUV420_U8, true>(L, P, arena, img, tx, ty);
+        } else {
+            switch (P.fmt) {
+            case IN_RGB_U8: prep_quads<ST, IN_RGB_U8>(L, P, arena, img, tx, ty); break;
+            case IN_RGB_U16: prep_quads<ST, IN_RGB_U16>(L, P, arena, img, tx, ty); break;
+            case IN_RGB_F16: prep_quads<ST, IN_RGB_F16>(L, P, arena, img, tx, ty); break;
+            case IN_RGB_F32: prep_quads<ST, IN_RGB_F32>(L, P, arena, img, tx, ty); break;
+            default:  // YUV 4:2:0 with left-located chroma: the staged path
+                if (fmt_wide(P.fmt)) prep_quads<ST, IN_YUV420_U16>(L, P, arena, img, tx, ty);
+                else prep_quads<ST, IN_YUV420_U8>(L, P, arena, img, tx, ty);
+                break;
+            }
         }
         __syncthreads();
         const int nv1x = min(32, P.sw[1] - tx * 32), nv1y = min(32, P.sh[1] - ty * 32);
@@ -683,12 +778,16 @@ __global__ __launch_bounds__(256) void finalize_kernel(const PairDev* __restrict
 // ---------------------------------------------------------------------------
 // host-side launchers
 // ---------------------------------------------------------------------------
-hipError_t launch_prep(bool fp16, int grid, const PairDev* pairs, const int32_t* prep_start, const BatchHdr* hdr,
-                       void* arena, hipStream_t st) {
-    if (fp16)
-        prep_kernel<__half><<<grid, 256, 0, st>>>(pairs, prep_start, hdr, reinterpret_cast<uint8_t*>(arena));
-    else
-        prep_kernel<float><<<grid, 256, 0, st>>>(pairs, prep_start, hdr, reinterpret_cast<uint8_t*>(arena));
+hipError_t launch_prep(bool fp16, bool generic, int grid, const PairDev* pairs, const int32_t* prep_start,
+                       const BatchHdr* hdr, void* arena, hipStream_t st) {
+    uint8_t* a = reinterpret_cast<uint8_t*>(arena);
+    if (fp16) {
+        if (generic) prep_kernel<__half, true><<<grid, 256, 0, st>>>(pairs, prep_start, hdr, a);
+        else prep_kernel<__half, false><<<grid, 256, 0, st>>>(pairs, prep_start, hdr, a);
+    } else {
+        if (generic) prep_kernel<float, true><<<grid, 256, 0, st>>>(pairs, prep_start, hdr, a);
+        else prep_kernel<float, false><<<grid, 256, 0, st>>>(pairs, prep_start, hdr, a);
+    }
     return hipGetLastError();
 }
 

@@ -6,8 +6,10 @@
 
 namespace vship {
 
-hipError_t launch_prep(bool fp16, int grid, const PairDev* pairs, const int32_t* prep_start, const BatchHdr* hdr,
-                       void* arena, hipStream_t st);
+// generic: the prep kernel of the pairs on the generic chroma path
+// (prep_generic), else the one of every other pair.
+hipError_t launch_prep(bool fp16, bool generic, int grid, const PairDev* pairs, const int32_t* prep_start,
+                       const BatchHdr* hdr, void* arena, hipStream_t st);
 // Map tiles of one group: cls 0 FP32 moments with tile height th (48, 64 or
 // 112), cls 1 FP64 moments with tile height kSmallTH; half: the group's
 // planes are FP16.

@@ -97,6 +97,7 @@ Vship_Exception Vship_SSIMU2BatchTune(const Vship_SSIMU2Pair* pairs, int npairs,
             x.fmt = p.format;
             x.bits = p.bits;
             x.matrix = p.matrix;
+            x.chroma_loc = p.chroma_location;
             mw = std::max(mw, p.width);
             mh = std::max(mh, p.height);
             arena += pair_bytes(p.width, p.height, fp16_planes != 0);

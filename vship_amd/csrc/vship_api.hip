@@ -261,6 +261,7 @@ PairInput to_pair_input(const Vship_SSIMU2Pair& p) {
     in.fmt = p.format;
     in.bits = p.bits;
     in.matrix = p.matrix;
+    in.chroma_loc = p.chroma_location;
     return in;
 }
 

@@ -60,6 +60,19 @@ def build_lib(jobs=8, verbose=False):
     return LIB
 
 
+CLI_SRC = os.path.join(CSRC, "ffvship_cli.cpp")
+CLI_BIN = os.path.join(HERE, "ffvship")
+
+
+def build_cli():
+    """The FFVship-style command line (vship_amd/ffvship) over libvship.so."""
+    tmp = CLI_BIN + ".tmp"
+    _run(["g++", "-O2", "-std=c++17", "-Wall", "-pthread", "-I", os.path.join(ROOT, "include"), "-o", tmp, CLI_SRC,
+          "-L", HERE, "-lvship", "-Wl,-rpath,$ORIGIN"])
+    os.replace(tmp, CLI_BIN)
+    return CLI_BIN
+
+
 def build_synth():
     tmp = SYNTH_LIB + ".tmp"
     _run(["gcc", "-O3", "-mavx2", "-mfma", "-fPIC", "-fopenmp", "-shared", "-o", tmp, SYNTH_SRC, "-lm"])
@@ -69,14 +82,20 @@ def build_synth():
 
 MOCK_SRC = os.path.join(ROOT, "tests", "vs_mock_host.cpp")
 MOCK_LIB = os.path.join(ROOT, "tests", "_build", "libvs_mock_host.so")
+CLI_HOOKS_LIB = os.path.join(ROOT, "tests", "_build", "libffvship_host.so")
 
 
 def build_test_helpers():
-    """The VapourSynth mock host the plugin tests drive (tests/vs_mock_host.cpp)."""
+    """The VapourSynth mock host the plugin tests drive (tests/vs_mock_host.cpp),
+    and the command line's host logic with C entry points for the CPU tests."""
     os.makedirs(os.path.dirname(MOCK_LIB), exist_ok=True)
     tmp = MOCK_LIB + ".tmp"
     _run(["g++", "-O2", "-std=c++17", "-fPIC", "-shared", "-pthread", "-Wall", "-o", tmp, MOCK_SRC, "-ldl"])
     os.replace(tmp, MOCK_LIB)
+    tmp = CLI_HOOKS_LIB + ".tmp"
+    _run(["g++", "-O2", "-std=c++17", "-fPIC", "-shared", "-pthread", "-Wall", "-DFFVSHIP_TEST_HOOKS", "-I",
+          os.path.join(ROOT, "include"), "-o", tmp, CLI_SRC, "-L", HERE, "-lvship", f"-Wl,-rpath,{HERE}"])
+    os.replace(tmp, CLI_HOOKS_LIB)
     return MOCK_LIB
 
 
@@ -94,6 +113,7 @@ def main(argv=None):
     ap.add_argument("-v", action="store_true")
     a = ap.parse_args(argv)
     print(build_lib(a.j, a.v))
+    print(build_cli())
     print(build_synth())
     print(build_test_helpers())
     if a.all:

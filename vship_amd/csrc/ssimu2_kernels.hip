@@ -228,13 +228,18 @@ __device__ __forceinline__ void prep_quads(PrepLds& L, const PairDev& P, uint8_t
 
     if constexpr (yuv && !GEN) {
         // stage the 36x36 chroma neighbourhood (indices clamped to the plane)
+        // thread t < 252 owns column t % 36 and rows t / 36 + 7k of both planes,
+        // so a wave reads whole row runs and the clamped column is computed once
         const int cw = (W + 1) >> 1, chh = (H + 1) >> 1;
         const int cx0 = x0 / 2 - 1, cy0 = y0 / 2 - 2;
-        for (int idx = tid; idx < 2 * 36 * 36; idx += 256) {
-            const int pl = idx / 1296, rem = idx - pl * 1296;
-            const int r = rem / 36, c = rem - r * 36;
-            const int gy = min(max(cy0 + r, 0), chh - 1), gx = min(max(cx0 + c, 0), cw - 1);
-            L.ch[pl][r][c] = sample_u(src[pl + 1], stride[pl + 1], gx, gy, wide, maxcode);
+        if (tid < 252) {
+            const int c = tid % 36;
+            const int gx = min(max(cx0 + c, 0), cw - 1);
+            for (int r = tid / 36; r < 36; r += 7) {
+                const int gy = min(max(cy0 + r, 0), chh - 1);
+                L.ch[0][r][c] = sample_u(src[1], stride[1], gx, gy, wide, maxcode);
+                L.ch[1][r][c] = sample_u(src[2], stride[2], gx, gy, wide, maxcode);
+            }
         }
         __syncthreads();
     }

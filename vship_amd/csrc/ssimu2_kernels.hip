@@ -32,19 +32,21 @@ namespace vship {
 // ---------------------------------------------------------------------------
 
 // sign(v) |v|^2.4 as 2^(2.4 log2|v|) on the hardware exp2/log2 units
-// (src/ssimu2/makeXYB.hpp:52-64).
+// (src/ssimu2/makeXYB.hpp:52-64). |v| is clamped to 1e-30, whose power
+// underflows to exactly 0, so 0 (and NaN) need no branch.
 __device__ __forceinline__ float linearize(float v) {
-    const float a = fabsf(v);
-    const float r = a > 0.f ? __builtin_amdgcn_exp2f(2.4f * __builtin_amdgcn_logf(a)) : 0.f;
-    return copysignf(r, v);
+    const float a = fmaxf(fabsf(v), 1e-30f);
+    return copysignf(__builtin_amdgcn_exp2f(2.4f * __builtin_amdgcn_logf(a)), v);
 }
 
 // cube root of x >= 0 without a division: r = 2^(-log2(x)/3) ~ x^(-1/3) on
 // the hardware units, y = x r^2, then one Newton step on y^3 = x with the
 // derivative's reciprocal 1/(3 y^2) taken as r^2/3 (within 0.75 ulp of the
-// exact cube root where the hardware units are within 2 ulp).
+// exact cube root where the hardware units are within 2 ulp). x <= 0 (and
+// NaN) is clamped to 1e-30, whose cube root 1e-10 is below half an ulp of
+// the opsin bias it is added to: the same XYB as cbrt(0) = 0, with no branch.
 __device__ __forceinline__ float cbrt_pos(float x) {
-    if (!(x > 0.f)) return 0.f;
+    x = fmaxf(x, 1e-30f);
     const float r = __builtin_amdgcn_exp2f(__builtin_amdgcn_logf(x) * (-1.0f / 3.0f));
     const float rr = r * r;
     const float y = x * rr;

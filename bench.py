@@ -142,7 +142,9 @@ def measured_traffic(config, kernel):
 def roofline(st, map_ms, fp16, config):
     achieved = st["map_bytes_fp32"] / (map_ms * 1e-3) / 1e9
     elem = "2 B (scales 0-1) or 4 B (scales 2-5)" if fp16 else "4 B"
-    st_name = "__half" if fp16 else "float"
+    # with FP16 storage the FP32 class is two launches, one per plane group
+    # (FP16 scales 0-1, FP32 scales 2-5); the time and the traffic cover both
+    st_name = "__half+float" if fp16 else "float"
     if st["map_tile_height"] == 1:
         kernel = f"map_mx_kernel<{st_name}, SplitF16x2>"
         label = f"{kernel} (matrix cores, 64x64 tiles)"

@@ -4,12 +4,10 @@
  * Drop-in replacement for the reference's public header src/VshipAPI.h:
  * every type, enum value and function below has the same name, signature and
  * meaning as there (src/VshipAPI.h:20-148), so existing clients recompile or
- * relink unchanged. Only SSIMULACRA2 is implemented on the GPU; the
- * Butteraugli entry points are exported for link compatibility and return
- * Vship_BadErrorType.
+ * relink unchanged. SSIMULACRA2 and Butteraugli run on the GPU.
  *
- * Additive entry points (batched and served SSIMULACRA2) live in
- * VshipSSIMU2Batch.h.
+ * Additive entry points live in VshipSSIMU2Batch.h (batched and served
+ * SSIMULACRA2) and VshipButteraugliBatch.h (batched Butteraugli).
  */
 #ifndef VSHIP_API_HEADER
 #define VSHIP_API_HEADER
@@ -116,8 +114,13 @@ EXPORTPREPROCESS Vship_Exception Vship_ComputeSSIMU2Uint16v2(Vship_SSIMU2Handler
                                                              const uint8_t* srcp1[3], const uint8_t* srcp2[3],
                                                              int64_t stride, int64_t stride2);
 
-/* src/VshipAPI.h:113-148 — Butteraugli is outside this build; the symbols
- * exist for link compatibility and return Vship_BadErrorType. */
+/* src/VshipAPI.h:113-148 (src/VshipLib.cpp:252-365): Butteraugli of one
+ * frame pair of fixed size. The handle owns a GPU engine
+ * (vship_amd/csrc/butter.hip); Compute copies the host planes in, scores
+ * them, and, when dstp is not NULL, writes the distortion map as float rows
+ * of dststride bytes. The score's fields are ints: the norms are stored by
+ * plain conversion, i.e. truncated, as the reference stores its float norms
+ * in them. VshipButteraugliBatch.h returns them in double precision. */
 typedef struct {
     int id;
 } Vship_ButteraugliHandler;

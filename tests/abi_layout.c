@@ -2,6 +2,7 @@
 #include <stddef.h>
 #include <stdio.h>
 #include "VshipAPI.h"
+#include "VshipButteraugliBatch.h"
 #include "VshipSSIMU2Batch.h"
 int main(void) {
     printf("Vship_Version %zu\n", sizeof(Vship_Version));
@@ -14,5 +15,10 @@ int main(void) {
     printf("Vship_SSIMU2BatchStats %zu %zu\n", sizeof(Vship_SSIMU2BatchStats),
            offsetof(Vship_SSIMU2BatchStats, map_bytes));
     printf("Vship_ButteraugliScore %zu\n", sizeof(Vship_ButteraugliScore));
+    printf("Vship_ButteraugliBatchConfig %zu %zu\n", sizeof(Vship_ButteraugliBatchConfig),
+           offsetof(Vship_ButteraugliBatchConfig, intensity_multiplier));
+    printf("Vship_ButteraugliPair %zu %zu\n", sizeof(Vship_ButteraugliPair), offsetof(Vship_ButteraugliPair, distmap));
+    printf("Vship_ButteraugliBatchStats %zu %zu\n", sizeof(Vship_ButteraugliBatchStats),
+           offsetof(Vship_ButteraugliBatchStats, stage_bytes));
     return 0;
 }

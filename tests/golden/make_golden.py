@@ -2,6 +2,7 @@
 
     python tests/golden/make_golden.py oracle [DIR]      # CPU: oracle_vectors.json
     python tests/golden/make_golden.py reference [DIR]   # GPU: reference_hip.json
+    python tests/golden/make_golden.py reference-butter [DIR]   # GPU: reference_butter.json
 
 (DIR defaults to this directory.)
 
@@ -16,6 +17,11 @@ oracle/_ref/libvship_ref.so on a GPU) on the inputs of REF_CASES; the CPU
 test suite then checks the oracle against the reference's output without a
 GPU. Inputs are regenerated from their seeds (synth.config0 and
 case_inputs below are deterministic).
+
+reference_butter.json: Butteraugli norms (Q-norm, 3-norm, inf-norm), the
+sum of the distortion map and 64 map samples of the reference's own HIP
+Butteraugli (oracle/_ref/libvship_ref_butter.so) on BUTTER_CASES, for the
+CPU checks of the Butteraugli oracle.
 """
 import json
 import os
@@ -117,8 +123,46 @@ def make_reference(outdir=HERE):
     print(path)
 
 
+# Butteraugli cases: (name, w, h, format, seed, qnorm); inputs from case_inputs
+BUTTER_CASES = [
+    ("f32_64x48", 64, 48, "f32", 11, 2),
+    ("u16_75x47", 75, 47, "u16", 12, 2),
+    ("f32_160x90_q3", 160, 90, "f32", 13, 3),
+    ("f32_config0", 256, 256, "f32", None, 2),
+]
+
+
+def butter_inputs(w, h, fmt, seed):
+    if seed is None:  # the 256x256 RGB8 pair of config0, as float
+        ref, dist = case_inputs(w, h, "u8", None)
+        return [p.astype(np.float32) / 255.0 for p in ref], [p.astype(np.float32) / 255.0 for p in dist]
+    return case_inputs(w, h, fmt, seed)
+
+
+def make_reference_butter(outdir=HERE):
+    from tests import butter_lib as bl
+
+    if not bl.ref_available():
+        raise SystemExit("oracle/_ref/libvship_ref_butter.so not built (make -C oracle ref)")
+    out = {"source": "reference src/butter HIP code via oracle/ref_butter_shim.hip, FP32, intensity 203",
+           "cases": []}
+    for name, w, h, fmt, seed, q in BUTTER_CASES:
+        ref, dist = butter_inputs(w, h, fmt, seed)
+        print(f"case {name}: {w}x{h} {fmt} qnorm {q}", flush=True)
+        norms, dm = bl.ref_butteraugli(ref, dist, q, 203.0)
+        pts = bl.sample_positions(w, h)
+        out["cases"].append({"name": name, "w": w, "h": h, "format": fmt, "seed": seed, "qnorm": q,
+                             "norms": [float(x) for x in norms], "map_sum": float(dm.sum()),
+                             "map_samples": [[x, y, float(dm[y, x])] for x, y in pts]})
+        print(f"{name}: reference norms {norms}", flush=True)
+    path = os.path.join(outdir, "reference_butter.json")
+    with open(path, "w") as fh:
+        json.dump(out, fh, indent=1)
+    print(path)
+
+
 if __name__ == "__main__":
     what = sys.argv[1] if len(sys.argv) > 1 else "oracle"
     outdir = sys.argv[2] if len(sys.argv) > 2 else HERE
     os.makedirs(outdir, exist_ok=True)
-    {"oracle": make_oracle, "reference": make_reference}[what](outdir)
+    {"oracle": make_oracle, "reference": make_reference, "reference-butter": make_reference_butter}[what](outdir)

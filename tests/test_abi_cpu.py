@@ -12,7 +12,7 @@ import vship_amd
 from vship_amd import api
 
 ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
-HEADERS = [os.path.join(ROOT, "include", h) for h in ("VshipAPI.h", "VshipSSIMU2Batch.h")]
+HEADERS = [os.path.join(ROOT, "include", h) for h in ("VshipAPI.h", "VshipSSIMU2Batch.h", "VshipButteraugliBatch.h")]
 
 
 def declared_symbols():
@@ -85,10 +85,22 @@ def test_compute_on_unknown_handler_is_bad_handler():
     assert score.value == -5.0  # written only on success
 
 
-def test_butteraugli_is_reported_unavailable():
+def test_butteraugli_error_protocol_without_gpu():
+    # src/VshipLib.cpp:252-296: the slot is taken even when init fails, a bad
+    # handler is reported as BadHandler, and nothing is written on failure
     L = api.lib()
     h = api.Handler()
-    assert L.Vship_ButteraugliInit(ctypes.byref(h), 16, 16, ctypes.c_float(80.0)) == 11
+    code = L.Vship_ButteraugliInit(ctypes.byref(h), 16, 16, ctypes.c_float(80.0))
+    assert code != 0
+    assert L.Vship_ButteraugliFree(h) == 0
+    score = api.ButterScore(norm2=-7, norm3=-7, norminf=-7)
+    P3 = ctypes.c_void_p * 3
+    assert L.Vship_ComputeButteraugliFloat(api.Handler(id=999), ctypes.byref(score), None, 0, P3(), P3(), 64) == 9
+    assert score.norm2 == -7
+    assert L.Vship_ButteraugliPairFootprint(0, 5) == 0
+    assert L.Vship_ButteraugliPairFootprint(64, 48) > 0
+    cfg = api.ButterBatchConfig(width=16, height=16, max_pairs=0, qnorm=2, intensity_multiplier=203.0)
+    assert L.Vship_ButteraugliBatchInit(ctypes.byref(h), ctypes.byref(cfg)) == 10
 
 
 def test_struct_layouts_match_c(tmp_path):
@@ -107,6 +119,11 @@ def test_struct_layouts_match_c(tmp_path):
                                               api.BatchConfig.profile.offset]
     assert got["Vship_SSIMU2BatchStats"] == [ctypes.sizeof(api.BatchStats), api.BatchStats.map_bytes.offset]
     assert got["Vship_ButteraugliScore"] == [12]
+    assert got["Vship_ButteraugliBatchConfig"] == [ctypes.sizeof(api.ButterBatchConfig),
+                                                   api.ButterBatchConfig.intensity_multiplier.offset]
+    assert got["Vship_ButteraugliPair"] == [ctypes.sizeof(api.ButterPair), api.ButterPair.distmap.offset]
+    assert got["Vship_ButteraugliBatchStats"] == [ctypes.sizeof(api.ButterBatchStats),
+                                                  api.ButterBatchStats.stage_bytes.offset]
 
 
 def test_device_count_without_gpu_reports_an_error_code():

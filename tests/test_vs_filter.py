@@ -1,4 +1,4 @@
-"""The VapourSynth plugin (vship.SSIMULACRA2, vship.GpuInfo) through a mock
+"""The VapourSynth plugin (vship.SSIMULACRA2, vship.BUTTERAUGLI, vship.GpuInfo) through a mock
 VapourSynth host (tests/vs_mock_host.cpp): registration, argument and error
 handling on the CPU; scores, output frames and reference counting on the GPU.
 The plugin's own declarations of the VapourSynth API (vship_amd/csrc/vs4.hpp)
@@ -28,6 +28,10 @@ def mock():
     L.vsmock_ssimu2.argtypes = [ctypes.c_char_p] + [ctypes.c_int] * 5 + [PP, PP] + [ctypes.c_int] * 7 + [
         ctypes.POINTER(ctypes.c_double), ctypes.POINTER(ctypes.c_int), ctypes.c_char_p, ctypes.c_int]
     L.vsmock_gpuinfo.argtypes = [ctypes.c_char_p, ctypes.c_int, ctypes.c_char_p, ctypes.c_int]
+    L.vsmock_butter.argtypes = [ctypes.c_char_p] + [ctypes.c_int] * 5 + [PP, PP, ctypes.c_int, ctypes.c_int,
+                                                                          ctypes.c_double] + [ctypes.c_int] * 5 + [
+        ctypes.POINTER(ctypes.c_double), ctypes.POINTER(ctypes.c_float), ctypes.POINTER(ctypes.c_int),
+        ctypes.c_char_p, ctypes.c_int]
     return L
 
 
@@ -49,6 +53,23 @@ def run_ssimu2(frames_ref, frames_dist, w, h, stride, w2=None, h2=None, rgbs=(1,
                               scores.ctypes.data_as(ctypes.POINTER(ctypes.c_double)), ctypes.byref(matches), err,
                               len(err))
     return rc, scores, matches.value, err.value.decode()
+
+
+def run_butter(frames_ref, frames_dist, w, h, stride, w2=None, h2=None, qnorm=-1, intensity=-1.0, distmap=-1,
+               num_stream=-1, gpu_id=-1, core_threads=4, nthreads=4):
+    n = len(frames_ref)
+    ptrs_r = (ctypes.c_void_p * (3 * n))(*[p.ctypes.data for f in frames_ref for p in f])
+    ptrs_d = (ctypes.c_void_p * (3 * n))(*[p.ctypes.data for f in frames_dist for p in f])
+    norms = np.full((n, 3), np.nan)
+    maps = np.zeros((n, h, w), np.float32)
+    matches = ctypes.c_int(0)
+    err = ctypes.create_string_buffer(4096)
+    rc = mock().vsmock_butter(libpath(), w, h, w2 or w, h2 or h, n, ptrs_r, ptrs_d, stride, qnorm, intensity,
+                              distmap, num_stream, gpu_id, core_threads, nthreads,
+                              norms.ctypes.data_as(ctypes.POINTER(ctypes.c_double)),
+                              maps.ctypes.data_as(ctypes.POINTER(ctypes.c_float)), ctypes.byref(matches), err,
+                              len(err))
+    return rc, norms, maps, matches.value, err.value.decode()
 
 
 def padded_frames(n, w, h, stride_px, seed):
@@ -93,6 +114,9 @@ def test_plugin_registers_the_reference_interface():
     fns = {l.split("|")[1]: l.split("|")[2:] for l in lines[1:]}
     assert fns["SSIMULACRA2"] == ["reference:vnode;distorted:vnode;numStream:int:opt;gpu_id:int:opt;", "clip:vnode;"]
     assert fns["GpuInfo"] == ["gpu_id:int:opt;", "gpu_human_data:data;"]
+    # src/butter/vapoursynth.hpp argument names, src/VshipLib.cpp:61
+    assert fns["BUTTERAUGLI"] == ["reference:vnode;distorted:vnode;qnorm:int:opt;intensity_multiplier:float:opt;"
+                                  "distmap:int:opt;numStream:int:opt;gpu_id:int:opt;", "clip:vnode;"]
     assert set(fns) == {"SSIMULACRA2", "GpuInfo", "BUTTERAUGLI", "CVVDP"}
 
 
@@ -103,6 +127,14 @@ def test_differing_inputs_are_rejected_before_any_gpu_work():
     assert rc == 1, err
     assert err.startswith("VshipException\nDifferingInputType: Vship received 2 videos with different properties")
     assert "\n - At line " in err and err.endswith("\n")
+
+
+@needs_mock
+def test_butteraugli_differing_inputs_are_rejected():
+    ref, dist = padded_frames(1, 64, 48, 64, 1)
+    rc, _, _, _, err = run_butter(ref, dist, 64, 48, 64 * 4, h2=40)
+    assert rc == 1, err
+    assert err.startswith("VshipException\nDifferingInputType:")
 
 
 @needs_mock
@@ -165,3 +197,33 @@ def test_gpuinfo():
     assert "WarpSize: 64" in text and "PassKernelCheck : 1" in text
     assert mock().vsmock_gpuinfo(libpath(), 99, buf, len(buf)) == 1
     assert buf.value.decode().startswith("VshipException\nBadDeviceArgument:")
+
+
+@pytest.mark.gpu
+@needs_mock
+def test_butteraugli_filter_scores_every_frame():
+    from tests import butter_lib as bl
+    import vship_amd
+
+    vship_amd.gpu_full_check(0)
+    w, h, stride_px, n = 200, 120, 208, 4
+    ref, dist = padded_frames(n, w, h, stride_px, 70)
+    print(f"case: vship.BUTTERAUGLI via the mock host, {n} frames {w}x{h}, qnorm 3, distmap", flush=True)
+    rc, norms, maps, _, err = run_butter(ref, dist, w, h, stride_px * 4, qnorm=3, distmap=1, num_stream=2, gpu_id=0)
+    assert rc == 0, err
+    for i in range(n):
+        r = [np.ascontiguousarray(p[:, :w]) for p in ref[i]]
+        d = [np.ascontiguousarray(p[:, :w]) for p in dist[i]]
+        want, wmap = bl.butteraugli(r, d, 3, 203.0, diffmap=True)
+        assert np.all(np.abs(norms[i] - want) <= 1e-4 * want), (norms[i], want)
+        assert np.all(np.abs(maps[i] - wmap) <= 1e-6 + 1e-4 * np.abs(wmap))
+    print("case: vship.BUTTERAUGLI defaults, output = distorted frames", flush=True)
+    rc, norms2, _, matches, err = run_butter(ref, dist, w, h, stride_px * 4, gpu_id=0)
+    assert rc == 0, err
+    assert matches == n
+    for i in range(n):
+        r = [np.ascontiguousarray(p[:, :w]) for p in ref[i]]
+        d = [np.ascontiguousarray(p[:, :w]) for p in dist[i]]
+        want = bl.butteraugli(r, d, 2, 203.0)
+        assert np.all(np.abs(norms2[i] - want) <= 1e-4 * want)
+

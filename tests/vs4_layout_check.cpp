@@ -26,6 +26,7 @@ int main() {
     SLOT(getFrameFilter); SLOT(requestFrameFilter); SLOT(setFilterError); SLOT(createMap); SLOT(freeMap);
     SLOT(mapSetError); SLOT(mapGetError); SLOT(mapGetInt); SLOT(mapSetInt); SLOT(mapSetFloat); SLOT(mapSetData);
     SLOT(mapGetNode); SLOT(mapConsumeNode); SLOT(getPluginByID); SLOT(invoke); SLOT(getCoreInfo);
+    SLOT(newVideoFrame); SLOT(getWritePtr); SLOT(queryVideoFormat); SLOT(mapGetFloat);
     FIELD(VSPLUGINAPI, vs4::PluginApi, getAPIVersion); FIELD(VSPLUGINAPI, vs4::PluginApi, configPlugin);
     FIELD(VSPLUGINAPI, vs4::PluginApi, registerFunction);
     FIELD(VSVideoFormat, vs4::VideoFormat, colorFamily); FIELD(VSVideoFormat, vs4::VideoFormat, sampleType);
@@ -44,7 +45,7 @@ int main() {
     FIELD(VSFilterDependency, vs4::FilterDependency, requestPattern);
     SIZE(VSFilterDependency, vs4::FilterDependency);
     CONST(VAPOURSYNTH_API_VERSION, vs4::kApiVersion); CONST(VS_MAKE_VERSION(3, 3), vs4::make_version(3, 3));
-    CONST(cfRGB, vs4::cfRGB); CONST(cfYUV, vs4::cfYUV); CONST(stFloat, vs4::stFloat); CONST(stInteger, vs4::stInteger);
+    CONST(cfGray, vs4::cfGray); CONST(cfRGB, vs4::cfRGB); CONST(cfYUV, vs4::cfYUV); CONST(stFloat, vs4::stFloat); CONST(stInteger, vs4::stInteger);
     CONST(pfRGBS, vs4::pfRGBS); CONST(fmParallel, vs4::fmParallel); CONST(peSuccess, vs4::peSuccess);
     CONST(peUnset, vs4::peUnset); CONST(maReplace, vs4::maReplace); CONST(arInitial, vs4::arInitial);
     CONST(arAllFramesReady, vs4::arAllFramesReady); CONST(dtUtf8, vs4::dtUtf8);

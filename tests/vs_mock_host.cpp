@@ -115,6 +115,29 @@ Frame* m_copyFrame(const Frame* f, Core*) {
     g_output_frames++;
     return c;
 }
+Frame* m_newVideoFrame(const VideoFormat* fmt, int w, int h, const Frame* prop_src, Core*) {
+    Frame* c = new Frame;
+    c->w = w;
+    c->h = h;
+    c->stride = (ptrdiff_t)w * fmt->bytesPerSample;
+    c->own.assign((size_t)c->stride * h * fmt->numPlanes, 0);
+    for (int p = 0; p < fmt->numPlanes && p < 3; p++) c->planes[p] = c->own.data() + (size_t)p * c->stride * h;
+    if (prop_src) c->props = prop_src->props;
+    c->output = true;
+    g_output_frames++;
+    return c;
+}
+uint8_t* m_getWritePtr(Frame* f, int p) { return const_cast<uint8_t*>(f->planes[p]); }
+int m_queryVideoFormat(VideoFormat* f, int family, int sample, int bits, int ssw, int ssh, Core*) {
+    f->colorFamily = family;
+    f->sampleType = sample;
+    f->bitsPerSample = bits;
+    f->bytesPerSample = (bits + 7) / 8;
+    f->subSamplingW = ssw;
+    f->subSamplingH = ssh;
+    f->numPlanes = family == cfGray ? 1 : 3;
+    return 1;
+}
 Map* m_getFramePropertiesRW(Frame* f) { return &f->props; }
 ptrdiff_t m_getStride(const Frame* f, int) { return f->stride; }
 const uint8_t* m_getReadPtr(const Frame* f, int p) { return f->planes[p]; }
@@ -144,6 +167,15 @@ int m_mapSetInt(Map* m, const char* key, int64_t i, int) {
     v.i = i;
     m->kv[key] = {v};
     return 0;
+}
+double m_mapGetFloat(const Map* m, const char* key, int index, int* err) {
+    auto it = m->kv.find(key);
+    if (it == m->kv.end() || index >= (int)it->second.size() || it->second[index].kind != 2) {
+        if (err) *err = peUnset;
+        return 0;
+    }
+    if (err) *err = peSuccess;
+    return it->second[index].f;
 }
 int m_mapSetFloat(Map* m, const char* key, double d, int) {
     Value v;
@@ -226,6 +258,10 @@ Api make_api() {
     SET(getPluginByID);
     SET(invoke);
     SET(getCoreInfo);
+    SET(newVideoFrame);
+    SET(getWritePtr);
+    SET(queryVideoFormat);
+    SET(mapGetFloat);
 #undef SET
     return a;
 }
@@ -377,6 +413,105 @@ int vsmock_ssimu2(const char* libpath, int w, int h, int w2, int h2, int nframes
             for (int pl = 0; pl < 3 && ok; pl++)
                 ok = std::memcmp(r->planes[pl], dist[i * 3 + pl], (size_t)stride * h) == 0;
             matches += ok ? 1 : 0;
+            m_freeFrame(r);
+        }
+    };
+    std::vector<std::thread> th;
+    for (int t = 0; t < nthreads; t++) th.emplace_back(work, t);
+    for (auto& x : th) x.join();
+    filt->free(filt->inst, &core, &api);
+    if (output_matches) *output_matches = matches;
+    if (!first_err.empty()) {
+        copy_out(first_err, err, errlen);
+        return 2;
+    }
+    if (n1->freed != n1->gets || n2->freed != n2->gets || g_output_frames != 0) {
+        copy_out("node or frame references not released exactly once", err, errlen);
+        return 4;
+    }
+    return 0;
+}
+
+// Calls BUTTERAUGLI on two RGBS clips of nframes frames and scores every
+// frame from `nthreads` threads. qnorm/numStream/gpu_id < 0 and intensity < 0:
+// argument absent. norms[3 * i ...] receive the three frame properties; with
+// distmap, maps[i * w * h ...] the GrayS output frame, otherwise the output
+// frame must equal the distorted one (counted in *output_matches). Returns as
+// vsmock_ssimu2; 5: the output clip's format is not the expected one.
+int vsmock_butter(const char* libpath, int w, int h, int w2, int h2, int nframes, const uint8_t* const* ref,
+                  const uint8_t* const* dist, int stride, int qnorm, double intensity, int distmap, int numStream,
+                  int gpu_id, int core_threads, int nthreads, double* norms, float* maps, int* output_matches,
+                  char* err, int errlen) {
+    Plugin p;
+    std::string e;
+    if (!load_plugin(libpath, &p, &e)) {
+        copy_out(e, err, errlen);
+        return 1;
+    }
+    PublicFunction create = find_fn(p, "BUTTERAUGLI");
+    if (!create) {
+        copy_out("BUTTERAUGLI not registered", err, errlen);
+        return 1;
+    }
+    const Api api = make_api();
+    Core core;
+    core.threads = core_threads;
+    Node* n1 = source_node(w, h, nframes, ref, stride, rgbs());
+    Node* n2 = source_node(w2, h2, nframes, dist, stride, rgbs());
+    Map in, out;
+    m_mapConsumeNode(&in, "reference", n1, 0);
+    m_mapConsumeNode(&in, "distorted", n2, 0);
+    if (qnorm >= 0) m_mapSetInt(&in, "qnorm", qnorm, 0);
+    if (intensity >= 0) m_mapSetFloat(&in, "intensity_multiplier", intensity, 0);
+    if (distmap >= 0) m_mapSetInt(&in, "distmap", distmap, 0);
+    if (numStream >= 0) m_mapSetInt(&in, "numStream", numStream, 0);
+    if (gpu_id >= 0) m_mapSetInt(&in, "gpu_id", gpu_id, 0);
+    create(&in, &out, nullptr, &core, &api);
+    if (!out.error.empty()) {
+        copy_out(out.error, err, errlen);
+        const bool released = n1->freed == n1->gets && n2->freed == n2->gets;
+        return released ? 1 : 3;
+    }
+    Node* filt = out.kv["clip"][0].node;
+    const bool want_gray = distmap > 0;
+    if ((filt->vi.format.colorFamily == cfGray) != want_gray || filt->vi.format.bitsPerSample != 32) {
+        filt->free(filt->inst, &core, &api);
+        copy_out("unexpected output format", err, errlen);
+        return 5;
+    }
+    std::mutex mu;
+    std::string first_err;
+    int matches = 0;
+    static const char* keys[3] = {"_BUTTERAUGLI_QNorm", "_BUTTERAUGLI_3Norm", "_BUTTERAUGLI_INFNorm"};
+    auto work = [&](int t) {
+        for (int i = t; i < nframes; i += nthreads) {
+            FrameContext ctx;
+            void* fd = nullptr;
+            const Frame* r = filt->get(i, arInitial, filt->inst, &fd, &ctx, &core, &api);
+            bool ok = r == nullptr && ctx.requested.size() == 2;
+            r = filt->get(i, arAllFramesReady, filt->inst, &fd, &ctx, &core, &api);
+            std::lock_guard<std::mutex> g(mu);
+            if (!r) {
+                if (first_err.empty()) first_err = ctx.error.empty() ? "no frame and no error" : ctx.error;
+                continue;
+            }
+            for (int k = 0; k < 3; k++) {
+                auto it = r->props.kv.find(keys[k]);
+                if (it == r->props.kv.end() || it->second[0].kind != 2) {
+                    if (first_err.empty()) first_err = std::string("no property ") + keys[k];
+                } else {
+                    norms[3 * i + k] = it->second[0].f;
+                }
+            }
+            if (want_gray) {
+                for (int y = 0; y < h; y++)
+                    std::memcpy(maps + (size_t)i * w * h + (size_t)y * w, r->planes[0] + (size_t)y * r->stride,
+                                (size_t)w * 4);
+            } else {
+                for (int pl = 0; pl < 3 && ok; pl++)
+                    ok = std::memcmp(r->planes[pl], dist[i * 3 + pl], (size_t)stride * h) == 0;
+                matches += ok ? 1 : 0;
+            }
             m_freeFrame(r);
         }
     };

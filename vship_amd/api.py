@@ -77,6 +77,32 @@ class BatchStats(ctypes.Structure):
                 ("map_bytes_fp32", ctypes.c_uint64), ("prep_bytes", ctypes.c_uint64)]
 
 
+class ButterScore(ctypes.Structure):
+    _fields_ = [("norm2", ctypes.c_int), ("norm3", ctypes.c_int), ("norminf", ctypes.c_int)]
+
+
+class ButterBatchConfig(ctypes.Structure):
+    _fields_ = [("width", ctypes.c_int), ("height", ctypes.c_int), ("max_pairs", ctypes.c_int),
+                ("qnorm", ctypes.c_int), ("intensity_multiplier", ctypes.c_float), ("profile", ctypes.c_int)]
+
+
+class ButterPair(ctypes.Structure):
+    _fields_ = [("ref", ctypes.c_void_p * 3), ("dist", ctypes.c_void_p * 3),
+                ("ref_stride", ctypes.c_int64 * 3), ("dist_stride", ctypes.c_int64 * 3),
+                ("distmap", ctypes.c_void_p), ("distmap_stride", ctypes.c_int64), ("bits", ctypes.c_int),
+                ("matrix", ctypes.c_int), ("chroma_location", ctypes.c_int)]
+
+
+class ButterBatchStats(ctypes.Structure):
+    _fields_ = [("input_ms", ctypes.c_float), ("opsin_ms", ctypes.c_float), ("low_ms", ctypes.c_float),
+                ("mid_ms", ctypes.c_float), ("high_ms", ctypes.c_float), ("diff_ms", ctypes.c_float),
+                ("mask_ms", ctypes.c_float), ("norm_ms", ctypes.c_float), ("npairs", ctypes.c_int),
+                ("timed", ctypes.c_int), ("stage_bytes", ctypes.c_uint64 * 8)]
+
+
+BUTTER_STAGES = ["input", "opsin", "low", "mid", "high", "diff", "mask", "norm"]
+
+
 class TuneCandidate(ctypes.Structure):
     _fields_ = [("map_tile_height", ctypes.c_int), ("persistent_grid", ctypes.c_int), ("use_graph", ctypes.c_int),
                 ("ms", ctypes.c_float)]
@@ -186,6 +212,33 @@ def lib():
     L.Vship_SSIMU2TuneSave.argtypes = [ctypes.c_char_p]
     L.Vship_SSIMU2TuneLoad.restype = E
     L.Vship_SSIMU2TuneLoad.argtypes = [ctypes.c_char_p]
+    L.Vship_ButteraugliInit.restype = E
+    L.Vship_ButteraugliInit.argtypes = [ctypes.POINTER(Handler), ctypes.c_int, ctypes.c_int, ctypes.c_float]
+    L.Vship_ButteraugliInitv2.restype = E
+    L.Vship_ButteraugliInitv2.argtypes = [ctypes.POINTER(Handler), ctypes.c_int, ctypes.c_int, ctypes.c_int,
+                                          ctypes.c_float]
+    L.Vship_ButteraugliFree.restype = E
+    L.Vship_ButteraugliFree.argtypes = [Handler]
+    for name in ("Vship_ComputeButteraugliFloat", "Vship_ComputeButteraugliUint16"):
+        fn = getattr(L, name)
+        fn.restype = E
+        fn.argtypes = [Handler, ctypes.POINTER(ButterScore), ctypes.c_void_p, ctypes.c_int64, P3, P3, ctypes.c_int64]
+    for name in ("Vship_ComputeButteraugliFloatv2", "Vship_ComputeButteraugliUint16v2"):
+        fn = getattr(L, name)
+        fn.restype = E
+        fn.argtypes = [Handler, ctypes.POINTER(ButterScore), ctypes.c_void_p, ctypes.c_int64, P3, P3, ctypes.c_int64,
+                       ctypes.c_int64]
+    L.Vship_ButteraugliPairFootprint.restype = ctypes.c_uint64
+    L.Vship_ButteraugliPairFootprint.argtypes = [ctypes.c_int, ctypes.c_int]
+    L.Vship_ButteraugliBatchInit.restype = E
+    L.Vship_ButteraugliBatchInit.argtypes = [ctypes.POINTER(Handler), ctypes.POINTER(ButterBatchConfig)]
+    L.Vship_ButteraugliBatchFree.restype = E
+    L.Vship_ButteraugliBatchFree.argtypes = [Handler]
+    L.Vship_ButteraugliBatchCompute.restype = E
+    L.Vship_ButteraugliBatchCompute.argtypes = [Handler, ctypes.c_int, ctypes.POINTER(ButterPair), ctypes.c_int,
+                                                ctypes.c_void_p]
+    L.Vship_ButteraugliBatchGetStats.restype = E
+    L.Vship_ButteraugliBatchGetStats.argtypes = [Handler, ctypes.POINTER(ButterBatchStats)]
     L.Vship_SSIMU2ServerClockMs.restype = ctypes.c_double
     L.Vship_SSIMU2ServerClockMs.argtypes = []
     _lib = L
@@ -511,3 +564,150 @@ def tune_save(path):
 
 def tune_load(path):
     _check(lib().Vship_SSIMU2TuneLoad(str(path).encode()), "Vship_SSIMU2TuneLoad")
+
+
+class ButteraugliHandler:
+    """Per-frame Butteraugli handle of fixed size: the reference's
+    Vship_ButteraugliInit[v2] / Vship_ComputeButteraugli{Float,Uint16}[v2] /
+    Vship_ButteraugliFree (src/VshipAPI.h:113-148, src/VshipLib.cpp:252-365).
+    The score's fields are ints, the norms truncated, as there."""
+
+    def __init__(self, width, height, qnorm=2, intensity_multiplier=203.0):
+        self.width, self.height = int(width), int(height)
+        self._h = Handler()
+        code = lib().Vship_ButteraugliInitv2(ctypes.byref(self._h), self.width, self.height, int(qnorm),
+                                             float(intensity_multiplier))
+        if code != 0:
+            lib().Vship_ButteraugliFree(self._h)
+            raise VshipError(code, "Vship_ButteraugliInitv2")
+
+    def _compute(self, fn, ref, dist, distmap):
+        score = ButterScore()
+        dm = np.zeros((self.height, self.width), np.float32) if distmap else None
+        _check(fn(self._h, ctypes.byref(score), dm.ctypes.data if distmap else None,
+                  dm.strides[0] if distmap else 0, _plane_ptrs(ref), _plane_ptrs(dist), ref[0].strides[0],
+                  dist[0].strides[0]), fn.__name__)
+        norms = (score.norm2, score.norm3, score.norminf)
+        return (norms, dm) if distmap else norms
+
+    def compute_float(self, ref, dist, distmap=False):
+        """ref/dist: three 2-D float32 planes (R, G, B), gamma-encoded in [0, 1]."""
+        ref = [np.asarray(p, np.float32) for p in ref]
+        dist = [np.asarray(p, np.float32) for p in dist]
+        return self._compute(lib().Vship_ComputeButteraugliFloatv2, ref, dist, distmap)
+
+    def compute_uint16(self, ref, dist, distmap=False):
+        """ref/dist: three 2-D uint16 planes, full range."""
+        ref = [np.asarray(p, np.uint16) for p in ref]
+        dist = [np.asarray(p, np.uint16) for p in dist]
+        return self._compute(lib().Vship_ComputeButteraugliUint16v2, ref, dist, distmap)
+
+    def close(self):
+        if self._h is not None:
+            _check(lib().Vship_ButteraugliFree(self._h), "Vship_ButteraugliFree")
+            self._h = None
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
+
+def butter_pair_footprint(w, h):
+    return int(lib().Vship_ButteraugliPairFootprint(int(w), int(h)))
+
+
+class ButterDevicePairs:
+    """Host frame pairs (RGB planes, u16 or f32) copied into one device
+    allocation, with Vship_ButteraugliPair descriptors pointing into it and,
+    optionally, a device distortion map per pair."""
+
+    def __init__(self, pairs, distmaps=False, fmt=None, bits=8, matrix=MATRIX_BT709, chroma_location=CHROMA_LEFT):
+        """pairs: list of (ref_planes, dist_planes). fmt: a PF_* code; by
+        default PF_RGB_U16 for uint16 planes, PF_RGB_F32 otherwise."""
+        offs, total = [], 0
+        h, w = np.asarray(pairs[0][0][0]).shape
+        for ref, dist in pairs:
+            o = []
+            for p in list(ref) + list(dist):
+                o.append(total)
+                total += (np.ascontiguousarray(p).nbytes + 255) // 256 * 256
+            offs.append(o)
+        map_bytes = (w * h * 4 + 255) // 256 * 256
+        self.map_offset = total
+        if distmaps:
+            total += map_bytes * len(pairs)
+        self.buffer = DeviceBuffer(total)
+        self.descs = (ButterPair * len(pairs))()
+        self.width, self.height, self.map_bytes = w, h, map_bytes
+        for i, (ref, dist) in enumerate(pairs):
+            planes = [np.ascontiguousarray(p) for p in list(ref) + list(dist)]
+            for j, p in enumerate(planes):
+                self.buffer.upload(p, offs[i][j])
+            d = self.descs[i]
+            for j in range(3):
+                d.ref[j] = self.buffer.ptr + offs[i][j]
+                d.dist[j] = self.buffer.ptr + offs[i][3 + j]
+                d.ref_stride[j] = planes[j].strides[0]
+                d.dist_stride[j] = planes[3 + j].strides[0]
+            if distmaps:
+                d.distmap = self.buffer.ptr + self.map_offset + i * map_bytes
+                d.distmap_stride = w * 4
+            d.bits = bits
+            d.matrix = matrix
+            d.chroma_location = chroma_location
+        if fmt is None:
+            fmt = PF_RGB_U16 if np.asarray(pairs[0][0][0]).dtype == np.uint16 else PF_RGB_F32
+        self.format = fmt
+        self.n = len(pairs)
+        self.distmaps = distmaps
+
+    def distmap(self, i):
+        if not self.distmaps:
+            raise ValueError("no distortion maps requested")
+        raw = self.buffer.download(self.width * self.height * 4, self.map_offset + i * self.map_bytes)
+        return raw.view(np.float32).reshape(self.height, self.width).copy()
+
+    def free(self):
+        self.buffer.free()
+
+
+class ButteraugliBatch:
+    """Batched Butteraugli over device-resident pairs (Vship_ButteraugliBatch*)."""
+
+    def __init__(self, width, height, max_pairs=1, qnorm=2, intensity_multiplier=203.0, profile=False):
+        cfg = ButterBatchConfig(width=int(width), height=int(height), max_pairs=int(max_pairs), qnorm=int(qnorm),
+                                intensity_multiplier=float(intensity_multiplier), profile=int(bool(profile)))
+        self._h = Handler()
+        code = lib().Vship_ButteraugliBatchInit(ctypes.byref(self._h), ctypes.byref(cfg))
+        if code != 0:
+            self._h = None
+            raise VshipError(code, "Vship_ButteraugliBatchInit")
+
+    def compute(self, dp, n=None):
+        """dp: ButterDevicePairs. Returns an (n, 3) array of Q-norm, 3-norm, inf-norm."""
+        n = dp.n if n is None else n
+        norms = np.zeros((n, 3))
+        _check(lib().Vship_ButteraugliBatchCompute(self._h, dp.format, dp.descs, n, norms.ctypes.data),
+               "Vship_ButteraugliBatchCompute")
+        return norms
+
+    def stats(self):
+        s = ButterBatchStats()
+        _check(lib().Vship_ButteraugliBatchGetStats(self._h, ctypes.byref(s)), "Vship_ButteraugliBatchGetStats")
+        out = {k: getattr(s, k) for k, _ in ButterBatchStats._fields_ if k != "stage_bytes"}
+        out["stage_bytes"] = {BUTTER_STAGES[i]: int(s.stage_bytes[i]) for i in range(8)}
+        return out
+
+    def close(self):
+        if self._h is not None:
+            _check(lib().Vship_ButteraugliBatchFree(self._h), "Vship_ButteraugliBatchFree")
+            self._h = None
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+

@@ -4,6 +4,7 @@
 #include <cstdint>
 #include <memory>
 
+#include "butter.hpp"
 #include "engine.hpp"
 
 namespace vship {
@@ -41,6 +42,26 @@ struct FrameHandle {
     ~FrameHandle();
     // fmt: IN_RGB_U8 / IN_RGB_U16 / IN_RGB_F16 / IN_RGB_F32; host planes
     double compute(int fmt, const uint8_t* const p1[3], const uint8_t* const p2[3], int64_t s1, int64_t s2);
+};
+
+// One Butteraugli engine of fixed size, as a Vship_ButteraugliHandler refers
+// to (ButterComputingImplementation, src/butter/main.hpp:197-240 of the
+// reference): host RGB planes are copied into device staging and scored as a
+// batch of one.
+struct ButterHandle {
+    int width, height, device;
+    std::unique_ptr<bt::Engine> engine;
+    uint8_t* staging = nullptr;
+    size_t staging_bytes = 0;
+    float* map = nullptr;  // device map, rows of width floats
+    std::atomic<bool> busy{false};
+
+    ButterHandle(int w, int h, int qnorm, float intensity, int device);
+    ~ButterHandle();
+    // fmt: IN_RGB_F32 / IN_RGB_U16; host planes. dst (host, may be null)
+    // receives the map in rows of dst_stride bytes. norms: Q, 3, inf.
+    void compute(int fmt, const uint8_t* const p1[3], const uint8_t* const p2[3], int64_t s1, int64_t s2, float* dst,
+                 int64_t dst_stride, double norms[3]);
 };
 
 }  // namespace vship

@@ -1,0 +1,1122 @@
+// Butteraugli on MI355X (gfx950).
+//
+// The metric is the reference's src/butter/ (main.hpp:22-195): linear RGB at
+// full and half scale, opsin dynamics, four frequency bands, Malta line
+// differences, L2 terms, a psycho-visual mask, a blend of the two scales and
+// three norms. The arithmetic is FP32, as there.
+//
+// Layout and kernels. Every intermediate is an FP32 plane in one arena, per
+// pair and scale: 47 planes of w x h (rows packed, planes 64-float aligned).
+// The reference runs about 60 launches per scale, each a full pass over one
+// or two planes; here one scale is six launches, each a 2-D tile kernel that
+// stages its input with halo in LDS and fuses everything pointwise around
+// its stencil:
+//
+//   opsin   sigma 1.2 blur of linear RGB + opsin dynamics       -> XYB
+//   low     sigma 7.156 blur of XYB                             -> low band
+//                                                                  (as values), XYB - low
+//   mid     sigma 3.225 blur of XYB - low, range operations,
+//           X high band suppressed by Y                          -> middle band, high residual
+//   high    sigma 1.564 blur of the residual, range operations   -> high, ultra-high bands
+//   diff    the six Malta line terms from LDS tiles with a 4-pixel
+//           halo, L2 and asymmetric L2 terms, mask inputs        -> AC sum, DC sum, 2 masks
+//   mask    sigma 2.7 blur of both masks, fuzzy erosion, final
+//           map; at full scale also the blend with the half-scale
+//           map and the norm partial sums                        -> map, partials
+//
+// Both images of a pair run in one launch (grid z = pair x image), and a
+// batch of pairs in the same launches. The half scale runs first, so that
+// the full scale's mask kernel blends and reduces in its own pass.
+// Separable blurs divide each pass by the sum of the taps that fall inside
+// the image, as the reference does (gaussianblur.hpp:55-85): the per-column
+// and per-row factors come from the running tap sums.
+#include "butter.hpp"
+
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+
+#include "engine.hpp"  // VshipErr, HIP_OR_THROW
+#include "yuv_dev.hpp"
+
+namespace vship {
+namespace bt {
+
+namespace {
+
+constexpr int kThreads = 256;
+constexpr int kSigmaCount = 5;
+// src/butter/gaussianblur.hpp:27-42: window max(floor(3 sigma), 8)
+constexpr double kSigma[kSigmaCount] = {1.2, 1.56416327805, 2.7, 3.22489901262, 7.15593339443};
+constexpr int kRad[kSigmaCount] = {8, 8, 8, 9, 21};
+enum : int { SG_OPSIN = 0, SG_HIGH = 1, SG_MASK = 2, SG_MID = 3, SG_LOW = 4 };
+
+// Work planes of one pair at one scale. Groups of three are [image][X, Y, B],
+// groups of two [image][X, Y].
+enum : int {
+    P_LIN = 0,   // linear RGB
+    P_XYB = 6,   // opsin XYB
+    P_LFV = 12,  // low band, as values
+    P_T = 18,    // XYB - low band
+    P_MF = 24,   // middle band
+    P_HFR = 30,  // high residual (before its blur)
+    P_HF = 34,   // high band
+    P_UHF = 38,  // ultra-high band
+    P_ACS = 42,  // AC sum (all channels) without the mask term
+    P_DCS = 43,  // DC sum
+    P_M0 = 44,   // mask input of the reference image
+    P_M1 = 45,   // mask input of the distorted image
+    P_MAP = 46,  // distortion map (half scale)
+    P_COUNT = 47
+};
+
+struct Scale {
+    float* base;
+    int w, h;
+    int64_t plane;  // floats between planes
+    int64_t pair;   // floats between pairs
+};
+
+__device__ __forceinline__ float* plane_of(const Scale& S, int pair, int p) {
+    return S.base + (int64_t)pair * S.pair + (int64_t)p * S.plane;
+}
+
+// 1 / (sum of the taps inside [0, n)) at position p in [0, n) for a window of
+// radius R (gaussianblur.hpp:63-65, 83-84).
+template <int R>
+__device__ __forceinline__ float inv_total(const Taps* __restrict__ T, int p, int n) {
+    const int lo = max(0, R - p), hi = min(2 * R, n - 1 - p + R);
+    return 1.0f / (T->cum[hi + 1] - T->cum[lo]);
+}
+
+// in[r][c] = src(gx0 + c, gy0 + r), 0 outside the image
+template <int SW, int SH>
+__device__ __forceinline__ void stage_plane(float* __restrict__ in, const float* __restrict__ src, int w, int h,
+                                            int gx0, int gy0) {
+    for (int i = threadIdx.x; i < SW * SH; i += kThreads) {
+        const int r = i / SW, c = i - r * SW;
+        const int gx = gx0 + c, gy = gy0 + r;
+        float v = 0.0f;
+        if (gx >= 0 && gx < w && gy >= 0 && gy < h) v = src[(int64_t)gy * w + gx];
+        in[i] = v;
+    }
+}
+
+// mid[r][c] = colf[c] * sum_k g[k] in[r][c + k], for SH rows and OW columns
+template <int R, int SW, int SH, int OW>
+__device__ __forceinline__ void hpass(const float* __restrict__ in, float* __restrict__ mid,
+                                      const float* __restrict__ colf, const Taps* __restrict__ T) {
+    static_assert(SW == OW + 2 * R, "staged width");
+    for (int i = threadIdx.x; i < SH * OW; i += kThreads) {
+        const int r = i / OW, c = i - r * OW;
+        const float* p = in + r * SW + c;
+        float s = 0.0f;
+#pragma unroll
+        for (int k = 0; k <= 2 * R; k++) s = fmaf(T->g[k], p[k], s);
+        mid[i] = s * colf[c];
+    }
+}
+
+// sum_k g[k] mid[r + k][c]
+template <int R, int OW>
+__device__ __forceinline__ float vsum(const float* __restrict__ mid, const Taps* __restrict__ T, int r, int c) {
+    const float* p = mid + r * OW + c;
+    float s = 0.0f;
+#pragma unroll
+    for (int k = 0; k <= 2 * R; k++) s = fmaf(T->g[k], p[k * OW], s);
+    return s;
+}
+
+// Column and row factors of an output region [x0, x0 + OW) x [y0, y0 + OH):
+// 0 outside the image.
+template <int R, int OW, int OH>
+__device__ __forceinline__ void edge_factors(float* colf, float* rowf, const Taps* __restrict__ T, int w, int h,
+                                             int x0, int y0) {
+    static_assert(OW <= 128 && OH <= 128, "factor threads");
+    const int t = threadIdx.x;
+    if (t < OW) {
+        const int gx = x0 + t;
+        colf[t] = (gx >= 0 && gx < w) ? inv_total<R>(T, gx, w) : 0.0f;
+    } else if (t >= 128 && t < 128 + OH) {
+        const int gy = y0 + t - 128;
+        rowf[t - 128] = (gy >= 0 && gy < h) ? inv_total<R>(T, gy, h) : 0.0f;
+    }
+}
+
+// ---------------------------------------------------------------------------
+// point operations (src/butter/colors.hpp, separatefrequencies.hpp:3-50,
+// maskPsycho.hpp, combineMasks.hpp)
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ float linearize(float a) { return a < 0.0f ? -powf(-a, 2.4f) : powf(a, 2.4f); }
+
+__device__ __forceinline__ float opsin_gamma(float v) {
+    return fmaf(19.245013259874995f, logf(v + 9.9710635769299145f), -23.16046239805755f);
+}
+
+__device__ __forceinline__ void opsin_absorbance(float a0, float a1, float a2, float& o0, float& o1, float& o2) {
+    o0 = fmaf(0.29956550340058319f, a0, fmaf(0.63373087833825936f, a1, fmaf(0.077705617820981968f, a2, 1.7557483643287353f)));
+    o1 = fmaf(0.22158691104574774f, a0, fmaf(0.69391388044116142f, a1, fmaf(0.0987313588422f, a2, 1.7557483643287353f)));
+    o2 = fmaf(0.02f, a0, fmaf(0.02f, a1, fmaf(0.20480129041026129f, a2, 12.226454707163354f)));
+}
+
+__device__ __forceinline__ float remove_range(float x, float w) { return x > w ? x - w : (x < -w ? x + w : 0.0f); }
+__device__ __forceinline__ float amplify_range(float x, float w) { return x > w ? x + w : (x < -w ? x - w : 2.0f * x); }
+__device__ __forceinline__ float maximum_clamp(float v, float maxval) {
+    const float mul = 0.688059627878f;
+    if (v >= maxval) return (v - maxval) * mul + maxval;
+    if (v < -maxval) return (v + maxval) * mul - maxval;
+    return v;
+}
+
+// mask input: amplitude of the high bands, compressed (maskinit + diffPrecompute)
+__device__ __forceinline__ float mask_input(float hfx, float uhfx, float hfy, float uhfy) {
+    const float xd = (uhfx + hfx) * 2.5f;
+    const float yd = uhfy * 0.4f + hfy * 0.4f;
+    const float m = sqrtf(xd * xd + yd * yd);
+    const float mul = 6.19424080439f, bias = mul * 12.61050594197f;
+    return sqrtf(mul * fabsf(m) + bias) - sqrtf(bias);
+}
+
+constexpr double kMaskNorm = 1.0 / (double)(0.79079917404f * 17.83f);
+
+__device__ __forceinline__ float mask_y(float d) {
+    const float c = 2.5485944793f / ((0.451936922203f * d) + 0.829591754942f);
+    const float r = (float)((1.0 + (double)c) * kMaskNorm);
+    return r * r;
+}
+__device__ __forceinline__ float mask_dc_y(float d) {
+    const float c = 0.505054525019f / ((3.87449418804f * d) + 0.20025578522f);
+    const float r = (float)((1.0 + (double)c) * kMaskNorm);
+    return r * r;
+}
+
+__device__ __forceinline__ void store_min3(float v, float& m0, float& m1, float& m2) {
+    if (v < m2) {
+        if (v < m0) {
+            m2 = m1;
+            m1 = m0;
+            m0 = v;
+        } else if (v < m1) {
+            m2 = m1;
+            m1 = v;
+        } else {
+            m2 = v;
+        }
+    }
+}
+
+// ---------------------------------------------------------------------------
+// input: sample, linearise, 2x2 mean (Planed.hpp, colors.hpp:7-21,
+// downupsample.hpp:3-31). One thread per half-scale pixel writes the up to
+// four full-scale pixels it covers. RGB samples are read as the reference's
+// convertPointer does (src/util/float3operations.hpp:197-218). YUV is
+// converted to R'G'B' first, with the front end of the SSIMULACRA2 prep kernel:
+// limited range, the pair's matrix, Catmull-Rom chroma at the pair's chroma
+// location.
+// ---------------------------------------------------------------------------
+template <int FMT>
+__device__ __forceinline__ float rgb_in(const uint8_t* __restrict__ base, int64_t stride, int x, int y) {
+    const uint8_t* row = base + (int64_t)y * stride;
+    if constexpr (FMT == IN_RGB_U8) return (float)as_global(row)[x] / 255.0f;
+    else if constexpr (FMT == IN_RGB_U16) return (float)as_global(reinterpret_cast<const uint16_t*>(row))[x] / 65535.0f;
+    else if constexpr (FMT == IN_RGB_F16)
+        return __half2float(__ushort_as_half(as_global(reinterpret_cast<const unsigned short*>(row))[x]));
+    else return as_global(reinterpret_cast<const float*>(row))[x];
+}
+
+// R'G'B' of luma pixel (x, y) of a YUV image (container WIDE), as prep_quads'
+// generic chroma path computes it
+template <bool WIDE>
+__device__ __forceinline__ void yuv_rgb(const uint8_t* const* src, const int64_t* stride, int w, int h, int csw,
+                                        int csh, const PairDesc& P, int x, int y, float rgb[3]) {
+    const int maxcode = WIDE ? (1 << P.bits) - 1 : 255;
+    const float sc = (float)(1 << (P.bits - 8));
+    const float ymul = 1.0f / (219.0f * sc), yadd = -16.0f / 219.0f;
+    const float cmul1 = 1.0f / (224.0f * sc), cadd = -128.0f / 224.0f;
+    const float kr = P.kr, kb = P.kb, ikg = 1.0f / (1.0f - P.kr - P.kb);
+    const bool hc = cloc_hcentred(P.cloc), vc = cloc_vcentred(P.cloc);
+    const int cw = (w + (1 << csw) - 1) >> csw, chh = (h + (1 << csh) - 1) >> csh;
+    const float yn = fmaf(sample_u(src[0], stride[0], x, y, WIDE, maxcode), ymul, yadd);
+    const float cb = fmaf(chroma_at<WIDE>(src[1], stride[1], cw, chh, x, y, csw, csh, hc, vc, maxcode), cmul1, cadd);
+    const float cr = fmaf(chroma_at<WIDE>(src[2], stride[2], cw, chh, x, y, csw, csh, hc, vc, maxcode), cmul1, cadd);
+    const float R = fmaf(2.0f * (1.0f - kr), cr, yn);
+    const float B = fmaf(2.0f * (1.0f - kb), cb, yn);
+    rgb[0] = R;
+    rgb[1] = (yn - kr * R - kb * B) * ikg;
+    rgb[2] = B;
+}
+
+template <int FMT>
+__global__ __launch_bounds__(kThreads) void butter_input_kernel(const PairDesc* __restrict__ pairs, Scale S0,
+                                                                Scale S1) {
+    const int pair = blockIdx.z;
+    const int hx = blockIdx.x * 16 + (threadIdx.x & 15), hy = blockIdx.y * 16 + (threadIdx.x >> 4);
+    if (hx >= S1.w || hy >= S1.h) return;
+    const int w = S0.w, h = S0.h;
+    const bool xo = 2 * hx + 1 < w, yo = 2 * hy + 1 < h;
+    const int x0 = 2 * hx, y0 = 2 * hy, x1 = xo ? x0 + 1 : x0, y1 = yo ? y0 + 1 : y0;
+    const PairDesc& P = pairs[pair];
+    for (int im = 0; im < 2; im++) {
+        float v[4][3];  // (x0,y0), (x1,y0), (x0,y1), (x1,y1)
+        if constexpr (!fmt_yuv(FMT)) {
+#pragma unroll
+            for (int c = 0; c < 3; c++) {
+                const uint8_t* base = P.src[im][c];
+                const int64_t st = P.stride[im][c];
+                v[0][c] = rgb_in<FMT>(base, st, x0, y0);
+                v[1][c] = rgb_in<FMT>(base, st, x1, y0);
+                v[2][c] = rgb_in<FMT>(base, st, x0, y1);
+                v[3][c] = rgb_in<FMT>(base, st, x1, y1);
+            }
+        } else {
+            constexpr bool wide = fmt_wide(FMT);
+            constexpr int csw = fmt_csw(FMT), csh = fmt_csh(FMT);
+            yuv_rgb<wide>(P.src[im], P.stride[im], w, h, csw, csh, P, x0, y0, v[0]);
+            yuv_rgb<wide>(P.src[im], P.stride[im], w, h, csw, csh, P, x1, y0, v[1]);
+            yuv_rgb<wide>(P.src[im], P.stride[im], w, h, csw, csh, P, x0, y1, v[2]);
+            yuv_rgb<wide>(P.src[im], P.stride[im], w, h, csw, csh, P, x1, y1, v[3]);
+        }
+#pragma unroll
+        for (int c = 0; c < 3; c++) {
+            const float a = linearize(v[0][c]), b = linearize(v[1][c]);
+            const float cc = linearize(v[2][c]), d = linearize(v[3][c]);
+            float* L = plane_of(S0, pair, P_LIN + im * 3 + c);
+            L[(int64_t)y0 * w + x0] = a;
+            if (xo) L[(int64_t)y0 * w + x1] = b;
+            if (yo) {
+                L[(int64_t)y1 * w + x0] = cc;
+                if (xo) L[(int64_t)y1 * w + x1] = d;
+            }
+            float m = 0.0f;
+            m += a;
+            m += cc;
+            m += b;
+            m += d;
+            plane_of(S1, pair, P_LIN + im * 3 + c)[(int64_t)hy * S1.w + hx] = m * 0.25f;
+        }
+    }
+}
+
+template <int FMT>
+void launch_input(dim3 g, hipStream_t st, const PairDesc* pairs, const Scale& S0, const Scale& S1) {
+    butter_input_kernel<FMT><<<g, dim3(kThreads), 0, st>>>(pairs, S0, S1);
+}
+
+template <int... F>
+void launch_input_fmt(int fmt, dim3 g, hipStream_t st, const PairDesc* pairs, const Scale& S0, const Scale& S1,
+                      std::integer_sequence<int, F...>) {
+    ((fmt == F ? launch_input<F>(g, st, pairs, S0, S1) : void()), ...);
+}
+
+// ---------------------------------------------------------------------------
+// Blur tile kernels: 64 x 32 outputs, 256 threads; thread (tx, ty) owns
+// column tx and rows ty + 4 j, j < 8.
+// ---------------------------------------------------------------------------
+constexpr int kTW = 64, kTH = 32, kRows = kTH / (kThreads / kTW);
+
+template <int R>
+struct BlurLds {
+    static constexpr int SW = kTW + 2 * R, SH = kTH + 2 * R;
+    float in[SH * SW];
+    float mid[SH * kTW];
+    float colf[kTW];
+    float rowf[kTH];
+};
+
+// Blur plane src of the tile into out[j] (rows ty + 4 j); ctr[j] receives the
+// unblurred value at the same pixel. Ends with the LDS free for reuse only
+// after the caller's next __syncthreads.
+template <int R>
+__device__ __forceinline__ void blur_tile(BlurLds<R>& L, const float* __restrict__ src, const Taps* __restrict__ T,
+                                          int w, int h, int ox0, int oy0, float (&out)[kRows], float (&ctr)[kRows]) {
+    constexpr int SW = BlurLds<R>::SW, SH = BlurLds<R>::SH;
+    const int tx = threadIdx.x & (kTW - 1), ty = threadIdx.x / kTW;
+    __syncthreads();
+    stage_plane<SW, SH>(L.in, src, w, h, ox0 - R, oy0 - R);
+    __syncthreads();
+    hpass<R, SW, SH, kTW>(L.in, L.mid, L.colf, T);
+#pragma unroll
+    for (int j = 0; j < kRows; j++) ctr[j] = L.in[(ty + 4 * j + R) * SW + tx + R];
+    __syncthreads();
+#pragma unroll
+    for (int j = 0; j < kRows; j++) {
+        const int r = ty + 4 * j;
+        out[j] = vsum<R, kTW>(L.mid, T, r, tx) * L.rowf[r];
+    }
+}
+
+// sigma 1.2 blur of linear RGB + opsin dynamics (colors.hpp:50-91) -> XYB
+__global__ __launch_bounds__(kThreads) void butter_opsin_kernel(Scale S, const Taps* __restrict__ taps,
+                                                                float intensity) {
+    constexpr int R = kRad[SG_OPSIN];
+    __shared__ BlurLds<R> L;
+    const Taps* T = taps + SG_OPSIN;
+    const int pair = blockIdx.z >> 1, im = blockIdx.z & 1;
+    const int ox0 = blockIdx.x * kTW, oy0 = blockIdx.y * kTH;
+    edge_factors<R, kTW, kTH>(L.colf, L.rowf, T, S.w, S.h, ox0, oy0);
+    float bl[3][kRows], ctr[3][kRows];
+#pragma unroll
+    for (int c = 0; c < 3; c++)
+        blur_tile<R>(L, plane_of(S, pair, P_LIN + im * 3 + c), T, S.w, S.h, ox0, oy0, bl[c], ctr[c]);
+    const int tx = threadIdx.x & (kTW - 1), ty = threadIdx.x / kTW;
+    const int gx = ox0 + tx;
+    if (gx >= S.w) return;
+#pragma unroll
+    for (int j = 0; j < kRows; j++) {
+        const int gy = oy0 + ty + 4 * j;
+        if (gy >= S.h) continue;
+        float b0, b1, b2;
+        opsin_absorbance(bl[0][j] * intensity, bl[1][j] * intensity, bl[2][j] * intensity, b0, b1, b2);
+        b0 = fmaxf(fmaxf(b0, 1.7557483643287353f), 1e-4f);
+        b1 = fmaxf(fmaxf(b1, 1.7557483643287353f), 1e-4f);
+        b2 = fmaxf(fmaxf(b2, 12.226454707163354f), 1e-4f);
+        const float s0 = fmaxf(opsin_gamma(b0) / b0, 1e-4f);
+        const float s1 = fmaxf(opsin_gamma(b1) / b1, 1e-4f);
+        const float s2 = fmaxf(opsin_gamma(b2) / b2, 1e-4f);
+        float o0, o1, o2;
+        opsin_absorbance(ctr[0][j] * intensity, ctr[1][j] * intensity, ctr[2][j] * intensity, o0, o1, o2);
+        o0 = fmaxf(o0 * s0, 1.7557483643287353f);
+        o1 = fmaxf(o1 * s1, 1.7557483643287353f);
+        o2 = fmaxf(o2 * s2, 12.226454707163354f);
+        const int64_t g = (int64_t)gy * S.w + gx;
+        plane_of(S, pair, P_XYB + im * 3 + 0)[g] = o0 - o1;
+        plane_of(S, pair, P_XYB + im * 3 + 1)[g] = o0 + o1;
+        plane_of(S, pair, P_XYB + im * 3 + 2)[g] = o2;
+    }
+}
+
+// sigma 7.156 blur: low band (stored as values, XybLowFreqToVals) and the
+// residual XYB - low (separatefrequencies.hpp:142-170)
+__global__ __launch_bounds__(kThreads) void butter_low_kernel(Scale S, const Taps* __restrict__ taps) {
+    constexpr int R = kRad[SG_LOW];
+    __shared__ BlurLds<R> L;
+    const Taps* T = taps + SG_LOW;
+    const int pair = blockIdx.z >> 1, im = blockIdx.z & 1;
+    const int ox0 = blockIdx.x * kTW, oy0 = blockIdx.y * kTH;
+    const int tx = threadIdx.x & (kTW - 1), ty = threadIdx.x / kTW;
+    const int gx = ox0 + tx;
+    edge_factors<R, kTW, kTH>(L.colf, L.rowf, T, S.w, S.h, ox0, oy0);
+    float lf[3][kRows];
+#pragma unroll
+    for (int c = 0; c < 3; c++) {
+        float ctr[kRows];
+        blur_tile<R>(L, plane_of(S, pair, P_XYB + im * 3 + c), T, S.w, S.h, ox0, oy0, lf[c], ctr);
+        float* t = plane_of(S, pair, P_T + im * 3 + c);
+#pragma unroll
+        for (int j = 0; j < kRows; j++) {
+            const int gy = oy0 + ty + 4 * j;
+            if (gx < S.w && gy < S.h) t[(int64_t)gy * S.w + gx] = ctr[j] - lf[c][j];
+        }
+    }
+    if (gx >= S.w) return;
+#pragma unroll
+    for (int j = 0; j < kRows; j++) {
+        const int gy = oy0 + ty + 4 * j;
+        if (gy >= S.h) continue;
+        const int64_t g = (int64_t)gy * S.w + gx;
+        float b = lf[2][j] + -0.362267051518f * lf[1][j];
+        b = b * 49.87984651440f;
+        plane_of(S, pair, P_LFV + im * 3 + 0)[g] = lf[0][j] * 33.832837186260f;
+        plane_of(S, pair, P_LFV + im * 3 + 1)[g] = lf[1][j] * 14.458268100570f;
+        plane_of(S, pair, P_LFV + im * 3 + 2)[g] = b;
+    }
+}
+
+// sigma 3.225 blur of the residual: middle band after its range operation,
+// high residual = residual - middle band, X's residual suppressed by Y's
+// (separatefrequencies.hpp:160-189)
+__global__ __launch_bounds__(kThreads) void butter_mid_kernel(Scale S, const Taps* __restrict__ taps) {
+    constexpr int R = kRad[SG_MID];
+    __shared__ BlurLds<R> L;
+    const Taps* T = taps + SG_MID;
+    const int pair = blockIdx.z >> 1, im = blockIdx.z & 1;
+    const int ox0 = blockIdx.x * kTW, oy0 = blockIdx.y * kTH;
+    const int tx = threadIdx.x & (kTW - 1), ty = threadIdx.x / kTW;
+    const int gx = ox0 + tx;
+    edge_factors<R, kTW, kTH>(L.colf, L.rowf, T, S.w, S.h, ox0, oy0);
+    float hfr[2][kRows];
+    constexpr int kOrder[3] = {1, 0, 2};
+#pragma unroll
+    for (int o = 0; o < 3; o++) {
+        const int c = kOrder[o];
+        float mf[kRows], ctr[kRows];
+        blur_tile<R>(L, plane_of(S, pair, P_T + im * 3 + c), T, S.w, S.h, ox0, oy0, mf, ctr);
+        float* dst = plane_of(S, pair, P_MF + im * 3 + c);
+#pragma unroll
+        for (int j = 0; j < kRows; j++) {
+            float m = mf[j];
+            if (c < 2) {
+                hfr[c][j] = ctr[j] - m;
+                m = c == 0 ? remove_range(m, 0.29f) : amplify_range(m, 0.1f);
+            }
+            const int gy = oy0 + ty + 4 * j;
+            if (gx < S.w && gy < S.h) dst[(int64_t)gy * S.w + gx] = m;
+        }
+    }
+    if (gx >= S.w) return;
+#pragma unroll
+    for (int j = 0; j < kRows; j++) {
+        const int gy = oy0 + ty + 4 * j;
+        if (gy >= S.h) continue;
+        const float s = 0.653020556257f, yw = 46.0f, yv = hfr[1][j];
+        const float x = hfr[0][j] * (s + (yw * (1.0f - s)) / (yw + yv * yv));
+        const int64_t g = (int64_t)gy * S.w + gx;
+        plane_of(S, pair, P_HFR + im * 2 + 0)[g] = x;
+        plane_of(S, pair, P_HFR + im * 2 + 1)[g] = yv;
+    }
+}
+
+// sigma 1.564 blur of the high residual: high and ultra-high bands
+// (separatefrequencies.hpp:117-140, 190-200)
+__global__ __launch_bounds__(kThreads) void butter_high_kernel(Scale S, const Taps* __restrict__ taps) {
+    constexpr int R = kRad[SG_HIGH];
+    __shared__ BlurLds<R> L;
+    const Taps* T = taps + SG_HIGH;
+    const int pair = blockIdx.z >> 1, im = blockIdx.z & 1;
+    const int ox0 = blockIdx.x * kTW, oy0 = blockIdx.y * kTH;
+    const int tx = threadIdx.x & (kTW - 1), ty = threadIdx.x / kTW;
+    const int gx = ox0 + tx;
+    edge_factors<R, kTW, kTH>(L.colf, L.rowf, T, S.w, S.h, ox0, oy0);
+#pragma unroll
+    for (int c = 0; c < 2; c++) {
+        float hf[kRows], ctr[kRows];
+        blur_tile<R>(L, plane_of(S, pair, P_HFR + im * 2 + c), T, S.w, S.h, ox0, oy0, hf, ctr);
+        float* dh = plane_of(S, pair, P_HF + im * 2 + c);
+        float* du = plane_of(S, pair, P_UHF + im * 2 + c);
+#pragma unroll
+        for (int j = 0; j < kRows; j++) {
+            float hv = hf[j], uv = ctr[j];
+            if (c == 0) {
+                uv -= hv;
+                hv = remove_range(hv, 1.5f);
+                uv = remove_range(uv, 0.04f);
+            } else {
+                hv = maximum_clamp(hv, 28.4691806922f);
+                uv -= hv;
+                uv = maximum_clamp(uv, 5.19175294647f);
+                uv *= 2.69313763794f;
+                hv *= 2.155f;
+                hv = amplify_range(hv, 0.132f);
+            }
+            const int gy = oy0 + ty + 4 * j;
+            if (gx < S.w && gy < S.h) {
+                dh[(int64_t)gy * S.w + gx] = hv;
+                du[(int64_t)gy * S.w + gx] = uv;
+            }
+        }
+    }
+}
+
+// ---------------------------------------------------------------------------
+// diff: Malta line terms, L2 terms, mask inputs (main.hpp:54-110,
+// maltaDiff.hpp, simplerdiff.hpp:38-101, maskPsycho.hpp:24-40)
+// ---------------------------------------------------------------------------
+constexpr int kDW = 64, kDH = 16, kDHalo = 4;
+constexpr int kDSW = kDW + 2 * kDHalo, kDSH = kDH + 2 * kDHalo;
+constexpr int kDRows = kDH / (kThreads / kDW);
+
+// Offsets (dx, dy) of the 16 lines of each Malta filter, the same line
+// geometry as MaltaUnit / MaltaUnitLF (maltaDiff.hpp:5-512). Lines of fewer
+// than 9 taps are padded with a count.
+struct MaltaLines {
+    int n[16];
+    int d[16][9][2];
+};
+
+constexpr MaltaLines kMaltaHF = {
+    {9, 9, 7, 7, 9, 9, 9, 9, 7, 7, 7, 7, 8, 8, 8, 8},
+    {{{-4, 0}, {-3, 0}, {-2, 0}, {-1, 0}, {0, 0}, {1, 0}, {2, 0}, {3, 0}, {4, 0}},
+     {{0, -4}, {0, -3}, {0, -2}, {0, -1}, {0, 0}, {0, 1}, {0, 2}, {0, 3}, {0, 4}},
+     {{-3, -3}, {-2, -2}, {-1, -1}, {0, 0}, {1, 1}, {2, 2}, {3, 3}},
+     {{3, -3}, {2, -2}, {1, -1}, {0, 0}, {-1, 1}, {-2, 2}, {-3, 3}},
+     {{1, -4}, {1, -3}, {1, -2}, {0, -1}, {0, 0}, {0, 1}, {-1, 2}, {-1, 3}, {-1, 4}},
+     {{-1, -4}, {-1, -3}, {-1, -2}, {0, -1}, {0, 0}, {0, 1}, {1, 2}, {1, 3}, {1, 4}},
+     {{-4, -1}, {-3, -1}, {-2, -1}, {-1, 0}, {0, 0}, {1, 0}, {2, 1}, {3, 1}, {4, 1}},
+     {{-4, 1}, {-3, 1}, {-2, 1}, {-1, 0}, {0, 0}, {1, 0}, {2, -1}, {3, -1}, {4, -1}},
+     {{-2, -3}, {-1, -2}, {-1, -1}, {0, 0}, {1, 1}, {1, 2}, {2, 3}},
+     {{2, -3}, {1, -2}, {1, -1}, {0, 0}, {-1, 1}, {-1, 2}, {-2, 3}},
+     {{-3, -2}, {-2, -1}, {-1, -1}, {0, 0}, {1, 1}, {2, 1}, {3, 2}},
+     {{3, -2}, {2, -1}, {1, -1}, {0, 0}, {-1, 1}, {-2, 1}, {-3, 2}},
+     {{-4, 2}, {-3, 2}, {-2, 1}, {-1, 1}, {0, 0}, {1, 0}, {2, -1}, {3, -1}},
+     {{-4, -2}, {-3, -2}, {-2, -1}, {-1, -1}, {0, 0}, {1, 0}, {2, 1}, {3, 1}},
+     {{-2, -4}, {-2, -3}, {-1, -2}, {-1, -1}, {0, 0}, {0, 1}, {1, 2}, {1, 3}},
+     {{2, -4}, {2, -3}, {1, -2}, {1, -1}, {0, 0}, {0, 1}, {-1, 2}, {-1, 3}}},
+};
+
+constexpr MaltaLines kMaltaLF = {
+    {5, 5, 5, 5, 5, 5, 5, 5, 5, 5, 5, 5, 5, 5, 5, 5},
+    {{{-4, 0}, {-2, 0}, {0, 0}, {2, 0}, {4, 0}},
+     {{0, -4}, {0, -2}, {0, 0}, {0, 2}, {0, 4}},
+     {{-3, -3}, {-2, -2}, {0, 0}, {2, 2}, {3, 3}},
+     {{3, -3}, {2, -2}, {0, 0}, {-2, 2}, {-3, 3}},
+     {{1, -4}, {1, -2}, {0, 0}, {-1, 2}, {-1, 4}},
+     {{-1, -4}, {-1, -2}, {0, 0}, {1, 2}, {1, 4}},
+     {{-4, -1}, {-2, -1}, {0, 0}, {2, 1}, {4, 1}},
+     {{-4, 1}, {-2, 1}, {0, 0}, {2, -1}, {4, -1}},
+     {{-2, -3}, {-1, -2}, {0, 0}, {1, 2}, {2, 3}},
+     {{2, -3}, {1, -2}, {0, 0}, {-1, 2}, {-2, 3}},
+     {{-3, -2}, {-2, -1}, {0, 0}, {2, 1}, {3, 2}},
+     {{3, -2}, {2, -1}, {0, 0}, {-2, 1}, {-3, 2}},
+     {{-4, 2}, {-2, 1}, {0, 0}, {2, -1}, {4, -2}},
+     {{-4, -2}, {-2, -1}, {0, 0}, {2, 1}, {4, 2}},
+     {{-2, -4}, {-1, -2}, {0, 0}, {1, 2}, {2, 4}},
+     {{2, -4}, {1, -2}, {0, 0}, {-1, 2}, {-2, 4}}},
+};
+
+template <bool HF>
+__device__ __forceinline__ float malta_lines(const float* __restrict__ p) {
+    constexpr const MaltaLines& M = HF ? kMaltaHF : kMaltaLF;
+    float r = 0.0f;
+#pragma unroll
+    for (int l = 0; l < 16; l++) {
+        float s = 0.0f;
+#pragma unroll
+        for (int k = 0; k < 9; k++)
+            if (k < M.n[l]) s += p[M.d[l][k][1] * kDSW + M.d[l][k][0]];
+        r = fmaf(s, s, r);
+    }
+    return r;
+}
+
+// One Malta term: the scaled differences of the two planes.
+struct MaltaTerm {
+    float n2gt, n2lt, norm1;
+};
+
+__device__ __forceinline__ float malta_scaled_diff(float a, float b, const MaltaTerm& M) {
+    const float absval = 0.5f * fabsf(a) + 0.5f * fabsf(b);
+    const float diff = a - b;
+    const float scaler = M.n2gt / (M.norm1 + absval);
+    float v = scaler * diff;
+    const float scaler2 = M.n2lt / (M.norm1 + absval);
+    const float fabs0 = fabsf(a);
+    const float too_small = 0.55f * fabs0, too_big = 1.05f * fabs0;
+    float impact = 0.0f;
+    if (a < 0.0f) {
+        if (b > -too_small)
+            impact = b + too_small;
+        else if (b < -too_big)
+            impact = -b - too_big;
+    } else {
+        if (b < too_small)
+            impact = -b + too_small;
+        else if (b > too_big)
+            impact = b - too_big;
+    }
+    impact *= scaler2;
+    return diff < 0.0f ? v - impact : v + impact;
+}
+
+struct DiffParams {
+    MaltaTerm malta[6];  // uhf Y, uhf X, hf Y, hf X, mf Y, mf X
+    float asym_gt[2], asym_lt[2];
+};
+
+__device__ __forceinline__ float l2_asym(float a, float b, float wgt, float wlt) {
+    const float diff = a - b;
+    float r = wgt * diff * diff;
+    const float fabs0 = fabsf(a);
+    const float too_small = 0.4f * fabs0, too_big = fabs0;
+    float v = 0.0f;
+    if (a < 0.0f) {
+        if (b > -too_small)
+            v = b + too_small;
+        else if (b < -too_big)
+            v = -b - too_big;
+    } else {
+        if (b < too_small)
+            v = too_small - b;
+        else if (b > too_big)
+            v = b - too_big;
+    }
+    return r + wlt * v * v;
+}
+
+__global__ __launch_bounds__(kThreads) void butter_diff_kernel(Scale S, DiffParams D) {
+    __shared__ float dt[kDSH * kDSW];
+    const int pair = blockIdx.z;
+    const int ox0 = blockIdx.x * kDW, oy0 = blockIdx.y * kDH;
+    const int tx = threadIdx.x & (kDW - 1), ty = threadIdx.x / kDW;
+    const int w = S.w, h = S.h;
+    float ac[kDRows];
+#pragma unroll
+    for (int j = 0; j < kDRows; j++) ac[j] = 0.0f;
+    // (plane of image 0, of image 1, HF lines) per term
+    const int src0[6] = {P_UHF + 1, P_UHF + 0, P_HF + 1, P_HF + 0, P_MF + 1, P_MF + 0};
+    const int src1[6] = {P_UHF + 3, P_UHF + 2, P_HF + 3, P_HF + 2, P_MF + 4, P_MF + 3};
+#pragma unroll
+    for (int t = 0; t < 6; t++) {
+        const float* a = plane_of(S, pair, src0[t]);
+        const float* b = plane_of(S, pair, src1[t]);
+        __syncthreads();
+        for (int i = threadIdx.x; i < kDSW * kDSH; i += kThreads) {
+            const int r = i / kDSW, c = i - r * kDSW;
+            const int gx = ox0 - kDHalo + c, gy = oy0 - kDHalo + r;
+            float v = 0.0f;
+            if (gx >= 0 && gx < w && gy >= 0 && gy < h) {
+                const int64_t g = (int64_t)gy * w + gx;
+                v = malta_scaled_diff(a[g], b[g], D.malta[t]);
+            }
+            dt[i] = v;
+        }
+        __syncthreads();
+#pragma unroll
+        for (int j = 0; j < kDRows; j++) {
+            const float* p = dt + (ty + 4 * j + kDHalo) * kDSW + tx + kDHalo;
+            ac[j] += t < 2 ? malta_lines<true>(p) : malta_lines<false>(p);
+        }
+    }
+    const int gx = ox0 + tx;
+    if (gx >= w) return;
+#pragma unroll
+    for (int j = 0; j < kDRows; j++) {
+        const int gy = oy0 + ty + 4 * j;
+        if (gy >= h) continue;
+        const int64_t g = (int64_t)gy * w + gx;
+        float acc = ac[j];
+        float hf[2][2], uhf[2][2];
+#pragma unroll
+        for (int im = 0; im < 2; im++)
+#pragma unroll
+            for (int c = 0; c < 2; c++) {
+                hf[im][c] = plane_of(S, pair, P_HF + im * 2 + c)[g];
+                uhf[im][c] = plane_of(S, pair, P_UHF + im * 2 + c)[g];
+            }
+        acc += l2_asym(hf[0][0], hf[1][0], D.asym_gt[0], D.asym_lt[0]);
+        acc += l2_asym(hf[0][1], hf[1][1], D.asym_gt[1], D.asym_lt[1]);
+        const float wmf[3] = {2150.0f, 10.6195433239f, 16.2176043152f};
+        const float wlf[3] = {29.2353797994f, 0.844626970982f, 0.703646627719f};
+        float dc = 0.0f;
+#pragma unroll
+        for (int c = 0; c < 3; c++) {
+            const float dm = plane_of(S, pair, P_MF + c)[g] - plane_of(S, pair, P_MF + 3 + c)[g];
+            acc += wmf[c] * dm * dm;
+            const float dl = plane_of(S, pair, P_LFV + c)[g] - plane_of(S, pair, P_LFV + 3 + c)[g];
+            dc += wlf[c] * dl * dl;
+        }
+        plane_of(S, pair, P_ACS)[g] = acc;
+        plane_of(S, pair, P_DCS)[g] = dc;
+        plane_of(S, pair, P_M0)[g] = mask_input(hf[0][0], uhf[0][0], hf[0][1], uhf[0][1]);
+        plane_of(S, pair, P_M1)[g] = mask_input(hf[1][0], uhf[1][0], hf[1][1], uhf[1][1]);
+    }
+}
+
+// ---------------------------------------------------------------------------
+// mask: sigma 2.7 blur of both mask inputs, fuzzy erosion of the reference's,
+// L2 term of the two, final map (maskPsycho.hpp:57-132, combineMasks.hpp);
+// at full scale the blend with the half-scale map (downupsample.hpp:33-56)
+// and the norm partial sums (diffnorms.hpp:48-83).
+// 32 x 32 outputs; the eroded mask needs the blurred reference mask on a
+// 3-pixel halo.
+// ---------------------------------------------------------------------------
+constexpr int kMT = 32, kMErode = 3;
+constexpr int kMB = kMT + 2 * kMErode;  // blurred region 38 x 38
+
+struct MaskLds {
+    static constexpr int R = kRad[SG_MASK];
+    static constexpr int SW = kMB + 2 * R;  // 54
+    float in[SW * SW];
+    float mid[SW * kMB];
+    float b0[kMB * kMB];
+    float colf[kMB];
+    float rowf[kMB];
+    double red[3][kThreads / 64];
+};
+
+struct NormArgs {
+    int qnorm;
+    double* partials;  // [pair][tile][3]
+    int tiles;
+    const PairDesc* pairs;
+    const float* half_map_base;  // half-scale P_MAP of pair 0
+    int64_t half_pair;           // floats between pairs in the half scale
+    int half_w;
+};
+
+__device__ __forceinline__ double wave_sum(double v) {
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+    return v;
+}
+__device__ __forceinline__ double wave_max(double v) {
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) v = fmax(v, __shfl_xor(v, o, 64));
+    return v;
+}
+
+template <bool FULL>
+__global__ __launch_bounds__(kThreads) void butter_mask_kernel(Scale S, const Taps* __restrict__ taps, NormArgs N) {
+    constexpr int R = MaskLds::R, SW = MaskLds::SW;
+    __shared__ MaskLds L;
+    const Taps* T = taps + SG_MASK;
+    const int pair = blockIdx.z;
+    const int ox0 = blockIdx.x * kMT, oy0 = blockIdx.y * kMT;
+    const int w = S.w, h = S.h;
+    edge_factors<R, kMB, kMB>(L.colf, L.rowf, T, w, h, ox0 - kMErode, oy0 - kMErode);
+    // blurred reference mask on the 38 x 38 region
+    stage_plane<SW, SW>(L.in, plane_of(S, pair, P_M0), w, h, ox0 - kMErode - R, oy0 - kMErode - R);
+    __syncthreads();
+    hpass<R, SW, SW, kMB>(L.in, L.mid, L.colf, T);
+    __syncthreads();
+    for (int i = threadIdx.x; i < kMB * kMB; i += kThreads) {
+        const int r = i / kMB, c = i - r * kMB;
+        L.b0[i] = vsum<R, kMB>(L.mid, T, r, c) * L.rowf[r];
+    }
+    __syncthreads();
+    // blurred distorted mask on the 32 x 32 tile
+    constexpr int SW1 = kMT + 2 * R;
+    stage_plane<SW1, SW1>(L.in, plane_of(S, pair, P_M1), w, h, ox0 - R, oy0 - R);
+    __syncthreads();
+    hpass<R, SW1, SW1, kMT>(L.in, L.mid, L.colf + kMErode, T);
+    __syncthreads();
+    const int tx = threadIdx.x & (kMT - 1), ty = threadIdx.x / kMT;
+    constexpr int kMRows = kMT / (kThreads / kMT);
+    double sq = 0.0, s3 = 0.0, mx = 0.0;
+    const int gx = ox0 + tx;
+#pragma unroll
+    for (int j = 0; j < kMRows; j++) {
+        const int r = ty + 8 * j;
+        const int gy = oy0 + r;
+        const float b1 = vsum<R, kMT>(L.mid, T, r, tx) * L.rowf[r + kMErode];
+        if (gx >= w || gy >= h) continue;
+        const float* bc = L.b0 + (r + kMErode) * kMB + tx + kMErode;
+        const float b0 = bc[0];
+        float m0 = b0, m1 = 2.0f * b0, m2 = m1;
+        if (gx >= kMErode) {
+            store_min3(bc[-kMErode], m0, m1, m2);
+            if (gy >= kMErode) store_min3(bc[-kMErode * kMB - kMErode], m0, m1, m2);
+            if (gy < h - kMErode) store_min3(bc[kMErode * kMB - kMErode], m0, m1, m2);
+        }
+        if (gx < w - kMErode) {
+            store_min3(bc[kMErode], m0, m1, m2);
+            if (gy >= kMErode) store_min3(bc[-kMErode * kMB + kMErode], m0, m1, m2);
+            if (gy < h - kMErode) store_min3(bc[kMErode * kMB + kMErode], m0, m1, m2);
+        }
+        if (gy >= kMErode) store_min3(bc[-kMErode * kMB], m0, m1, m2);
+        if (gy < h - kMErode) store_min3(bc[kMErode * kMB], m0, m1, m2);
+        const float mask = 0.45f * m0 + 0.3f * m1 + 0.25f * m2;
+        const int64_t g = (int64_t)gy * w + gx;
+        const float db = b0 - b1;
+        const float acs = plane_of(S, pair, P_ACS)[g] + 10.0f * db * db;
+        const float dcs = plane_of(S, pair, P_DCS)[g];
+        float v = sqrtf(mask_y(mask) * acs + mask_dc_y(mask) * dcs);
+        if (!FULL) {
+            plane_of(S, pair, P_MAP)[g] = v;
+        } else {
+            v = (float)((double)v * (1.0 - 0.3 * 0.5));
+            v += 0.5f * N.half_map_base[(int64_t)pair * N.half_pair + (int64_t)(gy / 2) * N.half_w + gx / 2];
+            const PairDesc& P = N.pairs[pair];
+            if (P.map) ((float*)((uint8_t*)P.map + (int64_t)gy * P.map_stride))[gx] = v;
+            const double dv = v;
+            double pq = 1.0;
+            for (int k = 0; k < N.qnorm; k++) pq *= dv;
+            for (int k = 0; k < -N.qnorm; k++) pq /= dv;
+            sq += pq;
+            s3 += fabs(dv) * dv * dv;
+            mx = fmax(mx, fabs(dv));
+        }
+    }
+    if (FULL) {
+        sq = wave_sum(sq);
+        s3 = wave_sum(s3);
+        mx = wave_max(mx);
+        const int wv = threadIdx.x / 64;
+        if ((threadIdx.x & 63) == 0) {
+            L.red[0][wv] = sq;
+            L.red[1][wv] = s3;
+            L.red[2][wv] = mx;
+        }
+        __syncthreads();
+        if (threadIdx.x == 0) {
+            double a = 0.0, b = 0.0, m = 0.0;
+            for (int k = 0; k < kThreads / 64; k++) {
+                a += L.red[0][k];
+                b += L.red[1][k];
+                m = fmax(m, L.red[2][k]);
+            }
+            double* out = N.partials + ((int64_t)pair * N.tiles + (int64_t)blockIdx.y * gridDim.x + blockIdx.x) * 3;
+            out[0] = a;
+            out[1] = b;
+            out[2] = m;
+        }
+    }
+}
+
+// norms of each pair from its tile partials (diffnorms.hpp:85-127)
+__global__ __launch_bounds__(kThreads) void butter_norm_kernel(const double* __restrict__ partials, int tiles,
+                                                               int qnorm, int64_t npix, double* __restrict__ norms) {
+    __shared__ double red[3][kThreads / 64];
+    const int pair = blockIdx.x;
+    const double* p = partials + (int64_t)pair * tiles * 3;
+    double a = 0.0, b = 0.0, m = 0.0;
+    for (int i = threadIdx.x; i < tiles; i += kThreads) {
+        a += p[3 * i];
+        b += p[3 * i + 1];
+        m = fmax(m, p[3 * i + 2]);
+    }
+    a = wave_sum(a);
+    b = wave_sum(b);
+    m = wave_max(m);
+    const int wv = threadIdx.x / 64;
+    if ((threadIdx.x & 63) == 0) {
+        red[0][wv] = a;
+        red[1][wv] = b;
+        red[2][wv] = m;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        double sa = 0.0, sb = 0.0, sm = 0.0;
+        for (int k = 0; k < kThreads / 64; k++) {
+            sa += red[0][k];
+            sb += red[1][k];
+            sm = fmax(sm, red[2][k]);
+        }
+        norms[3 * pair + 0] = pow(sa / (double)npix, 1.0 / (double)qnorm);
+        norms[3 * pair + 1] = pow(sb / (double)npix, 1.0 / 3.0);
+        norms[3 * pair + 2] = sm;
+    }
+}
+
+int cdiv(int a, int b) { return (a + b - 1) / b; }
+
+int64_t plane_floats(int w, int h) { return ((int64_t)w * h + 63) / 64 * 64; }
+
+// Malta and asymmetric L2 weights as the reference's getdiffmap passes them
+// (main.hpp:57-89) and its launchers scale them (maltaDiff.hpp:663-683,
+// simplerdiff.hpp:103-107).
+DiffParams diff_params() {
+    DiffParams D;
+    const float asym = 0.8f, sasym = std::sqrt(asym);
+    const float len2 = 3.75f * 2.0f + 1.0f;
+    auto term = [&](bool hf, float wgt, float wlt, float norm1) {
+        const float mulli = hf ? 0.39905817637f : 0.611612573796f;
+        MaltaTerm t;
+        t.n2gt = mulli * std::sqrt(0.5f * wgt) / len2 * norm1;
+        t.n2lt = mulli * std::sqrt(0.33f * wlt) / len2 * norm1;
+        t.norm1 = norm1;
+        return t;
+    };
+    D.malta[0] = term(true, 1.10039032555f * asym, 1.10039032555f / asym, 71.7800275169f);
+    D.malta[1] = term(true, 173.5f * asym, 173.5f / asym, 5.0f);
+    D.malta[2] = term(false, 18.7237414387f * sasym, 18.7237414387f / sasym, 4498534.45232f);
+    D.malta[3] = term(false, 6923.99476109f * sasym, 6923.99476109f / sasym, 8051.15833247f);
+    D.malta[4] = term(false, 37.0819870399f, 37.0819870399f, 130262059.556f);
+    D.malta[5] = term(false, 8246.75321353f, 8246.75321353f, 1009002.70582f);
+    const float wmul[2] = {400.0f, 1.50815703118f};
+    for (int c = 0; c < 2; c++) {
+        D.asym_gt[c] = (float)((double)(wmul[c] * asym) * 0.8);
+        D.asym_lt[c] = (float)((double)(wmul[c] / asym) * 0.8);
+    }
+    return D;
+}
+
+}  // namespace
+
+uint64_t Engine::pair_bytes(int width, int height) {
+    const int w2 = (width - 1) / 2 + 1, h2 = (height - 1) / 2 + 1;
+    return (uint64_t)P_COUNT * (plane_floats(width, height) + plane_floats(w2, h2)) * sizeof(float);
+}
+
+Engine::Engine(int width, int height, int capacity, int qnorm, float intensity, int device)
+    : w_(width), h_(height), cap_(capacity), q_(qnorm), intensity_(intensity), device_(device) {
+    if (width <= 0 || height <= 0 || capacity <= 0 || capacity > 4096) VSHIP_THROW(E_BadPointer);
+    w2_ = (w_ - 1) / 2 + 1;
+    h2_ = (h_ - 1) / 2 + 1;
+    plane0_ = plane_floats(w_, h_);
+    plane1_ = plane_floats(w2_, h2_);
+    HIP_OR_THROW(hipSetDevice(device_), E_BadDeviceArgument);
+    try {
+        HIP_OR_THROW(hipStreamCreate(&stream_), E_BadDeviceArgument);
+        const size_t arena = (size_t)cap_ * P_COUNT * (plane0_ + plane1_) * sizeof(float);
+        HIP_OR_THROW(hipMalloc(&arena_, arena), E_OutOfVRAM);
+        mask_tiles_ = cdiv(w_, kMT) * cdiv(h_, kMT);
+        HIP_OR_THROW(hipMalloc(&partials_d_, sizeof(double) * 3 * mask_tiles_ * cap_), E_OutOfVRAM);
+        HIP_OR_THROW(hipMalloc(&norms_d_, sizeof(double) * 3 * cap_), E_OutOfVRAM);
+        HIP_OR_THROW(hipHostMalloc(&norms_h_, sizeof(double) * 3 * cap_), E_OutOfRAM);
+        HIP_OR_THROW(hipMalloc(&desc_d_, sizeof(PairDesc) * cap_), E_OutOfVRAM);
+        HIP_OR_THROW(hipHostMalloc(&desc_h_, sizeof(PairDesc) * cap_), E_OutOfRAM);
+        HIP_OR_THROW(hipMalloc(&taps_d_, sizeof(Taps) * kSigmaCount), E_OutOfVRAM);
+        Taps th[kSigmaCount];
+        std::memset(th, 0, sizeof(th));
+        for (int i = 0; i < kSigmaCount; i++) {
+            const int R = kRad[i];
+            double g[43], tot = 0.0;
+            for (int k = -R; k <= R; k++) tot += g[k + R] = std::exp(-(double)k * k / (2.0 * kSigma[i] * kSigma[i]));
+            double run = 0.0;
+            for (int k = 0; k <= 2 * R; k++) {
+                th[i].cum[k] = (float)(run / tot);
+                th[i].g[k] = (float)(g[k] / tot);
+                run += g[k];
+            }
+            th[i].cum[2 * R + 1] = (float)(run / tot);
+        }
+        HIP_OR_THROW(hipMemcpy(taps_d_, th, sizeof(th), hipMemcpyHostToDevice), E_BadDeviceArgument);
+        for (auto& e : ev_) HIP_OR_THROW(hipEventCreate(&e), E_BadDeviceArgument);
+    } catch (...) {
+        release();
+        throw;
+    }
+}
+
+Engine::~Engine() { release(); }
+
+void Engine::release() {
+    if (stream_) (void)hipStreamSynchronize(stream_);
+    for (auto& e : ev_)
+        if (e) (void)hipEventDestroy(e), e = nullptr;
+    if (arena_) (void)hipFree(arena_), arena_ = nullptr;
+    if (partials_d_) (void)hipFree(partials_d_), partials_d_ = nullptr;
+    if (norms_d_) (void)hipFree(norms_d_), norms_d_ = nullptr;
+    if (norms_h_) (void)hipHostFree(norms_h_), norms_h_ = nullptr;
+    if (desc_d_) (void)hipFree(desc_d_), desc_d_ = nullptr;
+    if (desc_h_) (void)hipHostFree(desc_h_), desc_h_ = nullptr;
+    if (taps_d_) (void)hipFree(taps_d_), taps_d_ = nullptr;
+    if (stream_) (void)hipStreamDestroy(stream_), stream_ = nullptr;
+}
+
+void Engine::submit(int fmt, const PairDesc* pairs, int n) {
+    if (n <= 0 || n > cap_ || !pairs) VSHIP_THROW(E_BadPointer);
+    if (fmt < 0 || fmt >= IN_NUM_FORMATS) VSHIP_THROW(E_BadPointer);
+    // bytes per sample and plane widths of the format: every row the kernel
+    // reads must lie within the plane's stride
+    const int bps = fmt_yuv(fmt) ? (fmt_wide(fmt) ? 2 : 1) : (fmt == IN_RGB_U8 ? 1 : fmt == IN_RGB_F32 ? 4 : 2);
+    int64_t row[3];
+    for (int c = 0; c < 3; c++) {
+        const int pw = (c == 0 || !fmt_yuv(fmt)) ? w_ : (w_ + (1 << fmt_csw(fmt)) - 1) >> fmt_csw(fmt);
+        row[c] = (int64_t)pw * bps;
+    }
+    for (int i = 0; i < n; i++) {
+        for (int im = 0; im < 2; im++)
+            for (int c = 0; c < 3; c++) {
+                if (!pairs[i].src[im][c]) VSHIP_THROW(E_BadPointer);
+                if (pairs[i].stride[im][c] < row[c]) VSHIP_THROW(E_BadPointer);
+            }
+        if (pairs[i].map && pairs[i].map_stride < (int64_t)w_ * 4) VSHIP_THROW(E_BadPointer);
+        if (fmt_yuv(fmt)) {
+            const int bits = pairs[i].bits;
+            if (fmt_wide(fmt) ? (bits < 9 || bits > 16) : bits != 8) VSHIP_THROW(E_BadPointer);
+            if (pairs[i].cloc < 0 || pairs[i].cloc > 3) VSHIP_THROW(E_BadPointer);
+        }
+    }
+    HIP_OR_THROW(hipSetDevice(device_), E_BadDeviceArgument);
+    // the previous batch may still read the pinned descriptors
+    HIP_OR_THROW(hipStreamSynchronize(stream_), E_BadDeviceArgument);
+    std::memcpy(desc_h_, pairs, sizeof(PairDesc) * n);
+    HIP_OR_THROW(hipMemcpyAsync(desc_d_, desc_h_, sizeof(PairDesc) * n, hipMemcpyHostToDevice, stream_),
+                 E_BadDeviceArgument);
+    launch(fmt, n);
+    last_fmt_ = fmt;
+    HIP_OR_THROW(hipMemcpyAsync(norms_h_, norms_d_, sizeof(double) * 3 * n, hipMemcpyDeviceToHost, stream_),
+                 E_BadDeviceArgument);
+    last_n_ = n;
+}
+
+void Engine::run(int fmt, const PairDesc* pairs, int n, double* norms) {
+    submit(fmt, pairs, n);
+    HIP_OR_THROW(hipStreamSynchronize(stream_), E_BadDeviceArgument);
+    if (norms) std::memcpy(norms, norms_h_, sizeof(double) * 3 * n);
+}
+
+void Engine::launch(int fmt, int n) {
+    const Scale S0{arena_, w_, h_, plane0_, (int64_t)P_COUNT * plane0_};
+    const Scale S1{arena_ + (int64_t)cap_ * P_COUNT * plane0_, w2_, h2_, plane1_, (int64_t)P_COUNT * plane1_};
+    const DiffParams D = diff_params();
+    const bool prof = profile_;
+    auto mark = [&](int st) {
+        if (prof) (void)hipEventRecord(ev_[st], stream_);
+    };
+    // stage marks: ev_[s] before the stage's first launch; the half and full
+    // scale launches of a stage are interleaved with others, so per-stage time
+    // is taken by a second, stage-ordered pass only when profiling
+    auto scale_pass = [&](const Scale& S, bool full, int only) {
+        const dim3 blk(kThreads);
+        const dim3 gt(cdiv(S.w, kTW), cdiv(S.h, kTH), 2 * n);
+        if (only < 0 || only == ST_OPSIN) butter_opsin_kernel<<<gt, blk, 0, stream_>>>(S, taps_d_, intensity_);
+        if (only < 0 || only == ST_LOW) butter_low_kernel<<<gt, blk, 0, stream_>>>(S, taps_d_);
+        if (only < 0 || only == ST_MID) butter_mid_kernel<<<gt, blk, 0, stream_>>>(S, taps_d_);
+        if (only < 0 || only == ST_HIGH) butter_high_kernel<<<gt, blk, 0, stream_>>>(S, taps_d_);
+        if (only < 0 || only == ST_DIFF)
+            butter_diff_kernel<<<dim3(cdiv(S.w, kDW), cdiv(S.h, kDH), n), blk, 0, stream_>>>(S, D);
+        if (only < 0 || only == ST_MASK) {
+            NormArgs N;
+            N.qnorm = q_;
+            N.partials = partials_d_;
+            N.tiles = mask_tiles_;
+            N.pairs = desc_d_;
+            N.half_map_base = S1.base + (int64_t)P_MAP * S1.plane;
+            N.half_pair = S1.pair;
+            N.half_w = S1.w;
+            const dim3 gm(cdiv(S.w, kMT), cdiv(S.h, kMT), n);
+            if (full)
+                butter_mask_kernel<true><<<gm, blk, 0, stream_>>>(S, taps_d_, N);
+            else
+                butter_mask_kernel<false><<<gm, blk, 0, stream_>>>(S, taps_d_, N);
+        }
+    };
+    const dim3 gi(cdiv(w2_, 16), cdiv(h2_, 16), n);
+    auto input = [&] {
+        launch_input_fmt(fmt, gi, stream_, desc_d_, S0, S1, std::make_integer_sequence<int, IN_NUM_FORMATS>{});
+    };
+    if (!prof) {
+        input();
+        scale_pass(S1, false, -1);
+        scale_pass(S0, true, -1);
+        butter_norm_kernel<<<dim3(n), dim3(kThreads), 0, stream_>>>(partials_d_, mask_tiles_, q_, (int64_t)w_ * h_,
+                                                                    norms_d_);
+        timed_ = false;
+    } else {
+        // stage-ordered: every stage's half- and full-scale launches back to
+        // back between two events; the data dependences hold because each
+        // stage of the half scale only reads half-scale planes
+        mark(ST_INPUT);
+        input();
+        for (int st = ST_OPSIN; st <= ST_MASK; st++) {
+            mark(st);
+            scale_pass(S1, false, st);
+            scale_pass(S0, true, st);
+        }
+        mark(ST_NORM);
+        butter_norm_kernel<<<dim3(n), dim3(kThreads), 0, stream_>>>(partials_d_, mask_tiles_, q_, (int64_t)w_ * h_,
+                                                                    norms_d_);
+        mark(ST_COUNT);
+        timed_ = true;
+    }
+    HIP_OR_THROW(hipGetLastError(), E_BadDeviceCode);
+}
+
+bool Engine::stage_ms(float* out) const {
+    if (!timed_) return false;
+    for (int s = 0; s < ST_COUNT; s++)
+        if (hipEventElapsedTime(&out[s], ev_[s], ev_[s + 1]) != hipSuccess) return false;
+    return true;
+}
+
+uint64_t Engine::stage_bytes(int stage) const {
+    // planes read and written per pixel and image pair, at each scale
+    // (inputs: the sampled bytes; everything else FP32)
+    const double px = (double)w_ * h_ + (double)w2_ * h2_;
+    const double full = (double)w_ * h_;
+    double b = 0.0;
+    switch (stage) {
+        case ST_INPUT: {
+            // samples read (chroma planes at their subsampled size), linear
+            // planes written at both scales
+            const int f = last_fmt_;
+            const double bps = fmt_yuv(f) ? (fmt_wide(f) ? 2 : 1) : (f == IN_RGB_U8 ? 1 : f == IN_RGB_F32 ? 4 : 2);
+            const double chroma = fmt_yuv(f) ? 1.0 / (double)(1 << (fmt_csw(f) + fmt_csh(f))) : 1.0;
+            b = full * 2 * (1 + 2 * chroma) * bps + full * 6 * 4 + (double)w2_ * h2_ * 6 * 4;
+            break;
+        }
+        case ST_OPSIN: b = px * (6 + 6) * 4; break;
+        case ST_LOW: b = px * (6 + 12) * 4; break;
+        case ST_MID: b = px * (6 + 10) * 4; break;
+        case ST_HIGH: b = px * (4 + 8) * 4; break;
+        case ST_DIFF: b = px * (20 + 4) * 4; break;
+        case ST_MASK: b = px * (4 + 1) * 4 + (double)w2_ * h2_ * 4; break;
+        default: b = 0.0;
+    }
+    return (uint64_t)b;
+}
+
+}  // namespace bt
+}  // namespace vship

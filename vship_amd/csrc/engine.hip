@@ -45,14 +45,6 @@ Taps make_taps() {
     return t;
 }
 
-void matrix_coefs(int matrix, float& kr, float& kb) {
-    switch (matrix) {
-    case 1: kr = 0.299f; kb = 0.114f; break;       // BT.601
-    case 2: kr = 0.2627f; kb = 0.0593f; break;     // BT.2020 non-constant luminance
-    default: kr = 0.2126f; kb = 0.0722f; break;    // BT.709
-    }
-}
-
 // Every plane must lie inside one device allocation with rows that hold the
 // image: a bad pointer or stride becomes an error here instead of a GPU fault.
 void validate_planes(const PairInput& p) {
@@ -77,6 +69,14 @@ void validate_planes(const PairInput& p) {
 }
 
 }  // namespace
+
+void matrix_coefs(int matrix, float& kr, float& kb) {
+    switch (matrix) {
+    case 1: kr = 0.299f; kb = 0.114f; break;       // BT.601
+    case 2: kr = 0.2627f; kb = 0.0593f; break;     // BT.2020 non-constant luminance
+    default: kr = 0.2126f; kb = 0.0722f; break;    // BT.709
+    }
+}
 
 PairLayout pair_layout(int w, int h, bool fp16) {
     PairLayout L;

@@ -49,6 +49,10 @@ struct VshipErr : std::exception {
         if (e_ != hipSuccess) VSHIP_THROW(code); \
     } while (0)
 
+// Luma coefficients (kr, kb) of a YUV matrix: 0 BT.709, 1 BT.601,
+// 2 BT.2020 non-constant luminance.
+void matrix_coefs(int matrix, float& kr, float& kb);
+
 struct PairInput {
     const uint8_t* src[2][3];  // device pointers [image][plane]
     int64_t stride[2][3];

@@ -90,13 +90,16 @@ enum Slot : int {
     createVideoFilter = 0,
     freeNode = 7,
     getVideoInfo = 10,
+    newVideoFrame = 12,
     freeFrame = 16,
     copyFrame = 18,
     getFramePropertiesRW = 20,
     getStride = 21,
     getReadPtr = 22,
+    getWritePtr = 23,
     getFrameWidth = 27,
     getFrameHeight = 28,
+    queryVideoFormat = 32,
     getFrameFilter = 38,
     requestFrameFilter = 39,
     setFilterError = 42,
@@ -106,6 +109,7 @@ enum Slot : int {
     mapGetError = 52,
     mapGetInt = 59,
     mapSetInt = 62,
+    mapGetFloat = 64,
     mapSetFloat = 67,
     mapSetData = 72,
     mapGetNode = 73,
@@ -129,13 +133,18 @@ VS4_SIG(createVideoFilter, void (*type)(Map* out, const char* name, const VideoI
                                         void* instanceData, Core* core));
 VS4_SIG(freeNode, void (*type)(Node* node));
 VS4_SIG(getVideoInfo, const VideoInfo* (*type)(Node* node));
+VS4_SIG(newVideoFrame,
+        Frame* (*type)(const VideoFormat* format, int width, int height, const Frame* propSrc, Core* core));
 VS4_SIG(freeFrame, void (*type)(const Frame* f));
 VS4_SIG(copyFrame, Frame* (*type)(const Frame* f, Core* core));
 VS4_SIG(getFramePropertiesRW, Map* (*type)(Frame* f));
 VS4_SIG(getStride, ptrdiff_t (*type)(const Frame* f, int plane));
 VS4_SIG(getReadPtr, const uint8_t* (*type)(const Frame* f, int plane));
+VS4_SIG(getWritePtr, uint8_t* (*type)(Frame* f, int plane));
 VS4_SIG(getFrameWidth, int (*type)(const Frame* f, int plane));
 VS4_SIG(getFrameHeight, int (*type)(const Frame* f, int plane));
+VS4_SIG(queryVideoFormat, int (*type)(VideoFormat* format, int colorFamily, int sampleType, int bitsPerSample,
+                                       int subSamplingW, int subSamplingH, Core* core));
 VS4_SIG(getFrameFilter, const Frame* (*type)(int n, Node* node, FrameContext* frameCtx));
 VS4_SIG(requestFrameFilter, void (*type)(int n, Node* node, FrameContext* frameCtx));
 VS4_SIG(setFilterError, void (*type)(const char* errorMessage, FrameContext* frameCtx));
@@ -145,6 +154,7 @@ VS4_SIG(mapSetError, void (*type)(Map* map, const char* errorMessage));
 VS4_SIG(mapGetError, const char* (*type)(const Map* map));
 VS4_SIG(mapGetInt, int64_t (*type)(const Map* map, const char* key, int index, int* error));
 VS4_SIG(mapSetInt, int (*type)(Map* map, const char* key, int64_t i, int append));
+VS4_SIG(mapGetFloat, double (*type)(const Map* map, const char* key, int index, int* error));
 VS4_SIG(mapSetFloat, int (*type)(Map* map, const char* key, double d, int append));
 VS4_SIG(mapSetData, int (*type)(Map* map, const char* key, const char* data, int size, int type, int append));
 VS4_SIG(mapGetNode, Node* (*type)(const Map* map, const char* key, int index, int* error));

@@ -1,6 +1,6 @@
-// VapourSynth plugin of libvship.so: vship.SSIMULACRA2 and vship.GpuInfo,
-// the filter interface of the reference (src/VshipLib.cpp:8-64,
-// src/ssimu2/vapoursynth.hpp:8-166).
+// VapourSynth plugin of libvship.so: vship.SSIMULACRA2, vship.BUTTERAUGLI and
+// vship.GpuInfo, the filter interface of the reference (src/VshipLib.cpp:8-64,
+// src/ssimu2/vapoursynth.hpp:8-166, src/butter/vapoursynth.hpp:6-193).
 //
 //   SSIMULACRA2(reference:vnode, distorted:vnode, numStream:int:opt, gpu_id:int:opt) -> clip
 //     output frame n = the RGBS form of distorted frame n with the float
@@ -10,6 +10,14 @@
 //     input matrix is BT.709 above 650 rows and BT.601 below), as
 //     src/util/torgbs.hpp:8-36 does, so the scores and output frames are
 //     the reference's.
+//   BUTTERAUGLI(reference:vnode, distorted:vnode, qnorm:int:opt,
+//               intensity_multiplier:float:opt, distmap:int:opt, numStream:int:opt,
+//               gpu_id:int:opt) -> clip
+//     output frame n = the RGBS form of distorted frame n, or with distmap=1
+//     the GrayS distortion map, carrying the float properties
+//     _BUTTERAUGLI_QNorm, _BUTTERAUGLI_3Norm and _BUTTERAUGLI_INFNorm.
+//     Defaults: qnorm 2, intensity_multiplier 203, numStream 4 (clamped to
+//     [1, threads]), as there.
 //   GpuInfo(gpu_id:int:opt) -> gpu_human_data
 //
 // numStream per-frame engines (default: the core's thread count, clamped
@@ -46,8 +54,9 @@ std::string vs_error_text(const VshipErr& e) {
 std::string vs_error_text(ErrCode c, const char* file, int line) { return vs_error_text(VshipErr(c, file, line)); }
 
 // A pool of per-frame engines; a frame request takes one, scores, returns it.
+template <typename H>
 struct EnginePool {
-    std::vector<std::unique_ptr<FrameHandle>> handles;
+    std::vector<std::unique_ptr<H>> handles;
     std::vector<int> free_ids;
     std::mutex mu;
     std::condition_variable cv;
@@ -71,7 +80,15 @@ struct EnginePool {
 struct Ssimu2Filter {
     vs4::Node* reference = nullptr;
     vs4::Node* distorted = nullptr;
-    EnginePool pool;
+    EnginePool<FrameHandle> pool;
+};
+
+struct ButterFilter {
+    vs4::Node* reference = nullptr;
+    vs4::Node* distorted = nullptr;
+    bool distmap = false;
+    vs4::VideoFormat gray{};
+    EnginePool<ButterHandle> pool;
 };
 
 // The RGBS form of a clip (src/util/torgbs.hpp:8-36): the clip itself when it
@@ -222,6 +239,124 @@ void ssimu2_create(const vs4::Map* in, vs4::Map* out, void*, vs4::Core* core, co
                                       deps, 2, (void*)d.release(), core);
 }
 
+const vs4::Frame* butter_get_frame(int n, int reason, void* inst, void** /*frameData*/, vs4::FrameContext* ctx,
+                                   vs4::Core* core, const Api* api) {
+    ButterFilter* d = static_cast<ButterFilter*>(inst);
+    if (reason == vs4::arInitial) {
+        api->call<vs4::requestFrameFilter>(n, d->reference, ctx);
+        api->call<vs4::requestFrameFilter>(n, d->distorted, ctx);
+        return nullptr;
+    }
+    if (reason != vs4::arAllFramesReady) return nullptr;
+    const vs4::Frame* f1 = api->call<vs4::getFrameFilter>(n, d->reference, ctx);
+    const vs4::Frame* f2 = api->call<vs4::getFrameFilter>(n, d->distorted, ctx);
+    const int w = api->call<vs4::getFrameWidth>(f1, 0), h = api->call<vs4::getFrameHeight>(f1, 0);
+    const uint8_t* p1[3];
+    const uint8_t* p2[3];
+    for (int i = 0; i < 3; i++) {
+        p1[i] = api->call<vs4::getReadPtr>(f1, i);
+        p2[i] = api->call<vs4::getReadPtr>(f2, i);
+    }
+    const int64_t s1 = (int64_t)api->call<vs4::getStride>(f1, 0);
+    const int64_t s2 = (int64_t)api->call<vs4::getStride>(f2, 0);
+    vs4::Frame* dst = d->distmap ? api->call<vs4::newVideoFrame>((const vs4::VideoFormat*)&d->gray, w, h,
+                                                                 (const vs4::Frame*)nullptr, core)
+                                 : api->call<vs4::copyFrame>(f2, core);
+    float* mp = d->distmap ? reinterpret_cast<float*>(api->call<vs4::getWritePtr>(dst, 0)) : nullptr;
+    const int64_t ms = d->distmap ? (int64_t)api->call<vs4::getStride>((const vs4::Frame*)dst, 0) : 0;
+    double norms[3] = {0.0, 0.0, 0.0};
+    std::string err;
+    const int id = d->pool.take();
+    try {
+        d->pool.handles[id]->compute(IN_RGB_F32, p1, p2, s1, s2, mp, ms, norms);
+    } catch (const VshipErr& e) {
+        err = vs_error_text(e);
+    } catch (...) {
+        err = vs_error_text(E_BadErrorType, __FILE__, __LINE__);
+    }
+    d->pool.give(id);
+    api->call<vs4::freeFrame>(f1);
+    api->call<vs4::freeFrame>(f2);
+    if (!err.empty()) {
+        api->call<vs4::freeFrame>((const vs4::Frame*)dst);
+        api->call<vs4::setFilterError>(err.c_str(), ctx);
+        return nullptr;
+    }
+    vs4::Map* props = api->call<vs4::getFramePropertiesRW>(dst);
+    api->call<vs4::mapSetFloat>(props, "_BUTTERAUGLI_QNorm", norms[0], (int)vs4::maReplace);
+    api->call<vs4::mapSetFloat>(props, "_BUTTERAUGLI_3Norm", norms[1], (int)vs4::maReplace);
+    api->call<vs4::mapSetFloat>(props, "_BUTTERAUGLI_INFNorm", norms[2], (int)vs4::maReplace);
+    return dst;
+}
+
+void butter_free(void* inst, vs4::Core*, const Api* api) {
+    ButterFilter* d = static_cast<ButterFilter*>(inst);
+    api->call<vs4::freeNode>(d->reference);
+    api->call<vs4::freeNode>(d->distorted);
+    delete d;
+}
+
+void butter_create(const vs4::Map* in, vs4::Map* out, void*, vs4::Core* core, const Api* api) {
+    std::string err;
+    vs4::Node* ref = to_rgbs(api->call<vs4::mapGetNode>(in, "reference", 0, (int*)nullptr), core, api, &err);
+    vs4::Node* dis = ref ? to_rgbs(api->call<vs4::mapGetNode>(in, "distorted", 0, (int*)nullptr), core, api, &err)
+                         : nullptr;
+    if (!ref || !dis) {
+        if (ref) api->call<vs4::freeNode>(ref);
+        api->call<vs4::mapSetError>(out, err.c_str());
+        return;
+    }
+    auto fail = [&](const std::string& msg) {
+        api->call<vs4::freeNode>(ref);
+        api->call<vs4::freeNode>(dis);
+        api->call<vs4::mapSetError>(out, msg.c_str());
+    };
+    const vs4::VideoInfo* vi1 = api->call<vs4::getVideoInfo>(ref);
+    const vs4::VideoInfo* vi2 = api->call<vs4::getVideoInfo>(dis);
+    if (!same_video_info(vi1, vi2)) return fail(vs_error_text(E_DifferingInputType, __FILE__, __LINE__));
+    if (vi1->format.bitsPerSample != 32 || vi1->format.colorFamily != vs4::cfRGB ||
+        vi1->format.sampleType != vs4::stFloat)
+        return fail(vs_error_text(E_NonRGBSInput, __FILE__, __LINE__));
+    int e = 0;
+    double intensity = api->call<vs4::mapGetFloat>(in, "intensity_multiplier", 0, &e);
+    if (e != vs4::peSuccess) intensity = 203.0;
+    int64_t gpu = api->call<vs4::mapGetInt>(in, "gpu_id", 0, &e);
+    if (e != vs4::peSuccess) gpu = 0;
+    int64_t distmap = api->call<vs4::mapGetInt>(in, "distmap", 0, &e);
+    if (e != vs4::peSuccess) distmap = 0;
+    int64_t qnorm = api->call<vs4::mapGetInt>(in, "qnorm", 0, &e);
+    if (e != vs4::peSuccess) qnorm = 2;
+    vs4::CoreInfo info{};
+    api->call<vs4::getCoreInfo>(core, &info);
+    int64_t streams = api->call<vs4::mapGetInt>(in, "numStream", 0, &e);
+    if (e != vs4::peSuccess) streams = 4;
+    streams = std::max<int64_t>(1, std::min<int64_t>(streams, std::max(1, info.numThreads)));
+    auto d = std::make_unique<ButterFilter>();
+    d->distmap = distmap != 0;
+    vs4::VideoInfo viout = *vi1;
+    if (d->distmap) {
+        api->call<vs4::queryVideoFormat>(&d->gray, (int)vs4::cfGray, (int)vs4::stFloat, 32, 0, 0, core);
+        viout.format = d->gray;
+    }
+    try {
+        gpu_full_check((int)gpu);  // also makes gpu the current device
+        for (int i = 0; i < (int)streams; i++) {
+            d->pool.handles.push_back(
+                std::make_unique<ButterHandle>(vi1->width, vi1->height, (int)qnorm, (float)intensity, (int)gpu));
+            d->pool.free_ids.push_back(i);
+        }
+    } catch (const VshipErr& x) {
+        return fail(vs_error_text(x));
+    } catch (...) {
+        return fail(vs_error_text(E_OutOfRAM, __FILE__, __LINE__));
+    }
+    d->reference = ref;
+    d->distorted = dis;
+    const vs4::FilterDependency deps[2] = {{ref, vs4::rpStrictSpatial}, {dis, vs4::rpStrictSpatial}};
+    api->call<vs4::createVideoFilter>(out, "vship", (const vs4::VideoInfo*)&viout, &butter_get_frame, &butter_free,
+                                      (int)vs4::fmParallel, deps, 2, (void*)d.release(), core);
+}
+
 // Device description (src/VshipLib.cpp:8-56): every device's name when no
 // gpu_id is given, else the properties of that device.
 void gpu_info(const vs4::Map* in, vs4::Map* out, void*, vs4::Core*, const Api* api) {
@@ -264,11 +399,12 @@ void gpu_info(const vs4::Map* in, vs4::Map* out, void*, vs4::Core*, const Api* a
                                (int)vs4::maReplace);
 }
 
-// The reference's other metrics (src/VshipLib.cpp:61-62) are not built here;
-// they stay registered so a script calling them gets this error, not a
+// The reference's CVVDP metric (src/VshipLib.cpp:62) is not built here; it
+// stays registered so a script calling it gets this error, not a
 // missing-function one.
 void metric_not_built(const vs4::Map*, vs4::Map* out, void*, vs4::Core*, const Api* api) {
-    const std::string msg = "VshipException\nBadErrorType: this vship build (MI355X) provides SSIMULACRA2 only\n";
+    const std::string msg =
+        "VshipException\nBadErrorType: this vship build (MI355X) provides SSIMULACRA2 and BUTTERAUGLI only\n";
     api->call<vs4::mapSetError>(out, msg.c_str());
 }
 
@@ -285,7 +421,7 @@ VSHIP_PLUGIN_EXPORT void VapourSynthPluginInit2(vs4::Plugin* plugin, const vs4::
     vspapi->registerFunction("BUTTERAUGLI",
                              "reference:vnode;distorted:vnode;qnorm:int:opt;intensity_multiplier:float:opt;"
                              "distmap:int:opt;numStream:int:opt;gpu_id:int:opt;",
-                             "clip:vnode;", &vship::metric_not_built, nullptr, plugin);
+                             "clip:vnode;", &vship::butter_create, nullptr, plugin);
     vspapi->registerFunction("CVVDP", "reference:vnode;distorted:vnode;display:data:opt;numStream:int:opt;gpu_id:int:opt;",
                              "clip:vnode;", &vship::metric_not_built, nullptr, plugin);
     vspapi->registerFunction("GpuInfo", "gpu_id:int:opt;", "gpu_human_data:data;", &vship::gpu_info, nullptr, plugin);

@@ -12,6 +12,7 @@
 #include <vector>
 
 #include "../../include/VshipAPI.h"
+#include "../../include/VshipButteraugliBatch.h"
 #include "../../include/VshipSSIMU2Batch.h"
 #include "api_internal.hpp"
 #include "engine.hpp"
@@ -152,6 +153,63 @@ double FrameHandle::compute(int fmt, const uint8_t* const p1[3], const uint8_t* 
     return score;
 }
 
+// ---------------------------------------------------------------------------
+// Butteraugli handles (Vship_ButteraugliHandler)
+// ---------------------------------------------------------------------------
+ButterHandle::ButterHandle(int w, int h, int qnorm, float intensity, int dev) : width(w), height(h), device(dev) {
+    engine = std::make_unique<bt::Engine>(w, h, 1, qnorm, intensity, dev);
+    HIP_OR_THROW(hipMalloc(reinterpret_cast<void**>(&map), sizeof(float) * (size_t)w * h), E_OutOfVRAM);
+}
+
+ButterHandle::~ButterHandle() {
+    if (staging) (void)hipFree(staging);
+    if (map) (void)hipFree(map);
+}
+
+void ButterHandle::compute(int fmt, const uint8_t* const p1[3], const uint8_t* const p2[3], int64_t s1, int64_t s2,
+                           float* dst, int64_t dst_stride, double norms[3]) {
+    if (!p1 || !p2) VSHIP_THROW(E_BadPointer);
+    for (int i = 0; i < 3; i++)
+        if (!p1[i] || !p2[i]) VSHIP_THROW(E_BadPointer);
+    const int64_t row = (int64_t)width * (fmt == IN_RGB_F32 ? 4 : 2);
+    if (s1 < row || s2 < row) VSHIP_THROW(E_BadPointer);
+    if (dst && dst_stride < (int64_t)width * 4) VSHIP_THROW(E_BadPointer);
+    // each plane is one contiguous block of (height - 1) strides plus a row
+    const size_t blk1 = (size_t)s1 * (height - 1) + row, blk2 = (size_t)s2 * (height - 1) + row;
+    const size_t a1 = (blk1 + 255) / 256 * 256, a2 = (blk2 + 255) / 256 * 256;
+    const size_t need = 3 * (a1 + a2);
+    HIP_OR_THROW(hipSetDevice(device), E_BadDeviceArgument);
+    if (need > staging_bytes) {
+        if (staging) (void)hipFree(staging);
+        staging = nullptr;
+        staging_bytes = 0;
+        HIP_OR_THROW(hipMalloc(reinterpret_cast<void**>(&staging), need), E_OutOfVRAM);
+        staging_bytes = need;
+    }
+    hipStream_t st = engine->stream();
+    bt::PairDesc d{};
+    for (int im = 0; im < 2; im++) {
+        const uint8_t* const* src = im == 0 ? p1 : p2;
+        const int64_t sp = im == 0 ? s1 : s2;
+        const size_t blk = im == 0 ? blk1 : blk2;
+        for (int pl = 0; pl < 3; pl++) {
+            uint8_t* dstp = staging + (im == 0 ? pl * a1 : 3 * a1 + pl * a2);
+            HIP_OR_THROW(hipMemcpyAsync(dstp, src[pl], blk, hipMemcpyHostToDevice, st), E_BadPointer);
+            d.src[im][pl] = dstp;
+            d.stride[im][pl] = sp;
+        }
+    }
+    d.map = dst ? map : nullptr;
+    d.map_stride = (int64_t)width * 4;
+    engine->run(fmt, &d, 1, norms);
+    if (dst) {
+        HIP_OR_THROW(hipMemcpy2DAsync(dst, dst_stride, map, (size_t)width * 4, (size_t)width * 4, height,
+                                      hipMemcpyDeviceToHost, st),
+                     E_BadPointer);
+        HIP_OR_THROW(hipStreamSynchronize(st), E_BadPointer);
+    }
+}
+
 template <typename T>
 struct Table {
     std::mutex lock;
@@ -195,6 +253,21 @@ struct Table {
 
 Table<FrameHandle>& frame_table() {
     static Table<FrameHandle> t;
+    return t;
+}
+
+Table<ButterHandle>& butter_table() {
+    static Table<ButterHandle> t;
+    return t;
+}
+
+struct ButterBatchHandle {
+    std::unique_ptr<bt::Engine> engine;
+    std::atomic<bool> busy{false};
+};
+
+Table<ButterBatchHandle>& butter_batch_table() {
+    static Table<ButterBatchHandle> t;
     return t;
 }
 
@@ -245,6 +318,29 @@ Vship_Exception compute_frame(Vship_SSIMU2Handler handler, double* score, const 
     double r = 0.0;
     const Vship_Exception e = guarded([&] { r = h->compute(fmt, srcp1, srcp2, s1, s2); });
     if (e == Vship_NoError) *score = r;
+    return e;
+}
+
+// Vship_ComputeButteraugli*: the norms are stored into the score's int
+// fields by plain conversion, truncating, as the reference does
+// (src/VshipLib.cpp:308-310, where a float tuple is assigned to them).
+Vship_Exception compute_butter(Vship_ButteraugliHandler handler, Vship_ButteraugliScore* score, const uint8_t* dstp,
+                               int64_t dststride, const uint8_t* srcp1[3], const uint8_t* srcp2[3], int64_t s1,
+                               int64_t s2, int fmt) {
+    std::shared_ptr<ButterHandle> h = butter_table().get(handler.id);
+    if (!h || !h->engine) return Vship_BadHandler;
+    if (!score) return Vship_BadPointer;
+    BusyGuard g(h->busy);
+    if (!g.ok) return Vship_BadHandler;
+    double n[3] = {0.0, 0.0, 0.0};
+    const Vship_Exception e = guarded([&] {
+        h->compute(fmt, srcp1, srcp2, s1, s2, reinterpret_cast<float*>(const_cast<uint8_t*>(dstp)), dststride, n);
+    });
+    if (e == Vship_NoError) {
+        score->norm2 = (int)(float)n[0];
+        score->norm3 = (int)(float)n[1];
+        score->norminf = (int)(float)n[2];
+    }
     return e;
 }
 
@@ -369,27 +465,129 @@ Vship_Exception Vship_ComputeSSIMU2Uint16v2(Vship_SSIMU2Handler handler, double*
     return compute_frame(handler, score, srcp1, srcp2, stride, stride2, IN_RGB_U16);
 }
 
-// Butteraugli is outside this build (see VshipAPI.h).
-Vship_Exception Vship_ButteraugliInit(Vship_ButteraugliHandler*, int, int, float) { return Vship_BadErrorType; }
-Vship_Exception Vship_ButteraugliInitv2(Vship_ButteraugliHandler*, int, int, int, float) {
-    return Vship_BadErrorType;
+Vship_Exception Vship_ButteraugliInit(Vship_ButteraugliHandler* handler, int width, int height,
+                                      float intensity_multiplier) {
+    return Vship_ButteraugliInitv2(handler, width, height, 2, intensity_multiplier);
 }
-Vship_Exception Vship_ButteraugliFree(Vship_ButteraugliHandler) { return Vship_BadHandler; }
-Vship_Exception Vship_ComputeButteraugliFloat(Vship_ButteraugliHandler, Vship_ButteraugliScore*, const uint8_t*,
-                                              int64_t, const uint8_t**, const uint8_t**, int64_t) {
-    return Vship_BadHandler;
+
+Vship_Exception Vship_ButteraugliInitv2(Vship_ButteraugliHandler* handler, int width, int height, int Qnorm,
+                                        float intensity_multiplier) {
+    if (!handler) return Vship_BadPointer;
+    // the slot is taken even when initialisation fails, as in the reference
+    // (src/VshipLib.cpp:267-280); Vship_ButteraugliFree releases it
+    handler->id = butter_table().allocate();
+    if (width <= 0 || height <= 0) return Vship_OutOfVRAM;
+    return guarded([&] {
+        int dev = 0;
+        if (hipGetDevice(&dev) != hipSuccess) VSHIP_THROW(E_DeviceCountError);
+        butter_table().set(handler->id,
+                           std::make_shared<ButterHandle>(width, height, Qnorm, intensity_multiplier, dev));
+    });
 }
-Vship_Exception Vship_ComputeButteraugliFloatv2(Vship_ButteraugliHandler, Vship_ButteraugliScore*, const uint8_t*,
-                                                int64_t, const uint8_t**, const uint8_t**, int64_t, int64_t) {
-    return Vship_BadHandler;
+
+Vship_Exception Vship_ButteraugliFree(Vship_ButteraugliHandler handler) {
+    return butter_table().release(handler.id) ? Vship_NoError : Vship_BadHandler;
 }
-Vship_Exception Vship_ComputeButteraugliUint16(Vship_ButteraugliHandler, Vship_ButteraugliScore*, const uint8_t*,
-                                               int64_t, const uint8_t**, const uint8_t**, int64_t) {
-    return Vship_BadHandler;
+
+Vship_Exception Vship_ComputeButteraugliFloat(Vship_ButteraugliHandler handler, Vship_ButteraugliScore* score,
+                                              const uint8_t* dstp, int64_t dststride, const uint8_t* srcp1[3],
+                                              const uint8_t* srcp2[3], int64_t stride) {
+    return compute_butter(handler, score, dstp, dststride, srcp1, srcp2, stride, stride, IN_RGB_F32);
 }
-Vship_Exception Vship_ComputeButteraugliUint16v2(Vship_ButteraugliHandler, Vship_ButteraugliScore*, const uint8_t*,
-                                                 int64_t, const uint8_t**, const uint8_t**, int64_t, int64_t) {
-    return Vship_BadHandler;
+Vship_Exception Vship_ComputeButteraugliFloatv2(Vship_ButteraugliHandler handler, Vship_ButteraugliScore* score,
+                                                const uint8_t* dstp, int64_t dststride, const uint8_t* srcp1[3],
+                                                const uint8_t* srcp2[3], int64_t stride, int64_t stride2) {
+    return compute_butter(handler, score, dstp, dststride, srcp1, srcp2, stride, stride2, IN_RGB_F32);
+}
+Vship_Exception Vship_ComputeButteraugliUint16(Vship_ButteraugliHandler handler, Vship_ButteraugliScore* score,
+                                               const uint8_t* dstp, int64_t dststride, const uint8_t* srcp1[3],
+                                               const uint8_t* srcp2[3], int64_t stride) {
+    return compute_butter(handler, score, dstp, dststride, srcp1, srcp2, stride, stride, IN_RGB_U16);
+}
+Vship_Exception Vship_ComputeButteraugliUint16v2(Vship_ButteraugliHandler handler, Vship_ButteraugliScore* score,
+                                                 const uint8_t* dstp, int64_t dststride, const uint8_t* srcp1[3],
+                                                 const uint8_t* srcp2[3], int64_t stride, int64_t stride2) {
+    return compute_butter(handler, score, dstp, dststride, srcp1, srcp2, stride, stride2, IN_RGB_U16);
+}
+
+// ---------------------------------------------------------------------------
+// batched Butteraugli (VshipButteraugliBatch.h)
+// ---------------------------------------------------------------------------
+uint64_t Vship_ButteraugliPairFootprint(int width, int height) {
+    if (width <= 0 || height <= 0) return 0;
+    return bt::Engine::pair_bytes(width, height);
+}
+
+Vship_Exception Vship_ButteraugliBatchInit(Vship_ButteraugliBatchHandler* handler,
+                                           const Vship_ButteraugliBatchConfig* cfg) {
+    if (!handler || !cfg) return Vship_BadPointer;
+    if (cfg->max_pairs < 1 || cfg->max_pairs > 4096 || cfg->width < 1 || cfg->height < 1) return Vship_BadPointer;
+    const int id = butter_batch_table().allocate();
+    handler->id = id;
+    const Vship_Exception e = guarded([&] {
+        int dev = 0;
+        if (hipGetDevice(&dev) != hipSuccess) VSHIP_THROW(E_DeviceCountError);
+        auto b = std::make_shared<ButterBatchHandle>();
+        b->engine = std::make_unique<bt::Engine>(cfg->width, cfg->height, cfg->max_pairs, cfg->qnorm,
+                                                 cfg->intensity_multiplier, dev);
+        b->engine->set_profile(cfg->profile != 0);
+        butter_batch_table().set(id, std::move(b));
+    });
+    if (e != Vship_NoError) butter_batch_table().release(id);
+    return e;
+}
+
+Vship_Exception Vship_ButteraugliBatchFree(Vship_ButteraugliBatchHandler handler) {
+    return butter_batch_table().release(handler.id) ? Vship_NoError : Vship_BadHandler;
+}
+
+Vship_Exception Vship_ButteraugliBatchCompute(Vship_ButteraugliBatchHandler handler, int format,
+                                              const Vship_ButteraugliPair* pairs, int npairs, double* norms) {
+    std::shared_ptr<ButterBatchHandle> b = butter_batch_table().get(handler.id);
+    if (!b) return Vship_BadHandler;
+    if (!pairs || !norms || npairs < 1) return Vship_BadPointer;
+    if (format < 0 || format >= IN_NUM_FORMATS) return Vship_BadPointer;
+    BusyGuard g(b->busy);
+    if (!g.ok) return Vship_BadHandler;
+    return guarded([&] {
+        std::vector<bt::PairDesc> d(npairs);
+        for (int i = 0; i < npairs; i++) {
+            for (int c = 0; c < 3; c++) {
+                d[i].src[0][c] = pairs[i].ref[c];
+                d[i].src[1][c] = pairs[i].dist[c];
+                d[i].stride[0][c] = pairs[i].ref_stride[c];
+                d[i].stride[1][c] = pairs[i].dist_stride[c];
+            }
+            d[i].map = pairs[i].distmap;
+            d[i].map_stride = pairs[i].distmap_stride;
+            d[i].bits = fmt_yuv(format) ? pairs[i].bits : 8;
+            d[i].cloc = pairs[i].chroma_location;
+            matrix_coefs(pairs[i].matrix, d[i].kr, d[i].kb);
+        }
+        b->engine->run(format, d.data(), npairs, norms);
+    });
+}
+
+Vship_Exception Vship_ButteraugliBatchGetStats(Vship_ButteraugliBatchHandler handler,
+                                               Vship_ButteraugliBatchStats* st) {
+    std::shared_ptr<ButterBatchHandle> b = butter_batch_table().get(handler.id);
+    if (!b) return Vship_BadHandler;
+    if (!st) return Vship_BadPointer;
+    std::memset(st, 0, sizeof(*st));
+    const bt::Engine& e = *b->engine;
+    float ms[bt::ST_COUNT] = {};
+    st->timed = e.stage_ms(ms) ? 1 : 0;
+    st->input_ms = ms[bt::ST_INPUT];
+    st->opsin_ms = ms[bt::ST_OPSIN];
+    st->low_ms = ms[bt::ST_LOW];
+    st->mid_ms = ms[bt::ST_MID];
+    st->high_ms = ms[bt::ST_HIGH];
+    st->diff_ms = ms[bt::ST_DIFF];
+    st->mask_ms = ms[bt::ST_MASK];
+    st->norm_ms = ms[bt::ST_NORM];
+    st->npairs = e.last_npairs();
+    for (int s = 0; s < bt::ST_COUNT; s++) st->stage_bytes[s] = e.stage_bytes(s) * (uint64_t)e.last_npairs();
+    return Vship_NoError;
 }
 
 // ---------------------------------------------------------------------------

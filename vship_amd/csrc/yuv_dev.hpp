@@ -1,0 +1,82 @@
+// Device-side YUV sampling shared by the SSIMULACRA2 prep kernel and the
+// Butteraugli input kernel: sample reads with the declared depth's clamp, and
+// Catmull-Rom chroma upsampling at the sample positions of every layout and
+// chroma location (src/ffvship_utility/gpuColorToLinear/chromaUpsample.hpp).
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include "kernel_util.hpp"
+#include "ssimu2_dev.hpp"
+
+namespace vship {
+
+// One YUV sample; 16-bit-container samples above the declared depth's
+// largest code are clamped to it.
+__device__ __forceinline__ float sample_u(const uint8_t* plane, int64_t stride, int x, int y, bool wide,
+                                          int maxcode) {
+    if (wide) return (float)min((int)as_global(reinterpret_cast<const uint16_t*>(plane + (int64_t)y * stride))[x], maxcode);
+    return (float)as_global(plane)[(int64_t)y * stride + x];
+}
+
+// Catmull-Rom weights of samples c-1 .. c+2 at fraction t of [c, c+1]: the
+// cubic Hermite interpolant with central-difference tangents
+// (src/ffvship_utility/gpuColorToLinear/chromaUpsample.hpp:5-24). Exact in
+// FP32 for the fractions k/8 that occur here.
+__device__ __forceinline__ void cr_weights(float t, float w[4]) {
+    w[0] = 0.5f * t * ((2.0f - t) * t - 1.0f);
+    w[1] = 0.5f * ((3.0f * t - 5.0f) * t * t + 2.0f);
+    w[2] = 0.5f * t * ((4.0f - 3.0f * t) * t + 1.0f);
+    w[3] = 0.5f * t * t * (t - 1.0f);
+}
+
+// Chroma interval c and fraction t of luma position x under subsampling 2^s:
+// chroma sample i sits at luma position 2^s i (co-sited) or
+// 2^s i + (2^s - 1) / 2 (centred), so x maps to u = x / 2^s or
+// (2x + 1 - 2^s) / 2^(s+1), c = floor(u), t = u - c
+// (chromaUpsample.hpp:44-128: left/top t = 0, 1/2 or k/4; centre 1/4, 3/4 or
+// 1/8, 3/8, 5/8, 7/8).
+__device__ __forceinline__ void chroma_pos(int x, int s, bool centred, int& c, float& t) {
+    if (!centred) {
+        c = x >> s;
+        t = (float)(x & ((1 << s) - 1)) / (float)(1 << s);
+    } else {
+        const int n = 2 * x + 1 - (1 << s);
+        c = n >> (s + 1);  // floor, n may be negative
+        t = (float)(n & ((2 << s) - 1)) / (float)(2 << s);
+    }
+}
+
+// One row of chroma at interval cx: the sample itself without horizontal
+// subsampling, else the four-tap horizontal interpolation (indices clamped).
+template <bool WIDE>
+__device__ __forceinline__ float chroma_row(const uint8_t* plane, int64_t stride, int cw, int y, int cx,
+                                            const float wx[4], bool direct, int maxcode) {
+    if (direct) return sample_u(plane, stride, cx, y, WIDE, maxcode);
+    float a = 0.f;
+#pragma unroll
+    for (int b = 0; b < 4; b++) a = fmaf(wx[b], sample_u(plane, stride, min(max(cx - 1 + b, 0), cw - 1), y, WIDE, maxcode), a);
+    return a;
+}
+
+// Chroma sample value at luma pixel (x, y) for any layout and chroma location:
+// horizontal interpolation of up to four rows, then vertical
+// (chromaUpsample.hpp:131-205 order).
+template <bool WIDE>
+__device__ __forceinline__ float chroma_at(const uint8_t* plane, int64_t stride, int cw, int chh, int x, int y,
+                                           int csw, int csh, bool hc, bool vc, int maxcode) {
+    int cx, cy;
+    float tx, ty, wx[4], wy[4];
+    chroma_pos(x, csw, hc, cx, tx);
+    chroma_pos(y, csh, vc, cy, ty);
+    cr_weights(tx, wx);
+    cr_weights(ty, wy);
+    if (csh == 0) return chroma_row<WIDE>(plane, stride, cw, cy, cx, wx, csw == 0, maxcode);
+    float a = 0.f;
+#pragma unroll
+    for (int r = 0; r < 4; r++)
+        a = fmaf(wy[r], chroma_row<WIDE>(plane, stride, cw, min(max(cy - 1 + r, 0), chh - 1), cx, wx, csw == 0, maxcode),
+                 a);
+    return a;
+}
+
+}  // namespace vship

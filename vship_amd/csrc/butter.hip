@@ -190,6 +190,14 @@ __device__ __forceinline__ float mask_dc_y(float d) {
     return r * r;
 }
 
+// a * b rounded to FP32 on its own: never contracted into a following add
+// (where it would be kept exact), so that two values computed alike compare
+// equal however they are used
+__device__ __forceinline__ float mul_rounded(float a, float b) {
+#pragma clang fp contract(off)
+    return a * b;
+}
+
 __device__ __forceinline__ void store_min3(float v, float& m0, float& m1, float& m2) {
     if (v < m2) {
         if (v < m0) {
@@ -760,7 +768,7 @@ __global__ __launch_bounds__(kThreads) void butter_mask_kernel(Scale S, const Ta
     __syncthreads();
     for (int i = threadIdx.x; i < kMB * kMB; i += kThreads) {
         const int r = i / kMB, c = i - r * kMB;
-        L.b0[i] = vsum<R, kMB>(L.mid, T, r, c) * L.rowf[r];
+        L.b0[i] = mul_rounded(vsum<R, kMB>(L.mid, T, r, c), L.rowf[r]);
     }
     __syncthreads();
     // blurred distorted mask on the 32 x 32 tile
@@ -777,7 +785,10 @@ __global__ __launch_bounds__(kThreads) void butter_mask_kernel(Scale S, const Ta
     for (int j = 0; j < kMRows; j++) {
         const int r = ty + 8 * j;
         const int gy = oy0 + r;
-        const float b1 = vsum<R, kMT>(L.mid, T, r, tx) * L.rowf[r + kMErode];
+        // rounded like b0 (stored in LDS): a contraction of the product into
+        // b0 - b1 would make identical masks differ near the top and bottom
+        // edges, where the row factor is not 1
+        const float b1 = mul_rounded(vsum<R, kMT>(L.mid, T, r, tx), L.rowf[r + kMErode]);
         if (gx >= w || gy >= h) continue;
         const float* bc = L.b0 + (r + kMErode) * kMB + tx + kMErode;
         const float b0 = bc[0];

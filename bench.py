@@ -16,6 +16,10 @@ blocks; vship_amd/synth/synth.c holds the generator):
   served  Poisson clip arrivals (200 clips/s), 8..64 pairs per clip at the
           mixed resolutions, 8/10-bit 50/50, continuous-batching server under a
           200 GB budget; frames/s and p50/p99 clip latency.
+  butter1080p  Butteraugli (not SSIMULACRA2): 16 pairs 1920x1080 YUV420P8
+          BT.709 limited per step, resident in HBM; norms Q=2, 3, inf at
+          203 nits; stage times, the dominant kernel's roofline position and
+          the FP64 CPU oracle as baseline.
 
 Prints one JSON line. Besides throughput: the dominant kernel's roofline
 position (HIP events on the library's stream), the per-frame path, p50/p99
@@ -35,7 +39,7 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0
-CONFIGS = ["1080p", "rgb256", "4k10", "mixed", "served"]
+CONFIGS = ["1080p", "rgb256", "4k10", "mixed", "served", "butter1080p"]
 
 
 def parse():
@@ -321,6 +325,97 @@ def run_served(a):
     return out, host_pool, alone
 
 
+BUTTER_KERNELS = {"input": "butter_input_kernel<4>", "opsin": "butter_opsin_kernel", "low": "butter_low_kernel",
+                  "mid": "butter_mid_kernel", "high": "butter_high_kernel", "diff": "butter_diff_kernel",
+                  "mask": "butter_mask_kernel", "norm": "butter_norm_kernel"}
+BUTTER_BYTES = {"input": "Y + 2 quarter-size chroma bytes read per image, 6 linear FP32 planes written at "
+                         "full and quarter size",
+                "opsin": "6 FP32 planes read, 6 written, per pixel of both scales",
+                "low": "6 FP32 planes read, 12 written (low band as values, residual)",
+                "mid": "6 read, 10 written (middle band, high residual)",
+                "high": "4 read, 8 written (high, ultra-high bands)",
+                "diff": "20 read (bands of both images), 4 written (AC, DC sums, 2 mask inputs)",
+                "mask": "4 read, 1 written, plus the half-scale map read at full scale",
+                "norm": "tile partials"}
+
+
+def run_butter(a):
+    """Butteraugli: 16 1920x1080 YUV420P8 pairs per step, device-resident."""
+    from vship_amd import api
+
+    w, h, npairs = 1920, 1080, 16
+    pairs = video_pairs(w, h, 8, npairs, seed=4)
+    dp = api.ButterDevicePairs(pairs, fmt=api.PF_YUV420_U8, bits=8, matrix=api.MATRIX_BT709)
+    print(f"# butteraugli: {npairs} pairs {w}x{h} YUV420P8 on device, "
+          f"{api.butter_pair_footprint(w, h) * npairs / 1e9:.1f} GB of work planes", flush=True)
+    batch = api.ButteraugliBatch(w, h, npairs)
+    norms = None
+    for _ in range(a.warmup):
+        norms = batch.compute(dp)
+    step_ms = []
+    t0 = time.perf_counter()
+    for _ in range(a.steps):
+        t1 = time.perf_counter()
+        norms = batch.compute(dp)
+        step_ms.append((time.perf_counter() - t1) * 1e3)
+    elapsed = time.perf_counter() - t0
+    batch.close()
+    # stage times: the same batch launched stage by stage between HIP events
+    prof = api.ButteraugliBatch(w, h, npairs, profile=True)
+    stages = {k: [] for k in api.BUTTER_STAGES}
+    for _ in range(5):
+        pn = prof.compute(dp)
+        st = prof.stats()
+        for k in stages:
+            stages[k].append(st[k + "_ms"])
+    if not np.array_equal(pn, norms):
+        raise RuntimeError("stage-ordered launches give other norms than the fused order")
+    stage_ms = {k: float(np.mean(v[1:])) for k, v in stages.items()}
+    stage_bytes = st["stage_bytes"]
+    prof.close()
+    value = npairs * a.steps / elapsed
+    out = record(a, value, elapsed, a.steps, "f32", "butteraugli 1920x1080 YUV420P8 BT.709 limited, 16 pairs per "
+                 "step, inputs resident in HBM, qnorm 2, 203 nits", {"pairs_per_step": npairs})
+    out["metric"] = "Butteraugli frame-pairs/s"
+    out["latency_ms"] = {"p50": round(float(np.percentile(step_ms, 50)), 3),
+                         "p99": round(float(np.percentile(step_ms, 99)), 3), "per": f"step of {npairs} pairs"}
+    out["kernels_ms"] = {k: round(v, 3) for k, v in stage_ms.items()}
+    top = max((k for k in stage_ms if k != "norm"), key=lambda k: stage_ms[k])
+    achieved = stage_bytes[top] / (stage_ms[top] * 1e-3) / 1e9
+    kernel = BUTTER_KERNELS[top]
+    traffic = measured_traffic(a.config, kernel)
+    out["roofline"] = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                       "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic,
+                       "traffic_source": ("profiles/hbm_traffic.json: rocprofv3 FETCH_SIZE x 2.0 + WRITE_SIZE x 1.0 "
+                                          "per launch" if traffic is not None
+                                          else "no PMC run of this kernel on this workload"),
+                       "kernel": f"{kernel} (both scales, {top} stage)",
+                       "algorithmic_bytes_per_launch": int(stage_bytes[top]),
+                       "note": BUTTER_BYTES[top] + "; FP32 stencil sums on the vector ALUs"}
+    out["stage_bytes"] = {k: int(v) for k, v in stage_bytes.items()}
+    return out, pairs, norms
+
+
+def butter_cpu_baseline(pairs, norms, a):
+    """The FP64 C oracle (single thread) on the first pairs within the budget,
+    and the GPU norms of those pairs against it."""
+    from tests import butter_lib as bl
+    from tests import oracle_lib as ol
+
+    n, rel, t0 = 0, [], time.perf_counter()
+    while n < len(pairs) and (n == 0 or time.perf_counter() - t0 < a.cpu_seconds):
+        r, d = pairs[n]
+        rgb = [ol.yuv_to_rgb_any(im, 8, ol.SO_MAT_BT709, 1, 1, 0).astype(np.float32) for im in (r, d)]
+        want = bl.butteraugli(list(rgb[0]), list(rgb[1]))
+        rel.append(float(np.max(np.abs(norms[n] - want) / want)))
+        n += 1
+    rate = n / (time.perf_counter() - t0)
+    base = {"value": round(rate, 4), "unit": "frame-pairs/s", "cores": 1, "kind": "port",
+            "sample": f"first {n} pair(s) of the workload through the FP64 C oracle (YUV to R'G'B' and the "
+                      f"metric), single thread, {a.cpu_seconds:.0f}s budget"}
+    return base, {"pairs": list(range(n)), "max_rel_norm_diff": max(rel)}
+
+
 def cpu_baseline(pairs, bits, gpu_scores, a):
     """FP32 CPU restatement timed on the host cores (bounded sample) and the FP64
     oracle on a few pairs for |dscore|. bits: 0 for RGB8, else per-pair list or int."""
@@ -390,6 +485,12 @@ def main():
         out, scores = run_batched(a, pairs, 8, False, "ssimulacra2 uneven batch of 64 pairs, 640x360..3840x2160 "
                                   "YUV420P8 BT.709 limited, inputs resident in HBM", tune=True, graph=True)
         out["config"]["resolutions"] = {f"{w}x{h}": res.count((w, h)) for (w, h) in sorted(set(res))}
+    elif a.config == "butter1080p":
+        out, pairs, norms = run_butter(a)
+        if not a.no_cpu:
+            out["cpu_baseline"], out["norms_vs_fp64_oracle"] = butter_cpu_baseline(pairs, norms, a)
+        print(json.dumps(out), flush=True)
+        return
     else:
         out, host_pool, alone = run_served(a)
         pairs, scores, bits = [], [], []

@@ -318,39 +318,106 @@ void launch_input_fmt(int fmt, dim3 g, hipStream_t st, const PairDesc* pairs, co
 
 // ---------------------------------------------------------------------------
 // Blur tile kernels: 64 x 32 outputs, 256 threads; thread (tx, ty) owns
-// column tx and rows ty + 4 j, j < 8.
+// column tx and the kRows consecutive rows kRows * ty + j.
+//
+// Both passes are register-blocked so that LDS reads do not bound them:
+// the row pass gives each job a run of 8 outputs of one staged row, read as
+// 16-byte vectors (rows padded to a multiple of 4 floats), and the column
+// pass gives each thread 8 outputs of one column from 8 + 2R reads. Every
+// output is the same sequence of FMAs over the taps as a plain convolution.
 // ---------------------------------------------------------------------------
 constexpr int kTW = 64, kTH = 32, kRows = kTH / (kThreads / kTW);
+constexpr int kRun = 8;  // outputs per row-pass job
+static_assert(kRows == kRun, "the column pass computes kRun rows per thread");
 
 template <int R>
-struct BlurLds {
+struct alignas(16) BlurLds {
     static constexpr int SW = kTW + 2 * R, SH = kTH + 2 * R;
-    float in[SH * SW];
+    static constexpr int SWP = (SW + 3) / 4 * 4;  // padded row
+    float in[SH * SWP];
     float mid[SH * kTW];
     float colf[kTW];
     float rowf[kTH];
 };
 
-// Blur plane src of the tile into out[j] (rows ty + 4 j); ctr[j] receives the
-// unblurred value at the same pixel. Ends with the LDS free for reuse only
-// after the caller's next __syncthreads.
+// in[r][c] = src(gx0 + c, gy0 + r) for c < SW, 0 outside the image and in
+// the row padding
+template <int SW, int SWP, int SH>
+__device__ __forceinline__ void stage_padded(float* __restrict__ in, const float* __restrict__ src, int w, int h,
+                                             int gx0, int gy0) {
+    for (int i = threadIdx.x; i < SWP * SH; i += kThreads) {
+        const int r = i / SWP, c = i - r * SWP;
+        const int gx = gx0 + c, gy = gy0 + r;
+        float v = 0.0f;
+        if (c < SW && gx >= 0 && gx < w && gy >= 0 && gy < h) v = src[(int64_t)gy * w + gx];
+        in[i] = v;
+    }
+}
+
+// row pass: mid[r][c] = colf[c] * sum_k g[k] in[r][c + k]
+template <int R>
+__device__ __forceinline__ void hpass_runs(const BlurLds<R>& L, float* __restrict__ mid, const Taps* __restrict__ T) {
+    constexpr int SWP = BlurLds<R>::SWP, SH = BlurLds<R>::SH, RUNS = kTW / kRun;
+    constexpr int NV = (kRun + 2 * R + 3) / 4;  // float4 loads per run
+    static_assert((RUNS - 1) * kRun + 4 * NV <= SWP, "run loads stay inside the padded row");
+    for (int job = threadIdx.x; job < SH * RUNS; job += kThreads) {
+        const int r = job / RUNS, c0 = (job - r * RUNS) * kRun;
+        const float4* p = reinterpret_cast<const float4*>(L.in + r * SWP + c0);
+        float v[4 * NV];
+#pragma unroll
+        for (int q = 0; q < NV; q++) {
+            const float4 t = p[q];
+            v[4 * q] = t.x;
+            v[4 * q + 1] = t.y;
+            v[4 * q + 2] = t.z;
+            v[4 * q + 3] = t.w;
+        }
+        float acc[kRun];
+#pragma unroll
+        for (int i = 0; i < kRun; i++) acc[i] = 0.0f;
+#pragma unroll
+        for (int k = 0; k <= 2 * R; k++) {
+            const float g = T->g[k];
+#pragma unroll
+            for (int i = 0; i < kRun; i++) acc[i] = fmaf(g, v[i + k], acc[i]);
+        }
+        float4* o = reinterpret_cast<float4*>(mid + r * kTW + c0);
+        o[0] = make_float4(acc[0] * L.colf[c0], acc[1] * L.colf[c0 + 1], acc[2] * L.colf[c0 + 2],
+                           acc[3] * L.colf[c0 + 3]);
+        o[1] = make_float4(acc[4] * L.colf[c0 + 4], acc[5] * L.colf[c0 + 5], acc[6] * L.colf[c0 + 6],
+                           acc[7] * L.colf[c0 + 7]);
+    }
+}
+
+// Blur plane src of the tile into out[j] (rows kRows * ty + j); ctr[j]
+// receives the unblurred value at the same pixel. The LDS is free for reuse
+// after the caller's next __syncthreads (the next blur_tile begins with one).
 template <int R>
 __device__ __forceinline__ void blur_tile(BlurLds<R>& L, const float* __restrict__ src, const Taps* __restrict__ T,
                                           int w, int h, int ox0, int oy0, float (&out)[kRows], float (&ctr)[kRows]) {
-    constexpr int SW = BlurLds<R>::SW, SH = BlurLds<R>::SH;
+    constexpr int SW = BlurLds<R>::SW, SWP = BlurLds<R>::SWP, SH = BlurLds<R>::SH;
     const int tx = threadIdx.x & (kTW - 1), ty = threadIdx.x / kTW;
     __syncthreads();
-    stage_plane<SW, SH>(L.in, src, w, h, ox0 - R, oy0 - R);
+    stage_padded<SW, SWP, SH>(L.in, src, w, h, ox0 - R, oy0 - R);
     __syncthreads();
-    hpass<R, SW, SH, kTW>(L.in, L.mid, L.colf, T);
+    hpass_runs<R>(L, L.mid, T);
 #pragma unroll
-    for (int j = 0; j < kRows; j++) ctr[j] = L.in[(ty + 4 * j + R) * SW + tx + R];
+    for (int j = 0; j < kRows; j++) ctr[j] = L.in[(kRows * ty + j + R) * SWP + tx + R];
     __syncthreads();
+    // column pass: 8 outputs of column tx from rows kRows * ty .. + kRows - 1 + 2R
+    float v[kRows + 2 * R];
 #pragma unroll
-    for (int j = 0; j < kRows; j++) {
-        const int r = ty + 4 * j;
-        out[j] = vsum<R, kTW>(L.mid, T, r, tx) * L.rowf[r];
+    for (int i = 0; i < kRows + 2 * R; i++) v[i] = L.mid[(kRows * ty + i) * kTW + tx];
+#pragma unroll
+    for (int j = 0; j < kRows; j++) out[j] = 0.0f;
+#pragma unroll
+    for (int k = 0; k <= 2 * R; k++) {
+        const float g = T->g[k];
+#pragma unroll
+        for (int j = 0; j < kRows; j++) out[j] = fmaf(g, v[j + k], out[j]);
     }
+#pragma unroll
+    for (int j = 0; j < kRows; j++) out[j] *= L.rowf[kRows * ty + j];
 }
 
 // sigma 1.2 blur of linear RGB + opsin dynamics (colors.hpp:50-91) -> XYB
@@ -371,7 +438,7 @@ __global__ __launch_bounds__(kThreads) void butter_opsin_kernel(Scale S, const T
     if (gx >= S.w) return;
 #pragma unroll
     for (int j = 0; j < kRows; j++) {
-        const int gy = oy0 + ty + 4 * j;
+        const int gy = oy0 + kRows * ty + j;
         if (gy >= S.h) continue;
         float b0, b1, b2;
         opsin_absorbance(bl[0][j] * intensity, bl[1][j] * intensity, bl[2][j] * intensity, b0, b1, b2);
@@ -412,14 +479,14 @@ __global__ __launch_bounds__(kThreads) void butter_low_kernel(Scale S, const Tap
         float* t = plane_of(S, pair, P_T + im * 3 + c);
 #pragma unroll
         for (int j = 0; j < kRows; j++) {
-            const int gy = oy0 + ty + 4 * j;
+            const int gy = oy0 + kRows * ty + j;
             if (gx < S.w && gy < S.h) t[(int64_t)gy * S.w + gx] = ctr[j] - lf[c][j];
         }
     }
     if (gx >= S.w) return;
 #pragma unroll
     for (int j = 0; j < kRows; j++) {
-        const int gy = oy0 + ty + 4 * j;
+        const int gy = oy0 + kRows * ty + j;
         if (gy >= S.h) continue;
         const int64_t g = (int64_t)gy * S.w + gx;
         float b = lf[2][j] + -0.362267051518f * lf[1][j];
@@ -457,14 +524,14 @@ __global__ __launch_bounds__(kThreads) void butter_mid_kernel(Scale S, const Tap
                 hfr[c][j] = ctr[j] - m;
                 m = c == 0 ? remove_range(m, 0.29f) : amplify_range(m, 0.1f);
             }
-            const int gy = oy0 + ty + 4 * j;
+            const int gy = oy0 + kRows * ty + j;
             if (gx < S.w && gy < S.h) dst[(int64_t)gy * S.w + gx] = m;
         }
     }
     if (gx >= S.w) return;
 #pragma unroll
     for (int j = 0; j < kRows; j++) {
-        const int gy = oy0 + ty + 4 * j;
+        const int gy = oy0 + kRows * ty + j;
         if (gy >= S.h) continue;
         const float s = 0.653020556257f, yw = 46.0f, yv = hfr[1][j];
         const float x = hfr[0][j] * (s + (yw * (1.0f - s)) / (yw + yv * yv));
@@ -506,7 +573,7 @@ __global__ __launch_bounds__(kThreads) void butter_high_kernel(Scale S, const Ta
                 hv *= 2.155f;
                 hv = amplify_range(hv, 0.132f);
             }
-            const int gy = oy0 + ty + 4 * j;
+            const int gy = oy0 + kRows * ty + j;
             if (gx < S.w && gy < S.h) {
                 dh[(int64_t)gy * S.w + gx] = hv;
                 du[(int64_t)gy * S.w + gx] = uv;

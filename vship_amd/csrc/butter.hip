@@ -330,34 +330,53 @@ constexpr int kTW = 64, kTH = 32, kRows = kTH / (kThreads / kTW);
 constexpr int kRun = 8;  // outputs per row-pass job
 static_assert(kRows == kRun, "the column pass computes kRun rows per thread");
 
+// Row strides of the staged input and of the row-pass output are 4 floats
+// past a multiple of 8, so that the 16-byte accesses of the 16 lanes that
+// the LDS serves together (two rows of eight runs) fall in distinct banks.
 template <int R>
 struct alignas(16) BlurLds {
     static constexpr int SW = kTW + 2 * R, SH = kTH + 2 * R;
-    static constexpr int SWP = (SW + 3) / 4 * 4;  // padded row
+    static constexpr int SWP = (SW + 3) / 8 * 8 + 4;  // padded row, 4 mod 8, >= SW
+    static constexpr int MW = kTW + 4;                 // row-pass output row
     float in[SH * SWP];
-    float mid[SH * kTW];
+    float mid[SH * MW];
     float colf[kTW];
     float rowf[kTH];
 };
 
 // in[r][c] = src(gx0 + c, gy0 + r) for c < SW, 0 outside the image and in
-// the row padding
+// the row padding. Thread (tx, ty) fills columns tx, tx + 64, ... of rows
+// ty, ty + 4, ...
 template <int SW, int SWP, int SH>
 __device__ __forceinline__ void stage_padded(float* __restrict__ in, const float* __restrict__ src, int w, int h,
                                              int gx0, int gy0) {
-    for (int i = threadIdx.x; i < SWP * SH; i += kThreads) {
-        const int r = i / SWP, c = i - r * SWP;
-        const int gx = gx0 + c, gy = gy0 + r;
-        float v = 0.0f;
-        if (c < SW && gx >= 0 && gx < w && gy >= 0 && gy < h) v = src[(int64_t)gy * w + gx];
-        in[i] = v;
+    constexpr int NR = (SH + 3) / 4;  // rows per thread
+    const int tx = threadIdx.x & 63, ty = threadIdx.x >> 6;
+#pragma unroll
+    for (int c0 = 0; c0 < SWP; c0 += 64) {
+        const int c = c0 + tx;
+        const int gx = gx0 + c;
+        const bool col_ok = c < SWP && c < SW && gx >= 0 && gx < w;
+        const float* col = src + gx;
+        float v[NR];
+        // all loads of the column first, then the stores
+#pragma unroll
+        for (int k = 0; k < NR; k++) {
+            const int gy = gy0 + ty + 4 * k;
+            v[k] = (col_ok && ty + 4 * k < SH && gy >= 0 && gy < h) ? col[(int64_t)gy * w] : 0.0f;
+        }
+        if (c < SWP) {
+#pragma unroll
+            for (int k = 0; k < NR; k++)
+                if (ty + 4 * k < SH) in[(ty + 4 * k) * SWP + c] = v[k];
+        }
     }
 }
 
 // row pass: mid[r][c] = colf[c] * sum_k g[k] in[r][c + k]
 template <int R>
 __device__ __forceinline__ void hpass_runs(const BlurLds<R>& L, float* __restrict__ mid, const Taps* __restrict__ T) {
-    constexpr int SWP = BlurLds<R>::SWP, SH = BlurLds<R>::SH, RUNS = kTW / kRun;
+    constexpr int SWP = BlurLds<R>::SWP, SH = BlurLds<R>::SH, MW = BlurLds<R>::MW, RUNS = kTW / kRun;
     constexpr int NV = (kRun + 2 * R + 3) / 4;  // float4 loads per run
     static_assert((RUNS - 1) * kRun + 4 * NV <= SWP, "run loads stay inside the padded row");
     for (int job = threadIdx.x; job < SH * RUNS; job += kThreads) {
@@ -381,7 +400,7 @@ __device__ __forceinline__ void hpass_runs(const BlurLds<R>& L, float* __restric
 #pragma unroll
             for (int i = 0; i < kRun; i++) acc[i] = fmaf(g, v[i + k], acc[i]);
         }
-        float4* o = reinterpret_cast<float4*>(mid + r * kTW + c0);
+        float4* o = reinterpret_cast<float4*>(mid + r * MW + c0);
         o[0] = make_float4(acc[0] * L.colf[c0], acc[1] * L.colf[c0 + 1], acc[2] * L.colf[c0 + 2],
                            acc[3] * L.colf[c0 + 3]);
         o[1] = make_float4(acc[4] * L.colf[c0 + 4], acc[5] * L.colf[c0 + 5], acc[6] * L.colf[c0 + 6],
@@ -405,9 +424,10 @@ __device__ __forceinline__ void blur_tile(BlurLds<R>& L, const float* __restrict
     for (int j = 0; j < kRows; j++) ctr[j] = L.in[(kRows * ty + j + R) * SWP + tx + R];
     __syncthreads();
     // column pass: 8 outputs of column tx from rows kRows * ty .. + kRows - 1 + 2R
+    constexpr int MW = BlurLds<R>::MW;
     float v[kRows + 2 * R];
 #pragma unroll
-    for (int i = 0; i < kRows + 2 * R; i++) v[i] = L.mid[(kRows * ty + i) * kTW + tx];
+    for (int i = 0; i < kRows + 2 * R; i++) v[i] = L.mid[(kRows * ty + i) * MW + tx];
 #pragma unroll
     for (int j = 0; j < kRows; j++) out[j] = 0.0f;
 #pragma unroll
@@ -638,19 +658,33 @@ constexpr MaltaLines kMaltaLF = {
      {{2, -4}, {1, -2}, {0, 0}, {-1, 2}, {-2, 4}}},
 };
 
+// The Malta terms of kDRows vertically adjacent pixels: a thread loads the
+// (kDRows + 8) x 9 window of scaled differences around its column once and
+// sums every line of every pixel from registers (the lines' taps lie within
+// 4 pixels of the centre).
+constexpr int kWinR = kDRows + 2 * kDHalo, kWinC = 2 * kDHalo + 1;
+
 template <bool HF>
-__device__ __forceinline__ float malta_lines(const float* __restrict__ p) {
+__device__ __forceinline__ void malta_block(const float* __restrict__ win_origin, float (&ac)[kDRows]) {
     constexpr const MaltaLines& M = HF ? kMaltaHF : kMaltaLF;
-    float r = 0.0f;
+    float win[kWinR][kWinC];
 #pragma unroll
-    for (int l = 0; l < 16; l++) {
-        float s = 0.0f;
+    for (int r = 0; r < kWinR; r++)
 #pragma unroll
-        for (int k = 0; k < 9; k++)
-            if (k < M.n[l]) s += p[M.d[l][k][1] * kDSW + M.d[l][k][0]];
-        r = fmaf(s, s, r);
+        for (int c = 0; c < kWinC; c++) win[r][c] = win_origin[r * kDSW + c];
+#pragma unroll
+    for (int j = 0; j < kDRows; j++) {
+        float res = 0.0f;
+#pragma unroll
+        for (int l = 0; l < 16; l++) {
+            float sum = 0.0f;
+#pragma unroll
+            for (int k = 0; k < 9; k++)
+                if (k < M.n[l]) sum += win[kDHalo + j + M.d[l][k][1]][kDHalo + M.d[l][k][0]];
+            res = fmaf(sum, sum, res);
+        }
+        ac[j] += res;
     }
-    return r;
 }
 
 // One Malta term: the scaled differences of the two planes.
@@ -735,17 +769,17 @@ __global__ __launch_bounds__(kThreads) void butter_diff_kernel(Scale S, DiffPara
             dt[i] = v;
         }
         __syncthreads();
-#pragma unroll
-        for (int j = 0; j < kDRows; j++) {
-            const float* p = dt + (ty + 4 * j + kDHalo) * kDSW + tx + kDHalo;
-            ac[j] += t < 2 ? malta_lines<true>(p) : malta_lines<false>(p);
-        }
+        const float* origin = dt + (kDRows * ty) * kDSW + tx;
+        if (t < 2)
+            malta_block<true>(origin, ac);
+        else
+            malta_block<false>(origin, ac);
     }
     const int gx = ox0 + tx;
     if (gx >= w) return;
 #pragma unroll
     for (int j = 0; j < kDRows; j++) {
-        const int gy = oy0 + ty + 4 * j;
+        const int gy = oy0 + kDRows * ty + j;
         if (gy >= h) continue;
         const int64_t g = (int64_t)gy * w + gx;
         float acc = ac[j];

@@ -89,44 +89,6 @@ __device__ __forceinline__ float inv_total(const Taps* __restrict__ T, int p, in
     return 1.0f / (T->cum[hi + 1] - T->cum[lo]);
 }
 
-// in[r][c] = src(gx0 + c, gy0 + r), 0 outside the image
-template <int SW, int SH>
-__device__ __forceinline__ void stage_plane(float* __restrict__ in, const float* __restrict__ src, int w, int h,
-                                            int gx0, int gy0) {
-    for (int i = threadIdx.x; i < SW * SH; i += kThreads) {
-        const int r = i / SW, c = i - r * SW;
-        const int gx = gx0 + c, gy = gy0 + r;
-        float v = 0.0f;
-        if (gx >= 0 && gx < w && gy >= 0 && gy < h) v = src[(int64_t)gy * w + gx];
-        in[i] = v;
-    }
-}
-
-// mid[r][c] = colf[c] * sum_k g[k] in[r][c + k], for SH rows and OW columns
-template <int R, int SW, int SH, int OW>
-__device__ __forceinline__ void hpass(const float* __restrict__ in, float* __restrict__ mid,
-                                      const float* __restrict__ colf, const Taps* __restrict__ T) {
-    static_assert(SW == OW + 2 * R, "staged width");
-    for (int i = threadIdx.x; i < SH * OW; i += kThreads) {
-        const int r = i / OW, c = i - r * OW;
-        const float* p = in + r * SW + c;
-        float s = 0.0f;
-#pragma unroll
-        for (int k = 0; k <= 2 * R; k++) s = fmaf(T->g[k], p[k], s);
-        mid[i] = s * colf[c];
-    }
-}
-
-// sum_k g[k] mid[r + k][c]
-template <int R, int OW>
-__device__ __forceinline__ float vsum(const float* __restrict__ mid, const Taps* __restrict__ T, int r, int c) {
-    const float* p = mid + r * OW + c;
-    float s = 0.0f;
-#pragma unroll
-    for (int k = 0; k <= 2 * R; k++) s = fmaf(T->g[k], p[k * OW], s);
-    return s;
-}
-
 // Column and row factors of an output region [x0, x0 + OW) x [y0, y0 + OH):
 // 0 outside the image.
 template <int R, int OW, int OH>
@@ -373,15 +335,19 @@ __device__ __forceinline__ void stage_padded(float* __restrict__ in, const float
     }
 }
 
-// row pass: mid[r][c] = colf[c] * sum_k g[k] in[r][c + k]
-template <int R>
-__device__ __forceinline__ void hpass_runs(const BlurLds<R>& L, float* __restrict__ mid, const Taps* __restrict__ T) {
-    constexpr int SWP = BlurLds<R>::SWP, SH = BlurLds<R>::SH, MW = BlurLds<R>::MW, RUNS = kTW / kRun;
+// row pass over SH staged rows (stride SWP, 16-byte aligned) into OW
+// columns (stride MW): mid[r][c] = colf[c] * sum_k g[k] in[r][c + k], in
+// jobs of 8 consecutive outputs
+template <int R, int OW, int SWP, int MW, int SH>
+__device__ __forceinline__ void hpass_blocked(const float* __restrict__ in, float* __restrict__ mid,
+                                              const float* __restrict__ colf, const Taps* __restrict__ T) {
+    constexpr int RUNS = OW / kRun;
     constexpr int NV = (kRun + 2 * R + 3) / 4;  // float4 loads per run
+    static_assert(OW % kRun == 0 && SWP % 4 == 0 && MW % 4 == 0, "vector rows");
     static_assert((RUNS - 1) * kRun + 4 * NV <= SWP, "run loads stay inside the padded row");
     for (int job = threadIdx.x; job < SH * RUNS; job += kThreads) {
         const int r = job / RUNS, c0 = (job - r * RUNS) * kRun;
-        const float4* p = reinterpret_cast<const float4*>(L.in + r * SWP + c0);
+        const float4* p = reinterpret_cast<const float4*>(in + r * SWP + c0);
         float v[4 * NV];
 #pragma unroll
         for (int q = 0; q < NV; q++) {
@@ -401,10 +367,27 @@ __device__ __forceinline__ void hpass_runs(const BlurLds<R>& L, float* __restric
             for (int i = 0; i < kRun; i++) acc[i] = fmaf(g, v[i + k], acc[i]);
         }
         float4* o = reinterpret_cast<float4*>(mid + r * MW + c0);
-        o[0] = make_float4(acc[0] * L.colf[c0], acc[1] * L.colf[c0 + 1], acc[2] * L.colf[c0 + 2],
-                           acc[3] * L.colf[c0 + 3]);
-        o[1] = make_float4(acc[4] * L.colf[c0 + 4], acc[5] * L.colf[c0 + 5], acc[6] * L.colf[c0 + 6],
-                           acc[7] * L.colf[c0 + 7]);
+        o[0] = make_float4(acc[0] * colf[c0], acc[1] * colf[c0 + 1], acc[2] * colf[c0 + 2], acc[3] * colf[c0 + 3]);
+        o[1] = make_float4(acc[4] * colf[c0 + 4], acc[5] * colf[c0 + 5], acc[6] * colf[c0 + 6],
+                           acc[7] * colf[c0 + 7]);
+    }
+}
+
+// column pass: N outputs of column c, rows r0 .. r0 + N - 1 of the blur,
+// from mid rows r0 .. r0 + N - 1 + 2R (stride MW), not yet row-normalised
+template <int R, int MW, int N>
+__device__ __forceinline__ void vpass_block(const float* __restrict__ mid, const Taps* __restrict__ T, int r0, int c,
+                                            float (&out)[N]) {
+    float v[N + 2 * R];
+#pragma unroll
+    for (int i = 0; i < N + 2 * R; i++) v[i] = mid[(r0 + i) * MW + c];
+#pragma unroll
+    for (int j = 0; j < N; j++) out[j] = 0.0f;
+#pragma unroll
+    for (int k = 0; k <= 2 * R; k++) {
+        const float g = T->g[k];
+#pragma unroll
+        for (int j = 0; j < N; j++) out[j] = fmaf(g, v[j + k], out[j]);
     }
 }
 
@@ -419,23 +402,13 @@ __device__ __forceinline__ void blur_tile(BlurLds<R>& L, const float* __restrict
     __syncthreads();
     stage_padded<SW, SWP, SH>(L.in, src, w, h, ox0 - R, oy0 - R);
     __syncthreads();
-    hpass_runs<R>(L, L.mid, T);
+    constexpr int MW = BlurLds<R>::MW;
+    hpass_blocked<R, kTW, SWP, MW, SH>(L.in, L.mid, L.colf, T);
 #pragma unroll
     for (int j = 0; j < kRows; j++) ctr[j] = L.in[(kRows * ty + j + R) * SWP + tx + R];
     __syncthreads();
     // column pass: 8 outputs of column tx from rows kRows * ty .. + kRows - 1 + 2R
-    constexpr int MW = BlurLds<R>::MW;
-    float v[kRows + 2 * R];
-#pragma unroll
-    for (int i = 0; i < kRows + 2 * R; i++) v[i] = L.mid[(kRows * ty + i) * MW + tx];
-#pragma unroll
-    for (int j = 0; j < kRows; j++) out[j] = 0.0f;
-#pragma unroll
-    for (int k = 0; k <= 2 * R; k++) {
-        const float g = T->g[k];
-#pragma unroll
-        for (int j = 0; j < kRows; j++) out[j] = fmaf(g, v[j + k], out[j]);
-    }
+    vpass_block<R, MW, kRows>(L.mid, T, kRows * ty, tx, out);
 #pragma unroll
     for (int j = 0; j < kRows; j++) out[j] *= L.rowf[kRows * ty + j];
 }
@@ -692,12 +665,15 @@ struct MaltaTerm {
     float n2gt, n2lt, norm1;
 };
 
+// The two weights share one reciprocal of norm1 + |a|/2 + |b|/2 (the
+// reference divides twice; the results differ by rounding only).
 __device__ __forceinline__ float malta_scaled_diff(float a, float b, const MaltaTerm& M) {
     const float absval = 0.5f * fabsf(a) + 0.5f * fabsf(b);
     const float diff = a - b;
-    const float scaler = M.n2gt / (M.norm1 + absval);
+    const float inv = 1.0f / (M.norm1 + absval);
+    const float scaler = M.n2gt * inv;
     float v = scaler * diff;
-    const float scaler2 = M.n2lt / (M.norm1 + absval);
+    const float scaler2 = M.n2lt * inv;
     const float fabs0 = fabsf(a);
     const float too_small = 0.55f * fabs0, too_big = 1.05f * fabs0;
     float impact = 0.0f;
@@ -819,18 +795,28 @@ __global__ __launch_bounds__(kThreads) void butter_diff_kernel(Scale S, DiffPara
 // 3-pixel halo.
 // ---------------------------------------------------------------------------
 constexpr int kMT = 32, kMErode = 3;
-constexpr int kMB = kMT + 2 * kMErode;  // blurred region 38 x 38
+// the blurred reference mask is kept on a 40 x 40 region (the tile and a
+// 4-pixel frame, of which the erosion reads 3) so that both passes run in
+// jobs of 8
+constexpr int kMF = 4, kMB = kMT + 2 * kMF;
 
-struct MaskLds {
+struct alignas(16) MaskLds {
     static constexpr int R = kRad[SG_MASK];
-    static constexpr int SW = kMB + 2 * R;  // 54
-    float in[SW * SW];
-    float mid[SW * kMB];
+    static constexpr int SW = kMB + 2 * R;               // 56: staged reference mask
+    static constexpr int SWP = (SW + 3) / 8 * 8 + 4;     // 60
+    static constexpr int MW = kMB + 4;                   // 44
+    static constexpr int SW1 = kMT + 2 * R;              // 48: staged distorted mask
+    static constexpr int SWP1 = (SW1 + 3) / 8 * 8 + 4;   // 52
+    static constexpr int MW1 = kMT + 4;                  // 36
+    float in[SW * SWP];
+    float mid[SW * MW];
     float b0[kMB * kMB];
     float colf[kMB];
     float rowf[kMB];
     double red[3][kThreads / 64];
 };
+static_assert(MaskLds::SW1 * MaskLds::SWP1 <= MaskLds::SW * MaskLds::SWP, "staging reuse");
+static_assert(MaskLds::SW1 * MaskLds::MW1 <= MaskLds::SW * MaskLds::MW, "row-pass reuse");
 
 struct NormArgs {
     int qnorm;
@@ -855,43 +841,49 @@ __device__ __forceinline__ double wave_max(double v) {
 
 template <bool FULL>
 __global__ __launch_bounds__(kThreads) void butter_mask_kernel(Scale S, const Taps* __restrict__ taps, NormArgs N) {
-    constexpr int R = MaskLds::R, SW = MaskLds::SW;
+    using M = MaskLds;
+    constexpr int R = M::R;
     __shared__ MaskLds L;
     const Taps* T = taps + SG_MASK;
     const int pair = blockIdx.z;
     const int ox0 = blockIdx.x * kMT, oy0 = blockIdx.y * kMT;
     const int w = S.w, h = S.h;
-    edge_factors<R, kMB, kMB>(L.colf, L.rowf, T, w, h, ox0 - kMErode, oy0 - kMErode);
-    // blurred reference mask on the 38 x 38 region
-    stage_plane<SW, SW>(L.in, plane_of(S, pair, P_M0), w, h, ox0 - kMErode - R, oy0 - kMErode - R);
+    edge_factors<R, kMB, kMB>(L.colf, L.rowf, T, w, h, ox0 - kMF, oy0 - kMF);
+    // blurred reference mask on the 40 x 40 region
+    stage_padded<M::SW, M::SWP, M::SW>(L.in, plane_of(S, pair, P_M0), w, h, ox0 - kMF - R, oy0 - kMF - R);
     __syncthreads();
-    hpass<R, SW, SW, kMB>(L.in, L.mid, L.colf, T);
+    hpass_blocked<R, kMB, M::SWP, M::MW, M::SW>(L.in, L.mid, L.colf, T);
     __syncthreads();
-    for (int i = threadIdx.x; i < kMB * kMB; i += kThreads) {
-        const int r = i / kMB, c = i - r * kMB;
-        L.b0[i] = mul_rounded(vsum<R, kMB>(L.mid, T, r, c), L.rowf[r]);
+    for (int job = threadIdx.x; job < kMB * (kMB / 8); job += kThreads) {
+        const int c = job % kMB, r0 = (job / kMB) * 8;
+        float o[8];
+        vpass_block<R, M::MW, 8>(L.mid, T, r0, c, o);
+#pragma unroll
+        for (int j = 0; j < 8; j++) L.b0[(r0 + j) * kMB + c] = mul_rounded(o[j], L.rowf[r0 + j]);
     }
     __syncthreads();
     // blurred distorted mask on the 32 x 32 tile
-    constexpr int SW1 = kMT + 2 * R;
-    stage_plane<SW1, SW1>(L.in, plane_of(S, pair, P_M1), w, h, ox0 - R, oy0 - R);
+    stage_padded<M::SW1, M::SWP1, M::SW1>(L.in, plane_of(S, pair, P_M1), w, h, ox0 - R, oy0 - R);
     __syncthreads();
-    hpass<R, SW1, SW1, kMT>(L.in, L.mid, L.colf + kMErode, T);
+    hpass_blocked<R, kMT, M::SWP1, M::MW1, M::SW1>(L.in, L.mid, L.colf + kMF, T);
     __syncthreads();
+    // thread: column tx, rows 4 ty .. 4 ty + 3 of the tile
+    constexpr int kMRows = kMT * kMT / kThreads;
     const int tx = threadIdx.x & (kMT - 1), ty = threadIdx.x / kMT;
-    constexpr int kMRows = kMT / (kThreads / kMT);
+    float b1v[kMRows];
+    vpass_block<R, M::MW1, kMRows>(L.mid, T, kMRows * ty, tx, b1v);
     double sq = 0.0, s3 = 0.0, mx = 0.0;
     const int gx = ox0 + tx;
 #pragma unroll
     for (int j = 0; j < kMRows; j++) {
-        const int r = ty + 8 * j;
+        const int r = kMRows * ty + j;
         const int gy = oy0 + r;
         // rounded like b0 (stored in LDS): a contraction of the product into
         // b0 - b1 would make identical masks differ near the top and bottom
         // edges, where the row factor is not 1
-        const float b1 = mul_rounded(vsum<R, kMT>(L.mid, T, r, tx), L.rowf[r + kMErode]);
+        const float b1 = mul_rounded(b1v[j], L.rowf[r + kMF]);
         if (gx >= w || gy >= h) continue;
-        const float* bc = L.b0 + (r + kMErode) * kMB + tx + kMErode;
+        const float* bc = L.b0 + (r + kMF) * kMB + tx + kMF;
         const float b0 = bc[0];
         float m0 = b0, m1 = 2.0f * b0, m2 = m1;
         if (gx >= kMErode) {

@@ -267,6 +267,73 @@ __global__ __launch_bounds__(kThreads) void butter_input_kernel(const PairDesc* 
     }
 }
 
+// 4:2:0 with left-located chroma: 64 x 64 luma tiles, the chroma
+// neighbourhood staged in LDS (as the SSIMULACRA2 prep kernel does); a
+// thread converts four 2 x 2 quads per image.
+template <bool WIDE>
+__global__ __launch_bounds__(kThreads) void butter_input420_kernel(const PairDesc* __restrict__ pairs, Scale S0,
+                                                                   Scale S1) {
+    __shared__ float ch[2][kCh420][kCh420 + 1];
+    const int pair = blockIdx.z;
+    const PairDesc& P = pairs[pair];
+    const int W = S0.w, H = S0.h;
+    const int x0 = blockIdx.x * 64, y0 = blockIdx.y * 64;
+    const int maxcode = WIDE ? (1 << P.bits) - 1 : 255;
+    const float sc = (float)(1 << (P.bits - 8));
+    const float ymul = 1.0f / (219.0f * sc), yadd = -16.0f / 219.0f;
+    const float cmul = 1.0f / (128.0f * 224.0f * sc), cadd = -128.0f / 224.0f;
+    const float kr = P.kr, kb = P.kb, ikg = 1.0f / (1.0f - P.kr - P.kb);
+    const float rcr = 2.0f * (1.0f - kr), bcb = 2.0f * (1.0f - kb);
+    for (int im = 0; im < 2; im++) {
+        const uint8_t* const* src = P.src[im];
+        const int64_t* stride = P.stride[im];
+        __syncthreads();
+        stage_chroma420(ch, src, stride, W, H, x0, y0, WIDE, maxcode);
+        __syncthreads();
+#pragma unroll 1
+        for (int i = 0; i < 4; i++) {
+            const int q = threadIdx.x + kThreads * i;
+            const int qy = q >> 5, qx = q & 31;
+            const int xa = x0 + 2 * qx, ya = y0 + 2 * qy;
+            if (xa >= W || ya >= H) continue;
+            const bool hx = xa + 1 < W, hy = ya + 1 < H;
+            const int xb = hx ? xa + 1 : xa, yb = hy ? ya + 1 : ya;
+            const float yv[4] = {sample_u(src[0], stride[0], xa, ya, WIDE, maxcode),
+                                 sample_u(src[0], stride[0], xa, yb, WIDE, maxcode),
+                                 sample_u(src[0], stride[0], xb, ya, WIDE, maxcode),
+                                 sample_u(src[0], stride[0], xb, yb, WIDE, maxcode)};
+            float cb[4], cr[4];
+            chroma420_quad(ch[0], qx, qy, hx, hy, cmul, cadd, cb);
+            chroma420_quad(ch[1], qx, qy, hx, hy, cmul, cadd, cr);
+            float lin[4][3];  // (xa,ya), (xa,yb), (xb,ya), (xb,yb)
+#pragma unroll
+            for (int k = 0; k < 4; k++) {
+                const float yn = fmaf(yv[k], ymul, yadd);
+                const float R = fmaf(rcr, cr[k], yn);
+                const float B = fmaf(bcb, cb[k], yn);
+                lin[k][0] = linearize(R);
+                lin[k][1] = linearize((yn - kr * R - kb * B) * ikg);
+                lin[k][2] = linearize(B);
+            }
+#pragma unroll
+            for (int c = 0; c < 3; c++) {
+                float* L = plane_of(S0, pair, P_LIN + im * 3 + c);
+                L[(int64_t)ya * W + xa] = lin[0][c];
+                if (hy) L[(int64_t)yb * W + xa] = lin[1][c];
+                if (hx) L[(int64_t)ya * W + xb] = lin[2][c];
+                if (hx && hy) L[(int64_t)yb * W + xb] = lin[3][c];
+                // half scale: 2x2 mean, the reference's summation order
+                float m = 0.0f;
+                m += lin[0][c];
+                m += lin[1][c];
+                m += lin[2][c];
+                m += lin[3][c];
+                plane_of(S1, pair, P_LIN + im * 3 + c)[(int64_t)(ya / 2) * S1.w + xa / 2] = m * 0.25f;
+            }
+        }
+    }
+}
+
 template <int FMT>
 void launch_input(dim3 g, hipStream_t st, const PairDesc* pairs, const Scale& S0, const Scale& S1) {
     butter_input_kernel<FMT><<<g, dim3(kThreads), 0, st>>>(pairs, S0, S1);
@@ -1105,6 +1172,10 @@ void Engine::submit(int fmt, const PairDesc* pairs, int n) {
     HIP_OR_THROW(hipSetDevice(device_), E_BadDeviceArgument);
     // the previous batch may still read the pinned descriptors
     HIP_OR_THROW(hipStreamSynchronize(stream_), E_BadDeviceArgument);
+    // 4:2:0 batches whose pairs all have left-located chroma take the staged
+    // input kernel
+    staged420_ = fmt == IN_YUV420_U8 || fmt == IN_YUV420_U16;
+    for (int i = 0; i < n && staged420_; i++) staged420_ = pairs[i].cloc == CLOC_LEFT;
     std::memcpy(desc_h_, pairs, sizeof(PairDesc) * n);
     HIP_OR_THROW(hipMemcpyAsync(desc_d_, desc_h_, sizeof(PairDesc) * n, hipMemcpyHostToDevice, stream_),
                  E_BadDeviceArgument);
@@ -1159,6 +1230,14 @@ void Engine::launch(int fmt, int n) {
     };
     const dim3 gi(cdiv(w2_, 16), cdiv(h2_, 16), n);
     auto input = [&] {
+        if (staged420_) {
+            const dim3 g4(cdiv(w_, 64), cdiv(h_, 64), n);
+            if (fmt == IN_YUV420_U16)
+                butter_input420_kernel<true><<<g4, dim3(kThreads), 0, stream_>>>(desc_d_, S0, S1);
+            else
+                butter_input420_kernel<false><<<g4, dim3(kThreads), 0, stream_>>>(desc_d_, S0, S1);
+            return;
+        }
         launch_input_fmt(fmt, gi, stream_, desc_d_, S0, S1, std::make_integer_sequence<int, IN_NUM_FORMATS>{});
     };
     if (!prof) {

@@ -89,6 +89,7 @@ private:
     bool timed_ = false;
     int last_n_ = 0;
     int last_fmt_ = IN_RGB_F32;
+    bool staged420_ = false;
     hipStream_t stream_ = nullptr;
     float* arena_ = nullptr;
     int64_t plane0_ = 0, plane1_ = 0;

@@ -164,17 +164,7 @@ __device__ __forceinline__ void prep_quads(PrepLds& L, const PairDev& P, uint8_t
         // stage the 36x36 chroma neighbourhood (indices clamped to the plane)
         // thread t < 252 owns column t % 36 and rows t / 36 + 7k of both planes,
         // so a wave reads whole row runs and the clamped column is computed once
-        const int cw = (W + 1) >> 1, chh = (H + 1) >> 1;
-        const int cx0 = x0 / 2 - 1, cy0 = y0 / 2 - 2;
-        if (tid < 252) {
-            const int c = tid % 36;
-            const int gx = min(max(cx0 + c, 0), cw - 1);
-            for (int r = tid / 36; r < 36; r += 7) {
-                const int gy = min(max(cy0 + r, 0), chh - 1);
-                L.ch[0][r][c] = sample_u(src[1], stride[1], gx, gy, wide, maxcode);
-                L.ch[1][r][c] = sample_u(src[2], stride[2], gx, gy, wide, maxcode);
-            }
-        }
+        stage_chroma420(L.ch, src, stride, W, H, x0, y0, wide, maxcode);
         __syncthreads();
     }
 
@@ -226,28 +216,8 @@ __device__ __forceinline__ void prep_quads(PrepLds& L, const PairDev& P, uint8_t
                                            cmul1, cadd);
                 }
             } else {
-                // chroma: local staged rows qy..qy+4, columns qx..qx+3
 #pragma unroll
-                for (int pl = 0; pl < 2; pl++) {
-                    float he[5], ho[5];  // horizontal: even luma column (left-sited sample), odd (Catmull-Rom at 0.5)
-#pragma unroll
-                    for (int r = 0; r < 5; r++) {
-                        const float* row = L.ch[pl][qy + r];
-                        he[r] = row[qx + 1];
-                        ho[r] = (9.0f * (row[qx + 1] + row[qx + 2]) - (row[qx] + row[qx + 3])) * (1.0f / 16.0f);
-                    }
-                    // vertical, centre-sited: even luma row t = 0.75 from rows 0..3, odd row t = 0.25 from rows 1..4
-                    const float ve_e = -3.0f * he[0] + 29.0f * he[1] + 111.0f * he[2] - 9.0f * he[3];
-                    const float vo_e = -9.0f * he[1] + 111.0f * he[2] + 29.0f * he[3] - 3.0f * he[4];
-                    const float ve_o = -3.0f * ho[0] + 29.0f * ho[1] + 111.0f * ho[2] - 9.0f * ho[3];
-                    const float vo_o = -9.0f * ho[1] + 111.0f * ho[2] + 29.0f * ho[3] - 3.0f * ho[4];
-                    // (xa,ya): even col, even row; (xa,yb): even col, odd row; (xb,..): odd col.
-                    // A clamped column/row repeats the first one.
-                    cbcr[pl][0] = fmaf(ve_e, cmul, cadd);
-                    cbcr[pl][1] = fmaf(hy ? vo_e : ve_e, cmul, cadd);
-                    cbcr[pl][2] = fmaf(hx ? ve_o : ve_e, cmul, cadd);
-                    cbcr[pl][3] = fmaf(hx ? (hy ? vo_o : ve_o) : (hy ? vo_e : ve_e), cmul, cadd);
-                }
+                for (int pl = 0; pl < 2; pl++) chroma420_quad(L.ch[pl], qx, qy, hx, hy, cmul, cadd, cbcr[pl]);
             }
 #pragma unroll
             for (int k = 0; k < 4; k++) {

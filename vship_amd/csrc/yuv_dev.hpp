@@ -79,4 +79,55 @@ __device__ __forceinline__ float chroma_at(const uint8_t* plane, int64_t stride,
     return a;
 }
 
+// 4:2:0 with left-located chroma (the common case), staged: the 36 x 36
+// chroma neighbourhood of a 64 x 64 luma tile at (x0, y0), origin
+// (x0 / 2 - 1, y0 / 2 - 2), indices clamped to the plane. Thread t < 252 owns
+// column t % 36 and rows t / 36 + 7 k of both planes, so a wave reads whole
+// row runs and the clamped column is computed once.
+constexpr int kCh420 = 36;
+
+__device__ __forceinline__ void stage_chroma420(float (*ch)[kCh420][kCh420 + 1], const uint8_t* const* src,
+                                                const int64_t* stride, int W, int H, int x0, int y0, bool wide,
+                                                int maxcode) {
+    const int tid = threadIdx.x;
+    const int cw = (W + 1) >> 1, chh = (H + 1) >> 1;
+    const int cx0 = x0 / 2 - 1, cy0 = y0 / 2 - 2;
+    if (tid < 252) {
+        const int c = tid % 36;
+        const int gx = min(max(cx0 + c, 0), cw - 1);
+        for (int r = tid / 36; r < 36; r += 7) {
+            const int gy = min(max(cy0 + r, 0), chh - 1);
+            ch[0][r][c] = sample_u(src[1], stride[1], gx, gy, wide, maxcode);
+            ch[1][r][c] = sample_u(src[2], stride[2], gx, gy, wide, maxcode);
+        }
+    }
+}
+
+// Chroma (scaled by cmul, offset by cadd) of the quad of luma pixels
+// (2 qx + {0, 1}, 2 qy + {0, 1}) of the staged tile, in the order (xa, ya),
+// (xa, yb), (xb, ya), (xb, yb); hx / hy: the quad has a second column / row
+// (else it repeats the first). Horizontally the even column takes the
+// co-sited sample and the odd one Catmull-Rom at 1/2, (-1, 9, 9, -1) / 16;
+// vertically (centre-sited) the even row is Catmull-Rom at 3/4 from rows
+// 0..3 and the odd row at 1/4 from rows 1..4, in integer weights of total 128
+// that cmul takes out.
+__device__ __forceinline__ void chroma420_quad(const float (*ch)[kCh420 + 1], int qx, int qy, bool hx, bool hy,
+                                               float cmul, float cadd, float out[4]) {
+    float he[5], ho[5];
+#pragma unroll
+    for (int r = 0; r < 5; r++) {
+        const float* row = ch[qy + r];
+        he[r] = row[qx + 1];
+        ho[r] = (9.0f * (row[qx + 1] + row[qx + 2]) - (row[qx] + row[qx + 3])) * (1.0f / 16.0f);
+    }
+    const float ve_e = -3.0f * he[0] + 29.0f * he[1] + 111.0f * he[2] - 9.0f * he[3];
+    const float vo_e = -9.0f * he[1] + 111.0f * he[2] + 29.0f * he[3] - 3.0f * he[4];
+    const float ve_o = -3.0f * ho[0] + 29.0f * ho[1] + 111.0f * ho[2] - 9.0f * ho[3];
+    const float vo_o = -9.0f * ho[1] + 111.0f * ho[2] + 29.0f * ho[3] - 3.0f * ho[4];
+    out[0] = fmaf(ve_e, cmul, cadd);
+    out[1] = fmaf(hy ? vo_e : ve_e, cmul, cadd);
+    out[2] = fmaf(hx ? ve_o : ve_e, cmul, cadd);
+    out[3] = fmaf(hx ? (hy ? vo_o : ve_o) : (hy ? vo_e : ve_e), cmul, cadd);
+}
+
 }  // namespace vship

@@ -81,6 +81,24 @@ __device__ __forceinline__ float* plane_of(const Scale& S, int pair, int p) {
     return S.base + (int64_t)pair * S.pair + (int64_t)p * S.plane;
 }
 
+// Tile of this workgroup, XCD-aware: hardware workgroup b (in launch order
+// x fastest, then y, then z) runs on XCD b % 8, so tiles are numbered such
+// that each XCD takes one contiguous range of tile rows, and the halos that
+// neighbouring tiles share are read through that XCD's L2 once.
+struct TileIdx {
+    int x, y, z;
+};
+__device__ __forceinline__ TileIdx tile_of_block() {
+    const int nx = gridDim.x, ny = gridDim.y, nz = gridDim.z;
+    const int b = blockIdx.x + nx * (blockIdx.y + ny * blockIdx.z);
+    const int v = xcd_block(b, nx * ny * nz);
+    TileIdx t;
+    t.x = v % nx;
+    t.y = (v / nx) % ny;
+    t.z = v / (nx * ny);
+    return t;
+}
+
 // 1 / (sum of the taps inside [0, n)) at position p in [0, n) for a window of
 // radius R (gaussianblur.hpp:63-65, 83-84).
 template <int R>
@@ -274,10 +292,11 @@ template <bool WIDE>
 __global__ __launch_bounds__(kThreads) void butter_input420_kernel(const PairDesc* __restrict__ pairs, Scale S0,
                                                                    Scale S1) {
     __shared__ float ch[2][kCh420][kCh420 + 1];
-    const int pair = blockIdx.z;
+    const TileIdx ti = tile_of_block();
+    const int pair = ti.z;
     const PairDesc& P = pairs[pair];
     const int W = S0.w, H = S0.h;
-    const int x0 = blockIdx.x * 64, y0 = blockIdx.y * 64;
+    const int x0 = ti.x * 64, y0 = ti.y * 64;
     const int maxcode = WIDE ? (1 << P.bits) - 1 : 255;
     const float sc = (float)(1 << (P.bits - 8));
     const float ymul = 1.0f / (219.0f * sc), yadd = -16.0f / 219.0f;
@@ -486,8 +505,9 @@ __global__ __launch_bounds__(kThreads) void butter_opsin_kernel(Scale S, const T
     constexpr int R = kRad[SG_OPSIN];
     __shared__ BlurLds<R> L;
     const Taps* T = taps + SG_OPSIN;
-    const int pair = blockIdx.z >> 1, im = blockIdx.z & 1;
-    const int ox0 = blockIdx.x * kTW, oy0 = blockIdx.y * kTH;
+    const TileIdx ti = tile_of_block();
+    const int pair = ti.z >> 1, im = ti.z & 1;
+    const int ox0 = ti.x * kTW, oy0 = ti.y * kTH;
     edge_factors<R, kTW, kTH>(L.colf, L.rowf, T, S.w, S.h, ox0, oy0);
     float bl[3][kRows], ctr[3][kRows];
 #pragma unroll
@@ -526,8 +546,9 @@ __global__ __launch_bounds__(kThreads) void butter_low_kernel(Scale S, const Tap
     constexpr int R = kRad[SG_LOW];
     __shared__ BlurLds<R> L;
     const Taps* T = taps + SG_LOW;
-    const int pair = blockIdx.z >> 1, im = blockIdx.z & 1;
-    const int ox0 = blockIdx.x * kTW, oy0 = blockIdx.y * kTH;
+    const TileIdx ti = tile_of_block();
+    const int pair = ti.z >> 1, im = ti.z & 1;
+    const int ox0 = ti.x * kTW, oy0 = ti.y * kTH;
     const int tx = threadIdx.x & (kTW - 1), ty = threadIdx.x / kTW;
     const int gx = ox0 + tx;
     edge_factors<R, kTW, kTH>(L.colf, L.rowf, T, S.w, S.h, ox0, oy0);
@@ -564,8 +585,9 @@ __global__ __launch_bounds__(kThreads) void butter_mid_kernel(Scale S, const Tap
     constexpr int R = kRad[SG_MID];
     __shared__ BlurLds<R> L;
     const Taps* T = taps + SG_MID;
-    const int pair = blockIdx.z >> 1, im = blockIdx.z & 1;
-    const int ox0 = blockIdx.x * kTW, oy0 = blockIdx.y * kTH;
+    const TileIdx ti = tile_of_block();
+    const int pair = ti.z >> 1, im = ti.z & 1;
+    const int ox0 = ti.x * kTW, oy0 = ti.y * kTH;
     const int tx = threadIdx.x & (kTW - 1), ty = threadIdx.x / kTW;
     const int gx = ox0 + tx;
     edge_factors<R, kTW, kTH>(L.colf, L.rowf, T, S.w, S.h, ox0, oy0);
@@ -607,8 +629,9 @@ __global__ __launch_bounds__(kThreads) void butter_high_kernel(Scale S, const Ta
     constexpr int R = kRad[SG_HIGH];
     __shared__ BlurLds<R> L;
     const Taps* T = taps + SG_HIGH;
-    const int pair = blockIdx.z >> 1, im = blockIdx.z & 1;
-    const int ox0 = blockIdx.x * kTW, oy0 = blockIdx.y * kTH;
+    const TileIdx ti = tile_of_block();
+    const int pair = ti.z >> 1, im = ti.z & 1;
+    const int ox0 = ti.x * kTW, oy0 = ti.y * kTH;
     const int tx = threadIdx.x & (kTW - 1), ty = threadIdx.x / kTW;
     const int gx = ox0 + tx;
     edge_factors<R, kTW, kTH>(L.colf, L.rowf, T, S.w, S.h, ox0, oy0);
@@ -786,8 +809,9 @@ __device__ __forceinline__ float l2_asym(float a, float b, float wgt, float wlt)
 
 __global__ __launch_bounds__(kThreads) void butter_diff_kernel(Scale S, DiffParams D) {
     __shared__ float dt[kDSH * kDSW];
-    const int pair = blockIdx.z;
-    const int ox0 = blockIdx.x * kDW, oy0 = blockIdx.y * kDH;
+    const TileIdx ti = tile_of_block();
+    const int pair = ti.z;
+    const int ox0 = ti.x * kDW, oy0 = ti.y * kDH;
     const int tx = threadIdx.x & (kDW - 1), ty = threadIdx.x / kDW;
     const int w = S.w, h = S.h;
     float ac[kDRows];
@@ -912,8 +936,9 @@ __global__ __launch_bounds__(kThreads) void butter_mask_kernel(Scale S, const Ta
     constexpr int R = M::R;
     __shared__ MaskLds L;
     const Taps* T = taps + SG_MASK;
-    const int pair = blockIdx.z;
-    const int ox0 = blockIdx.x * kMT, oy0 = blockIdx.y * kMT;
+    const TileIdx ti = tile_of_block();
+    const int pair = ti.z;
+    const int ox0 = ti.x * kMT, oy0 = ti.y * kMT;
     const int w = S.w, h = S.h;
     edge_factors<R, kMB, kMB>(L.colf, L.rowf, T, w, h, ox0 - kMF, oy0 - kMF);
     // blurred reference mask on the 40 x 40 region
@@ -1005,7 +1030,7 @@ __global__ __launch_bounds__(kThreads) void butter_mask_kernel(Scale S, const Ta
                 b += L.red[1][k];
                 m = fmax(m, L.red[2][k]);
             }
-            double* out = N.partials + ((int64_t)pair * N.tiles + (int64_t)blockIdx.y * gridDim.x + blockIdx.x) * 3;
+            double* out = N.partials + ((int64_t)pair * N.tiles + (int64_t)ti.y * gridDim.x + ti.x) * 3;
             out[0] = a;
             out[1] = b;
             out[2] = m;

@@ -403,9 +403,51 @@ struct alignas(16) BlurLds {
 // the row padding. Thread (tx, ty) fills columns tx, tx + 64, ... of rows
 // ty, ty + 4, ...
 template <int SW, int SWP, int SH>
+struct StageRegs {
+    static constexpr int NC = (SWP + 63) / 64, NR = (SH + 3) / 4;  // column chunks, rows per thread
+    float v[NC][NR];
+};
+
+// the loads of stage_padded, into registers (issued together, so that their
+// latency overlaps whatever the caller does before stage_store)
+template <int SW, int SWP, int SH>
+__device__ __forceinline__ void stage_load(StageRegs<SW, SWP, SH>& S, const float* __restrict__ src, int w, int h,
+                                           int gx0, int gy0) {
+    using SR = StageRegs<SW, SWP, SH>;
+    const int tx = threadIdx.x & 63, ty = threadIdx.x >> 6;
+#pragma unroll
+    for (int cc = 0; cc < SR::NC; cc++) {
+        const int c = cc * 64 + tx;
+        const int gx = gx0 + c;
+        const bool col_ok = c < SW && gx >= 0 && gx < w;
+        const float* col = src + gx;
+#pragma unroll
+        for (int k = 0; k < SR::NR; k++) {
+            const int gy = gy0 + ty + 4 * k;
+            S.v[cc][k] = (col_ok && ty + 4 * k < SH && gy >= 0 && gy < h) ? col[(int64_t)gy * w] : 0.0f;
+        }
+    }
+}
+
+template <int SW, int SWP, int SH>
+__device__ __forceinline__ void stage_store(const StageRegs<SW, SWP, SH>& S, float* __restrict__ in) {
+    using SR = StageRegs<SW, SWP, SH>;
+    const int tx = threadIdx.x & 63, ty = threadIdx.x >> 6;
+#pragma unroll
+    for (int cc = 0; cc < SR::NC; cc++) {
+        const int c = cc * 64 + tx;
+        if (c >= SWP) continue;
+#pragma unroll
+        for (int k = 0; k < SR::NR; k++)
+            if (ty + 4 * k < SH) in[(ty + 4 * k) * SWP + c] = S.v[cc][k];
+    }
+}
+
+template <int SW, int SWP, int SH>
 __device__ __forceinline__ void stage_padded(float* __restrict__ in, const float* __restrict__ src, int w, int h,
                                              int gx0, int gy0) {
-    constexpr int NR = (SH + 3) / 4;  // rows per thread
+    // column chunk by column chunk, all loads of a chunk before its stores
+    constexpr int NR = (SH + 3) / 4;
     const int tx = threadIdx.x & 63, ty = threadIdx.x >> 6;
 #pragma unroll
     for (int c0 = 0; c0 < SWP; c0 += 64) {
@@ -414,7 +456,6 @@ __device__ __forceinline__ void stage_padded(float* __restrict__ in, const float
         const bool col_ok = c < SWP && c < SW && gx >= 0 && gx < w;
         const float* col = src + gx;
         float v[NR];
-        // all loads of the column first, then the stores
 #pragma unroll
         for (int k = 0; k < NR; k++) {
             const int gy = gy0 + ty + 4 * k;
@@ -948,10 +989,29 @@ __global__ __launch_bounds__(kThreads) void butter_mask_kernel(Scale S, const Ta
     const int ox0 = ti.x * kMT, oy0 = ti.y * kMT;
     const int w = S.w, h = S.h;
     edge_factors<R, kMB, kMB>(L.colf, L.rowf, T, w, h, ox0 - kMF, oy0 - kMF);
+    // thread: column tx, rows 4 ty .. 4 ty + 3 of the tile; the per-pixel
+    // inputs of the end are loaded first, to arrive during the blurs
+    constexpr int kMRows = kMT * kMT / kThreads;
+    const int tx = threadIdx.x & (kMT - 1), ty = threadIdx.x / kMT;
+    const int gx = ox0 + tx;
+    float acs_in[kMRows], dcs_in[kMRows], half_in[kMRows];
+#pragma unroll
+    for (int j = 0; j < kMRows; j++) {
+        const int gy = oy0 + kMRows * ty + j;
+        const bool ok = gx < w && gy < h;
+        const int64_t g = (int64_t)gy * w + gx;
+        acs_in[j] = ok ? plane_of(S, pair, P_ACS)[g] : 0.0f;
+        dcs_in[j] = ok ? plane_of(S, pair, P_DCS)[g] : 0.0f;
+        half_in[j] = (FULL && ok) ? N.half_map_base[(int64_t)pair * N.half_pair + (int64_t)(gy / 2) * N.half_w + gx / 2]
+                                  : 0.0f;
+    }
     // blurred reference mask on the 40 x 40 region
     stage_padded<M::SW, M::SWP, M::SW>(L.in, plane_of(S, pair, P_M0), w, h, ox0 - kMF - R, oy0 - kMF - R);
     __syncthreads();
     hpass_blocked<R, kMB, M::SWP, M::MW, M::SW>(L.in, L.mid, L.colf, T);
+    // the distorted mask's loads, in flight during the column pass
+    StageRegs<M::SW1, M::SWP1, M::SW1> m1_regs;
+    stage_load(m1_regs, plane_of(S, pair, P_M1), w, h, ox0 - R, oy0 - R);
     __syncthreads();
     for (int job = threadIdx.x; job < kMB * (kMB / 8); job += kThreads) {
         const int c = job % kMB, r0 = (job / kMB) * 8;
@@ -962,17 +1022,13 @@ __global__ __launch_bounds__(kThreads) void butter_mask_kernel(Scale S, const Ta
     }
     __syncthreads();
     // blurred distorted mask on the 32 x 32 tile
-    stage_padded<M::SW1, M::SWP1, M::SW1>(L.in, plane_of(S, pair, P_M1), w, h, ox0 - R, oy0 - R);
+    stage_store(m1_regs, L.in);
     __syncthreads();
     hpass_blocked<R, kMT, M::SWP1, M::MW1, M::SW1>(L.in, L.mid, L.colf + kMF, T);
     __syncthreads();
-    // thread: column tx, rows 4 ty .. 4 ty + 3 of the tile
-    constexpr int kMRows = kMT * kMT / kThreads;
-    const int tx = threadIdx.x & (kMT - 1), ty = threadIdx.x / kMT;
     float b1v[kMRows];
     vpass_block<R, M::MW1, kMRows>(L.mid, T, kMRows * ty, tx, b1v);
     double sq = 0.0, s3 = 0.0, mx = 0.0;
-    const int gx = ox0 + tx;
 #pragma unroll
     for (int j = 0; j < kMRows; j++) {
         const int r = kMRows * ty + j;
@@ -1000,14 +1056,14 @@ __global__ __launch_bounds__(kThreads) void butter_mask_kernel(Scale S, const Ta
         const float mask = 0.45f * m0 + 0.3f * m1 + 0.25f * m2;
         const int64_t g = (int64_t)gy * w + gx;
         const float db = b0 - b1;
-        const float acs = plane_of(S, pair, P_ACS)[g] + 10.0f * db * db;
-        const float dcs = plane_of(S, pair, P_DCS)[g];
+        const float acs = acs_in[j] + 10.0f * db * db;
+        const float dcs = dcs_in[j];
         float v = sqrtf(mask_y(mask) * acs + mask_dc_y(mask) * dcs);
         if (!FULL) {
             plane_of(S, pair, P_MAP)[g] = v;
         } else {
             v = (float)((double)v * (1.0 - 0.3 * 0.5));
-            v += 0.5f * N.half_map_base[(int64_t)pair * N.half_pair + (int64_t)(gy / 2) * N.half_w + gx / 2];
+            v += 0.5f * half_in[j];
             const PairDesc& P = N.pairs[pair];
             if (P.map) ((float*)((uint8_t*)P.map + (int64_t)gy * P.map_stride))[gx] = v;
             const double dv = v;

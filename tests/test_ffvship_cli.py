@@ -1,7 +1,8 @@
 """The FFVship-style command line (vship_amd/ffvship, vship_amd/csrc/ffvship_cli.cpp)
 on the CPU: YUV4MPEG2 header parsing, the reference's frame selection
 (src/FFVship.cpp:262-330, with the worked examples of doc/FFVship.md), the
-JSON and statistics formats (src/FFVship.cpp:142-200,439-468), and the
+JSON and statistics formats (src/FFVship.cpp:142-200,439-468) of both
+metrics, and the
 errors the command reports before it touches a GPU."""
 import ctypes
 import os
@@ -29,6 +30,8 @@ def hooks():
     F = ctypes.POINTER(ctypes.c_float)
     L.ffv_stats.argtypes = [F, ctypes.c_int, ctypes.c_char_p, ctypes.c_char_p, ctypes.c_int]
     L.ffv_json.argtypes = [F, ctypes.c_int, ctypes.c_char_p, ctypes.c_int]
+    L.ffv_json_norms.argtypes = [F, ctypes.c_int, ctypes.c_char_p, ctypes.c_int]
+    L.ffv_metric.argtypes = [ctypes.c_char_p]
     return L
 
 
@@ -133,6 +136,19 @@ def test_json_format():
     assert buf.value.decode() == "[[61.5], [100], [-3.25]]"
 
 
+@needs_hooks
+def test_butteraugli_json_format_and_metric_names():
+    # src/FFVship.cpp:448-452: [[q, 3, inf], ...] with ", " between values
+    a = np.array([2.5, 2.75, 4.0, 1.0, 1.25, 3.5], np.float32)
+    buf = ctypes.create_string_buffer(256)
+    hooks().ffv_json_norms(a.ctypes.data_as(ctypes.POINTER(ctypes.c_float)), 2, buf, len(buf))
+    assert buf.value.decode() == "[[2.5, 2.75, 4], [1, 1.25, 3.5]]"
+    # ffmpegmain.hpp:770-780: case-insensitive names and their short forms
+    names = {"SSIMULACRA2": 0, "ssimu2": 0, "Butteraugli": 1, "BUTTER": 1, "cvvdp": 2, "psnr": -1}
+    for n, v in names.items():
+        assert hooks().ffv_metric(n.encode()) == v
+
+
 def write_y4m(path, frames, w, h, tag="C420mpeg2", extra=""):
     with open(path, "wb") as f:
         f.write(f"YUV4MPEG2 W{w} H{h} F25:1 Ip A1:1 {tag}{extra}\n".encode())
@@ -159,7 +175,8 @@ def test_cli_errors_before_the_gpu(tmp_path):
     assert run_cli("--source", a, "--encoded", str(tmp_path / "missing.y4m"))[2].startswith("cannot open")
     assert "full-range" in run_cli("--source", a, "--encoded", b)[2]
     assert "differ in size, layout or depth" in run_cli("--source", a, "--encoded", c)[2]
-    assert "not supported by this build" in run_cli("-s", a, "-e", a, "-m", "Butteraugli")[2]
+    assert "not supported by this build" in run_cli("-s", a, "-e", a, "-m", "CVVDP")[2]
+    assert "not supported by this build" in run_cli("-s", a, "-e", a, "-m", "psnr")[2]
     assert "does not allow comparing" in run_cli("-s", a, "-e", a, "--encoded-offset", "5")[2]
     assert "unknown option" in run_cli("-s", a, "-e", a, "--frobnicate")[2]
     rc, out, _ = run_cli("--version")

@@ -97,3 +97,44 @@ def test_other_layouts(tmp_path, layout, bits, tag, loc):
     expect = oracle_scores(ref, dist, bits, sw, sh, loc, range(n))
     say(f"  {tag}: max |d| {max(abs(g - e) for g, e in zip(got, expect)):.1e}")
     assert all(abs(g - e) <= 1e-3 for g, e in zip(got, expect))
+
+
+@needs_cli
+def test_butteraugli_metric(tmp_path):
+    """-m Butteraugli: the three norms of every frame pair against the FP64
+    oracle (YUV to R'G'B' as the SSIMULACRA2 front end, then the metric), with
+    --qnorm and --intensity-target passed through; JSON [[q, 3, inf], ...],
+    live lines "index q 3 inf" and one statistics block per norm."""
+    from tests import butter_lib as bl
+
+    w, h, n = 160, 96, 5
+    ref, dist = clip(n, w, h, "420", 8, 950)
+    a, b, js = str(tmp_path / "ref.y4m"), str(tmp_path / "dist.y4m"), str(tmp_path / "b.json")
+    write_y4m(a, ref, w, h, "C420mpeg2")
+    write_y4m(b, dist, w, h, "C420mpeg2")
+    want = []
+    for i in range(n):
+        rgb = [ol.yuv_to_rgb_any(im, 8, ol.SO_MAT_BT601, 1, 1, ol.SO_LOC_LEFT).astype(np.float32)
+               for im in (ref[i], dist[i])]
+        want.append(bl.butteraugli(list(rgb[0]), list(rgb[1]), 3, 150.0))
+    want = np.array(want)
+    out = run("-s", a, "-e", b, "-m", "Butteraugli", "--qnorm", "3", "--intensity-target", "150", "-g", "1",
+              "--json", js)
+    got = np.array(json.load(open(js)))
+    assert got.shape == (n, 3)
+    # 1e-4 relative parity plus the JSON's six significant digits
+    assert np.all(np.abs(got - want) <= 1e-4 * want + 5e-6 * np.abs(want)), (got, want)
+    lines = out.splitlines()
+    assert lines[0] == f"Butteraugli Result between {a} and {b}"
+    # src/FFVship.cpp:496-498: "<q>-Norm", "3-Norm", "INF-Norm" blocks, in that order
+    titles = [l.strip("-") for l in lines if l.startswith("-")]
+    assert titles == ["3-Norm", "3-Norm", "INF-Norm"]
+    live = run("-s", a, "-e", b, "-m", "butter", "--live-score-output").splitlines()
+    assert live[0] == str(n)
+    for i, l in enumerate(live[1:]):
+        idx, q, n3, ninf = l.split()
+        assert int(idx) == i
+        w2 = bl.butteraugli(*[list(ol.yuv_to_rgb_any(im, 8, ol.SO_MAT_BT601, 1, 1, ol.SO_LOC_LEFT)
+                                   .astype(np.float32)) for im in (ref[i], dist[i])])
+        assert np.allclose([float(q), float(n3), float(ninf)], w2, rtol=2e-4)
+

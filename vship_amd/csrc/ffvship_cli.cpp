@@ -1,14 +1,18 @@
 // ffvship: the command line of the reference's FFVship (src/FFVship.cpp,
 // src/ffvship_utility/ffmpegmain.hpp:790-840, doc/FFVship.md) over this
-// library's batched SSIMULACRA2 engine, for YUV4MPEG2 (.y4m) inputs.
+// library's batched SSIMULACRA2 and Butteraugli engines, for YUV4MPEG2 (.y4m)
+// inputs.
 //
 // Same options and meanings for what a y4m input allows (--source/-s,
-// --encoded/-e, --metric/-m, --start, --end, --every, --encoded-offset,
-// --source-indices, --encoded-indices, --json, --live-score-output,
-// --threads/-t, --gpu-threads/-g, --gpu-id, --list-gpu, --version), the same
-// frame selection (src/FFVship.cpp:262-330), JSON file
-// ([[score], ...], :439-468), live lines (first the frame count, then
-// "index score") and console statistics (:142-200). Decoding goes away: y4m
+// --encoded/-e, --metric/-m SSIMULACRA2|Butteraugli, --intensity-target,
+// --qnorm, --start, --end, --every, --encoded-offset, --source-indices,
+// --encoded-indices, --json, --live-score-output, --threads/-t,
+// --gpu-threads/-g, --gpu-id, --list-gpu, --version), the same frame
+// selection (src/FFVship.cpp:262-330), JSON file ([[score], ...], or
+// [[qnorm, 3-norm, inf-norm], ...] for Butteraugli, :439-468), live lines
+// (first the frame count, then "index score" or "index q 3 inf", :113-140)
+// and console statistics (:142-200, one block per norm for Butteraugli,
+// :473-499). Decoding goes away: y4m
 // frames are raw planar YUV, read straight into pinned host memory, copied
 // to the device and scored in batches of 8 x --gpu-threads pairs, the next
 // batch read from disk while the GPU scores the current one.
@@ -24,6 +28,7 @@
 #include <unistd.h>
 
 #include <algorithm>
+#include <cctype>
 #include <chrono>
 #include <cmath>
 #include <cstdint>
@@ -38,6 +43,7 @@
 #include <thread>
 #include <vector>
 
+#include "VshipButteraugliBatch.h"
 #include "VshipSSIMU2Batch.h"
 
 namespace ffv {
@@ -296,6 +302,29 @@ std::string json_scores(const std::vector<float>& scores) {
     return o.str();
 }
 
+// Butteraugli: [[q, 3, inf], ...], values with the stream's default
+// formatting, separated by ", " (src/FFVship.cpp:448-452).
+std::string json_norms(const std::vector<float>& s3) {
+    std::ostringstream o;
+    const size_t n = s3.size() / 3;
+    o << "[";
+    for (size_t i = 0; i < n; i++)
+        o << "[" << s3[3 * i] << ", " << s3[3 * i + 1] << ", " << s3[3 * i + 2] << (i + 1 == n ? "]" : "], ");
+    o << "]";
+    return o.str();
+}
+
+// Metric names as FFVship parses them (ffmpegmain.hpp:770-780), lowercased:
+// 0 SSIMULACRA2, 1 Butteraugli, 2 CVVDP (not built here), -1 unknown.
+int metric_of(const std::string& name) {
+    std::string l = name;
+    for (char& c : l) c = (char)std::tolower((unsigned char)c);
+    if (l == "ssimulacra2" || l == "ssimu2") return 0;
+    if (l == "butteraugli" || l == "butter") return 1;
+    if (l == "cvvdp") return 2;
+    return -1;
+}
+
 // Statistics block (src/FFVship.cpp:142-200): the label centred in a dashed
 // rule as wide as "name : value", then the seven statistics right-aligned.
 std::string stats_block(const std::vector<float>& data, const std::string& label) {
@@ -334,11 +363,13 @@ std::string stats_block(const std::vector<float>& data, const std::string& label
 struct Options {
     std::string source, encoded, metric = "SSIMULACRA2", json, source_indices, encoded_indices;
     int start = 0, end = -1, every = 1, offset = 0, threads = 1, gpu_threads = 8, gpu_id = 0;
+    int qnorm = 2, intensity = 203;  // Butteraugli (ffmpegmain.hpp:725-727)
     bool live = false, list_gpu = false, version = false, help = false;
 };
 
 const char* kUsage =
-    "usage: ffvship [-h] --source SOURCE.y4m --encoded ENCODED.y4m [-m SSIMULACRA2]\n"
+    "usage: ffvship [-h] --source SOURCE.y4m --encoded ENCODED.y4m [-m SSIMULACRA2|Butteraugli]\n"
+    "               [--intensity-target NITS] [--qnorm Q]\n"
     "               [--start N] [--end N] [--every N] [--encoded-offset N]\n"
     "               [--source-indices i,j,..] [--encoded-indices i,j,..]\n"
     "               [-t THREADS] [-g GPU_THREADS] [--gpu-id ID] [--json OUTPUT]\n"
@@ -398,6 +429,10 @@ std::string parse_args(int argc, char** argv, Options& o) {
             if (!as_int(o.gpu_threads)) return "invalid value for " + a;
         } else if (a == "--gpu-id") {
             if (!as_int(o.gpu_id)) return "invalid value for " + a;
+        } else if (a == "--intensity-target") {
+            if (!as_int(o.intensity)) return "invalid value for " + a;
+        } else if (a == "--qnorm") {
+            if (!as_int(o.qnorm)) return "invalid value for " + a;
         } else {
             return "unknown option " + a;
         }
@@ -412,21 +447,32 @@ std::string vship_error(Vship_Exception e) {
 }
 
 // JSON file, then (unless live lines were asked for) the console summary.
+// scores: one per frame, or three (q, 3, inf) for Butteraugli.
 int report(const Options& o, const std::vector<float>& scores, double secs) {
+    const bool butter = metric_of(o.metric) == 1;
     if (!o.json.empty()) {
         std::ofstream js(o.json);
         if (!js) {
             std::cerr << "Failed to open output file" << std::endl;
             return 1;
         }
-        js << json_scores(scores);
+        js << (butter ? json_norms(scores) : json_scores(scores));
     }
     if (o.live) return 0;
-    const size_t n = scores.size();
+    const size_t n = butter ? scores.size() / 3 : scores.size();
     const double fps = secs > 0 ? (double)n / secs : 0.0;
-    std::cout << "SSIMU2 Result between " << o.source << " and " << o.encoded << std::endl;
+    std::cout << (butter ? "Butteraugli" : "SSIMU2") << " Result between " << o.source << " and " << o.encoded
+              << std::endl;
     std::cout << "Computed " << n << " frames at " << fps << " fps\n" << std::endl;
-    std::cout << stats_block(scores, "SSIMULACRA2");
+    if (!butter) {
+        std::cout << stats_block(scores, "SSIMULACRA2");
+        return 0;
+    }
+    std::vector<float> nq(n), n3(n), ninf(n);
+    for (size_t i = 0; i < n; i++) nq[i] = scores[3 * i], n3[i] = scores[3 * i + 1], ninf[i] = scores[3 * i + 2];
+    std::cout << stats_block(nq, std::to_string(o.qnorm) + "-Norm");
+    std::cout << stats_block(n3, "3-Norm");
+    std::cout << stats_block(ninf, "INF-Norm");
     return 0;
 }
 
@@ -435,15 +481,20 @@ struct Slot {
     uint8_t* host = nullptr;
     uint8_t* dev = nullptr;
     std::vector<Vship_SSIMU2Pair> pairs;
+    std::vector<Vship_ButteraugliPair> bpairs;
     std::vector<int> task;  // task index of each pair
     int n = 0;
 };
 
 int run(const Options& o) {
-    if (o.metric != "SSIMULACRA2" && o.metric != "ssimulacra2" && o.metric != "SSIMU2") {
-        std::cerr << "metric " << o.metric << " is not supported by this build (SSIMULACRA2 only)" << std::endl;
+    const int metric = metric_of(o.metric);
+    if (metric != 0 && metric != 1) {
+        std::cerr << "metric " << o.metric << " is not supported by this build (SSIMULACRA2, Butteraugli)"
+                  << std::endl;
         return 1;
     }
+    const bool butter = metric == 1;
+    const int per = butter ? 3 : 1;  // values per frame
     Y4mFile src, enc;
     if (!src.open_file(o.source)) {
         std::cerr << src.error << std::endl;
@@ -476,7 +527,7 @@ int run(const Options& o) {
     }
     const int nframes = (int)sel.source.size();
     if (o.live) std::cout << nframes << std::endl;
-    std::vector<float> scores(nframes, 0.0f);
+    std::vector<float> scores((size_t)nframes * per, 0.0f);
     if (nframes == 0) return report(o, scores, 0.0);
 
     Vship_Exception e = Vship_SetDevice(o.gpu_id);
@@ -490,8 +541,21 @@ int run(const Options& o) {
     cfg.max_pairs = batch;
     cfg.max_width = f.width;
     cfg.max_height = f.height;
-    Vship_SSIMU2BatchHandler h;
-    if ((e = Vship_SSIMU2BatchInit(&h, &cfg)) != Vship_NoError) {
+    Vship_SSIMU2BatchHandler h{};
+    Vship_ButteraugliBatchHandler bh{};
+    if (butter) {
+        Vship_ButteraugliBatchConfig bc;
+        std::memset(&bc, 0, sizeof(bc));
+        bc.width = f.width;
+        bc.height = f.height;
+        bc.max_pairs = batch;
+        bc.qnorm = o.qnorm;
+        bc.intensity_multiplier = (float)o.intensity;
+        e = Vship_ButteraugliBatchInit(&bh, &bc);
+    } else {
+        e = Vship_SSIMU2BatchInit(&h, &cfg);
+    }
+    if (e != Vship_NoError) {
         std::cerr << vship_error(e) << std::endl;
         return 1;
     }
@@ -506,12 +570,13 @@ int run(const Options& o) {
             rc = 1;
         }
         s.pairs.resize(batch);
+        s.bpairs.resize(batch);
         s.task.resize(batch);
     }
     const int matrix = f.height > 650 ? Vship_Matrix_BT709 : Vship_Matrix_BT601;
     const int fmt = pixel_format(f);
     const int nthreads = std::max(1, o.threads);
-    std::vector<double> out(batch);
+    std::vector<double> out((size_t)batch * per);
 
     // read tasks [first, first + n) into slot s (frames pinned, descriptors set)
     auto fill = [&](Slot& s, int first, int n) -> bool {
@@ -551,14 +616,38 @@ int run(const Options& o) {
             p.bits = f.bits;
             p.matrix = matrix;
             p.chroma_location = f.chroma_location;
+            Vship_ButteraugliPair& b = s.bpairs[k];
+            std::memset(&b, 0, sizeof(b));
+            for (int pl = 0; pl < 3; pl++) {
+                b.ref[pl] = p.ref[pl];
+                b.dist[pl] = p.dist[pl];
+                b.ref_stride[pl] = p.ref_stride[pl];
+                b.dist_stride[pl] = p.dist_stride[pl];
+            }
+            b.bits = f.bits;
+            b.matrix = matrix;
+            b.chroma_location = f.chroma_location;
             s.task[k] = first + k;
         }
         return true;
     };
+    // Butteraugli's batch call is synchronous: it runs on a thread of its
+    // own, so that reading the next batch still overlaps the GPU work
+    std::thread butter_worker;
+    Vship_Exception butter_rc = Vship_NoError;
     auto submit = [&](Slot& s) -> Vship_Exception {
         Vship_Exception x = Vship_MemcpyHtoD(s.dev, s.host, (uint64_t)(2 * fb * s.n));
         if (x != Vship_NoError) return x;
-        return Vship_SSIMU2BatchSubmit(h, s.pairs.data(), s.n);
+        if (!butter) return Vship_SSIMU2BatchSubmit(h, s.pairs.data(), s.n);
+        butter_worker = std::thread([&, sp = &s] {
+            butter_rc = Vship_ButteraugliBatchCompute(bh, fmt, sp->bpairs.data(), sp->n, out.data());
+        });
+        return Vship_NoError;
+    };
+    auto wait = [&]() -> Vship_Exception {
+        if (!butter) return Vship_SSIMU2BatchWait(h, out.data(), nullptr);
+        if (butter_worker.joinable()) butter_worker.join();
+        return butter_rc;
     };
 
     const auto t0 = std::chrono::steady_clock::now();
@@ -587,15 +676,20 @@ int run(const Options& o) {
             rc = 1;
         }
         Slot& s = slots[cur];
-        if ((e = Vship_SSIMU2BatchWait(h, out.data(), nullptr)) != Vship_NoError) {
+        if ((e = wait()) != Vship_NoError) {
             std::cerr << vship_error(e) << std::endl;
             rc = 1;
             break;
         }
         pending = false;
         for (int k = 0; k < s.n; k++) {
-            scores[s.task[k]] = (float)out[k];
-            if (o.live) std::cout << s.task[k] << " " << (float)out[k] << "\n";
+            for (int m = 0; m < per; m++) scores[(size_t)s.task[k] * per + m] = (float)out[(size_t)k * per + m];
+            if (!o.live) continue;
+            if (butter)
+                std::cout << s.task[k] << " " << (float)out[3 * k] << " " << (float)out[3 * k + 1] << " "
+                          << (float)out[3 * k + 2] << "\n";
+            else
+                std::cout << s.task[k] << " " << (float)out[k] << "\n";
         }
         if (o.live) std::cout.flush();
         if (rc == 0 && have_next) {
@@ -609,12 +703,16 @@ int run(const Options& o) {
             cur ^= 1;
         }
     }
+    if (butter_worker.joinable()) butter_worker.join();
     const double secs = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
     for (Slot& s : slots) {
         if (s.host) Vship_PinnedFree(s.host);
         if (s.dev) Vship_DeviceFree(s.dev);
     }
-    Vship_SSIMU2BatchFree(h);
+    if (butter)
+        Vship_ButteraugliBatchFree(bh);
+    else
+        Vship_SSIMU2BatchFree(h);
     if (rc != 0) return rc;
     return report(o, scores, secs);
 }
@@ -671,6 +769,12 @@ int ffv_json(const float* data, int n, char* out, int out_len) {
     snprintf(out, (size_t)out_len, "%s", s.c_str());
     return (int)s.size();
 }
+int ffv_json_norms(const float* data, int nframes, char* out, int out_len) {
+    const std::string s = ffv::json_norms(std::vector<float>(data, data + 3 * nframes));
+    snprintf(out, (size_t)out_len, "%s", s.c_str());
+    return (int)s.size();
+}
+int ffv_metric(const char* name) { return ffv::metric_of(name); }
 }
 #else
 int main(int argc, char** argv) {
@@ -686,7 +790,7 @@ int main(int argc, char** argv) {
     }
     if (o.version) {
         std::cout << "FFVship 3.1.0-b" << std::endl;
-        std::cout << "vship_amd: SSIMULACRA2 on MI355X, YUV4MPEG2 inputs" << std::endl;
+        std::cout << "vship_amd: SSIMULACRA2 and Butteraugli on MI355X, YUV4MPEG2 inputs" << std::endl;
         return 0;
     }
     if (o.list_gpu) return ffv::list_gpus();

@@ -325,7 +325,7 @@ def run_served(a):
     return out, host_pool, alone
 
 
-BUTTER_KERNELS = {"input": "butter_input_kernel<4>", "opsin": "butter_opsin_kernel", "low": "butter_low_kernel",
+BUTTER_KERNELS = {"input": "butter_input420_kernel<false>", "opsin": "butter_opsin_kernel", "low": "butter_low_kernel",
                   "mid": "butter_mid_kernel", "high": "butter_high_kernel", "diff": "butter_diff_kernel",
                   "mask": "butter_mask_kernel", "norm": "butter_norm_kernel"}
 BUTTER_BYTES = {"input": "Y + 2 quarter-size chroma bytes read per image, 6 linear FP32 planes written at "

@@ -868,20 +868,31 @@ __global__ __launch_bounds__(kThreads) void butter_diff_kernel(Scale S, DiffPara
     // (plane of image 0, of image 1, HF lines) per term
     const int src0[6] = {P_UHF + 1, P_UHF + 0, P_HF + 1, P_HF + 0, P_MF + 1, P_MF + 0};
     const int src1[6] = {P_UHF + 3, P_UHF + 2, P_HF + 3, P_HF + 2, P_MF + 4, P_MF + 3};
-#pragma unroll
+    // not unrolled: each term's loads stay in its own iteration (hoisted into
+    // the previous term they would hold registers across its line sums)
+#pragma unroll 1
     for (int t = 0; t < 6; t++) {
         const float* a = plane_of(S, pair, src0[t]);
         const float* b = plane_of(S, pair, src1[t]);
-        __syncthreads();
-        for (int i = threadIdx.x; i < kDSW * kDSH; i += kThreads) {
+        // stage the scaled differences of the tile and its halo: all of this
+        // thread's loads are issued before the first difference is formed
+        constexpr int NE = (kDSW * kDSH + kThreads - 1) / kThreads;
+        float va[NE], vb[NE];
+#pragma unroll
+        for (int k = 0; k < NE; k++) {
+            const int i = threadIdx.x + k * kThreads;
             const int r = i / kDSW, c = i - r * kDSW;
             const int gx = ox0 - kDHalo + c, gy = oy0 - kDHalo + r;
-            float v = 0.0f;
-            if (gx >= 0 && gx < w && gy >= 0 && gy < h) {
-                const int64_t g = (int64_t)gy * w + gx;
-                v = malta_scaled_diff(a[g], b[g], D.malta[t]);
-            }
-            dt[i] = v;
+            const bool ok = i < kDSW * kDSH && gx >= 0 && gx < w && gy >= 0 && gy < h;
+            const int64_t g = (int64_t)gy * w + gx;
+            va[k] = ok ? a[g] : 0.0f;
+            vb[k] = ok ? b[g] : 0.0f;
+        }
+        __syncthreads();
+#pragma unroll
+        for (int k = 0; k < NE; k++) {
+            const int i = threadIdx.x + k * kThreads;
+            if (i < kDSW * kDSH) dt[i] = malta_scaled_diff(va[k], vb[k], D.malta[t]);
         }
         __syncthreads();
         const float* origin = dt + (kDRows * ty) * kDSW + tx;

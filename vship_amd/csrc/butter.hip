@@ -73,6 +73,7 @@ enum : int {
 struct Scale {
     float* base;
     int w, h;
+    int pitch;      // floats between rows (a multiple of 64)
     int64_t plane;  // floats between planes
     int64_t pair;   // floats between pairs
 };
@@ -276,18 +277,18 @@ __global__ __launch_bounds__(kThreads) void butter_input_kernel(const PairDesc* 
             const float a = linearize(v[0][c]), b = linearize(v[1][c]);
             const float cc = linearize(v[2][c]), d = linearize(v[3][c]);
             float* L = plane_of(S0, pair, P_LIN + im * 3 + c);
-            L[(int64_t)y0 * w + x0] = a;
-            if (xo) L[(int64_t)y0 * w + x1] = b;
+            L[(int64_t)y0 * S0.pitch + x0] = a;
+            if (xo) L[(int64_t)y0 * S0.pitch + x1] = b;
             if (yo) {
-                L[(int64_t)y1 * w + x0] = cc;
-                if (xo) L[(int64_t)y1 * w + x1] = d;
+                L[(int64_t)y1 * S0.pitch + x0] = cc;
+                if (xo) L[(int64_t)y1 * S0.pitch + x1] = d;
             }
             float m = 0.0f;
             m += a;
             m += cc;
             m += b;
             m += d;
-            plane_of(S1, pair, P_LIN + im * 3 + c)[(int64_t)hy * S1.w + hx] = m * 0.25f;
+            plane_of(S1, pair, P_LIN + im * 3 + c)[(int64_t)hy * S1.pitch + hx] = m * 0.25f;
         }
     }
 }
@@ -344,17 +345,17 @@ __global__ __launch_bounds__(kThreads) void butter_input420_kernel(const PairDes
 #pragma unroll
             for (int c = 0; c < 3; c++) {
                 float* L = plane_of(S0, pair, P_LIN + im * 3 + c);
-                L[(int64_t)ya * W + xa] = lin[0][c];
-                if (hy) L[(int64_t)yb * W + xa] = lin[1][c];
-                if (hx) L[(int64_t)ya * W + xb] = lin[2][c];
-                if (hx && hy) L[(int64_t)yb * W + xb] = lin[3][c];
+                L[(int64_t)ya * S0.pitch + xa] = lin[0][c];
+                if (hy) L[(int64_t)yb * S0.pitch + xa] = lin[1][c];
+                if (hx) L[(int64_t)ya * S0.pitch + xb] = lin[2][c];
+                if (hx && hy) L[(int64_t)yb * S0.pitch + xb] = lin[3][c];
                 // half scale: 2x2 mean, the reference's summation order
                 float m = 0.0f;
                 m += lin[0][c];
                 m += lin[1][c];
                 m += lin[2][c];
                 m += lin[3][c];
-                plane_of(S1, pair, P_LIN + im * 3 + c)[(int64_t)(ya / 2) * S1.w + xa / 2] = m * 0.25f;
+                plane_of(S1, pair, P_LIN + im * 3 + c)[(int64_t)(ya / 2) * S1.pitch + xa / 2] = m * 0.25f;
             }
         }
     }
@@ -412,7 +413,7 @@ struct StageRegs {
 // latency overlaps whatever the caller does before stage_store)
 template <int SW, int SWP, int SH>
 __device__ __forceinline__ void stage_load(StageRegs<SW, SWP, SH>& S, const float* __restrict__ src, int w, int h,
-                                           int gx0, int gy0) {
+                                           int pitch, int gx0, int gy0) {
     using SR = StageRegs<SW, SWP, SH>;
     const int tx = threadIdx.x & 63, ty = threadIdx.x >> 6;
 #pragma unroll
@@ -424,7 +425,7 @@ __device__ __forceinline__ void stage_load(StageRegs<SW, SWP, SH>& S, const floa
 #pragma unroll
         for (int k = 0; k < SR::NR; k++) {
             const int gy = gy0 + ty + 4 * k;
-            S.v[cc][k] = (col_ok && ty + 4 * k < SH && gy >= 0 && gy < h) ? col[(int64_t)gy * w] : 0.0f;
+            S.v[cc][k] = (col_ok && ty + 4 * k < SH && gy >= 0 && gy < h) ? col[(int64_t)gy * pitch] : 0.0f;
         }
     }
 }
@@ -445,7 +446,7 @@ __device__ __forceinline__ void stage_store(const StageRegs<SW, SWP, SH>& S, flo
 
 template <int SW, int SWP, int SH>
 __device__ __forceinline__ void stage_padded(float* __restrict__ in, const float* __restrict__ src, int w, int h,
-                                             int gx0, int gy0) {
+                                             int pitch, int gx0, int gy0) {
     // column chunk by column chunk, all loads of a chunk before its stores
     constexpr int NR = (SH + 3) / 4;
     const int tx = threadIdx.x & 63, ty = threadIdx.x >> 6;
@@ -459,7 +460,7 @@ __device__ __forceinline__ void stage_padded(float* __restrict__ in, const float
 #pragma unroll
         for (int k = 0; k < NR; k++) {
             const int gy = gy0 + ty + 4 * k;
-            v[k] = (col_ok && ty + 4 * k < SH && gy >= 0 && gy < h) ? col[(int64_t)gy * w] : 0.0f;
+            v[k] = (col_ok && ty + 4 * k < SH && gy >= 0 && gy < h) ? col[(int64_t)gy * pitch] : 0.0f;
         }
         if (c < SWP) {
 #pragma unroll
@@ -530,11 +531,12 @@ __device__ __forceinline__ void vpass_block(const float* __restrict__ mid, const
 // after the caller's next __syncthreads (the next blur_tile begins with one).
 template <int R>
 __device__ __forceinline__ void blur_tile(BlurLds<R>& L, const float* __restrict__ src, const Taps* __restrict__ T,
-                                          int w, int h, int ox0, int oy0, float (&out)[kRows], float (&ctr)[kRows]) {
+                                          int w, int h, int pitch, int ox0, int oy0, float (&out)[kRows],
+                                          float (&ctr)[kRows]) {
     constexpr int SW = BlurLds<R>::SW, SWP = BlurLds<R>::SWP, SH = BlurLds<R>::SH;
     const int tx = threadIdx.x & (kTW - 1), ty = threadIdx.x / kTW;
     __syncthreads();
-    stage_padded<SW, SWP, SH>(L.in, src, w, h, ox0 - R, oy0 - R);
+    stage_padded<SW, SWP, SH>(L.in, src, w, h, pitch, ox0 - R, oy0 - R);
     __syncthreads();
     constexpr int MW = BlurLds<R>::MW;
     hpass_blocked<R, kTW, SWP, MW, SH>(L.in, L.mid, L.colf, T);
@@ -560,7 +562,7 @@ __global__ __launch_bounds__(kThreads) void butter_opsin_kernel(Scale S, const T
     float bl[3][kRows], ctr[3][kRows];
 #pragma unroll
     for (int c = 0; c < 3; c++)
-        blur_tile<R>(L, plane_of(S, pair, P_LIN + im * 3 + c), T, S.w, S.h, ox0, oy0, bl[c], ctr[c]);
+        blur_tile<R>(L, plane_of(S, pair, P_LIN + im * 3 + c), T, S.w, S.h, S.pitch, ox0, oy0, bl[c], ctr[c]);
     const int tx = threadIdx.x & (kTW - 1), ty = threadIdx.x / kTW;
     const int gx = ox0 + tx;
     if (gx >= S.w) return;
@@ -581,7 +583,7 @@ __global__ __launch_bounds__(kThreads) void butter_opsin_kernel(Scale S, const T
         o0 = fmaxf(o0 * s0, 1.7557483643287353f);
         o1 = fmaxf(o1 * s1, 1.7557483643287353f);
         o2 = fmaxf(o2 * s2, 12.226454707163354f);
-        const int64_t g = (int64_t)gy * S.w + gx;
+        const int64_t g = (int64_t)gy * S.pitch + gx;
         plane_of(S, pair, P_XYB + im * 3 + 0)[g] = o0 - o1;
         plane_of(S, pair, P_XYB + im * 3 + 1)[g] = o0 + o1;
         plane_of(S, pair, P_XYB + im * 3 + 2)[g] = o2;
@@ -604,12 +606,12 @@ __global__ __launch_bounds__(kThreads) void butter_low_kernel(Scale S, const Tap
 #pragma unroll
     for (int c = 0; c < 3; c++) {
         float ctr[kRows];
-        blur_tile<R>(L, plane_of(S, pair, P_XYB + im * 3 + c), T, S.w, S.h, ox0, oy0, lf[c], ctr);
+        blur_tile<R>(L, plane_of(S, pair, P_XYB + im * 3 + c), T, S.w, S.h, S.pitch, ox0, oy0, lf[c], ctr);
         float* t = plane_of(S, pair, P_T + im * 3 + c);
 #pragma unroll
         for (int j = 0; j < kRows; j++) {
             const int gy = oy0 + kRows * ty + j;
-            if (gx < S.w && gy < S.h) t[(int64_t)gy * S.w + gx] = ctr[j] - lf[c][j];
+            if (gx < S.w && gy < S.h) t[(int64_t)gy * S.pitch + gx] = ctr[j] - lf[c][j];
         }
     }
     if (gx >= S.w) return;
@@ -617,7 +619,7 @@ __global__ __launch_bounds__(kThreads) void butter_low_kernel(Scale S, const Tap
     for (int j = 0; j < kRows; j++) {
         const int gy = oy0 + kRows * ty + j;
         if (gy >= S.h) continue;
-        const int64_t g = (int64_t)gy * S.w + gx;
+        const int64_t g = (int64_t)gy * S.pitch + gx;
         float b = lf[2][j] + -0.362267051518f * lf[1][j];
         b = b * 49.87984651440f;
         plane_of(S, pair, P_LFV + im * 3 + 0)[g] = lf[0][j] * 33.832837186260f;
@@ -645,7 +647,7 @@ __global__ __launch_bounds__(kThreads) void butter_mid_kernel(Scale S, const Tap
     for (int o = 0; o < 3; o++) {
         const int c = kOrder[o];
         float mf[kRows], ctr[kRows];
-        blur_tile<R>(L, plane_of(S, pair, P_T + im * 3 + c), T, S.w, S.h, ox0, oy0, mf, ctr);
+        blur_tile<R>(L, plane_of(S, pair, P_T + im * 3 + c), T, S.w, S.h, S.pitch, ox0, oy0, mf, ctr);
         float* dst = plane_of(S, pair, P_MF + im * 3 + c);
 #pragma unroll
         for (int j = 0; j < kRows; j++) {
@@ -655,7 +657,7 @@ __global__ __launch_bounds__(kThreads) void butter_mid_kernel(Scale S, const Tap
                 m = c == 0 ? remove_range(m, 0.29f) : amplify_range(m, 0.1f);
             }
             const int gy = oy0 + kRows * ty + j;
-            if (gx < S.w && gy < S.h) dst[(int64_t)gy * S.w + gx] = m;
+            if (gx < S.w && gy < S.h) dst[(int64_t)gy * S.pitch + gx] = m;
         }
     }
     if (gx >= S.w) return;
@@ -665,7 +667,7 @@ __global__ __launch_bounds__(kThreads) void butter_mid_kernel(Scale S, const Tap
         if (gy >= S.h) continue;
         const float s = 0.653020556257f, yw = 46.0f, yv = hfr[1][j];
         const float x = hfr[0][j] * (s + (yw * (1.0f - s)) / (yw + yv * yv));
-        const int64_t g = (int64_t)gy * S.w + gx;
+        const int64_t g = (int64_t)gy * S.pitch + gx;
         plane_of(S, pair, P_HFR + im * 2 + 0)[g] = x;
         plane_of(S, pair, P_HFR + im * 2 + 1)[g] = yv;
     }
@@ -686,7 +688,7 @@ __global__ __launch_bounds__(kThreads) void butter_high_kernel(Scale S, const Ta
 #pragma unroll
     for (int c = 0; c < 2; c++) {
         float hf[kRows], ctr[kRows];
-        blur_tile<R>(L, plane_of(S, pair, P_HFR + im * 2 + c), T, S.w, S.h, ox0, oy0, hf, ctr);
+        blur_tile<R>(L, plane_of(S, pair, P_HFR + im * 2 + c), T, S.w, S.h, S.pitch, ox0, oy0, hf, ctr);
         float* dh = plane_of(S, pair, P_HF + im * 2 + c);
         float* du = plane_of(S, pair, P_UHF + im * 2 + c);
 #pragma unroll
@@ -706,8 +708,8 @@ __global__ __launch_bounds__(kThreads) void butter_high_kernel(Scale S, const Ta
             }
             const int gy = oy0 + kRows * ty + j;
             if (gx < S.w && gy < S.h) {
-                dh[(int64_t)gy * S.w + gx] = hv;
-                du[(int64_t)gy * S.w + gx] = uv;
+                dh[(int64_t)gy * S.pitch + gx] = hv;
+                du[(int64_t)gy * S.pitch + gx] = uv;
             }
         }
     }
@@ -884,7 +886,7 @@ __global__ __launch_bounds__(kThreads) void butter_diff_kernel(Scale S, DiffPara
             const int r = i / kDSW, c = i - r * kDSW;
             const int gx = ox0 - kDHalo + c, gy = oy0 - kDHalo + r;
             const bool ok = i < kDSW * kDSH && gx >= 0 && gx < w && gy >= 0 && gy < h;
-            const int64_t g = (int64_t)gy * w + gx;
+            const int64_t g = (int64_t)gy * S.pitch + gx;
             va[k] = ok ? a[g] : 0.0f;
             vb[k] = ok ? b[g] : 0.0f;
         }
@@ -907,7 +909,7 @@ __global__ __launch_bounds__(kThreads) void butter_diff_kernel(Scale S, DiffPara
     for (int j = 0; j < kDRows; j++) {
         const int gy = oy0 + kDRows * ty + j;
         if (gy >= h) continue;
-        const int64_t g = (int64_t)gy * w + gx;
+        const int64_t g = (int64_t)gy * S.pitch + gx;
         float acc = ac[j];
         float hf[2][2], uhf[2][2];
 #pragma unroll
@@ -975,7 +977,7 @@ struct NormArgs {
     const PairDesc* pairs;
     const float* half_map_base;  // half-scale P_MAP of pair 0
     int64_t half_pair;           // floats between pairs in the half scale
-    int half_w;
+    int half_pitch;              // floats between rows in the half scale
 };
 
 __device__ __forceinline__ double wave_sum(double v) {
@@ -1010,19 +1012,19 @@ __global__ __launch_bounds__(kThreads) void butter_mask_kernel(Scale S, const Ta
     for (int j = 0; j < kMRows; j++) {
         const int gy = oy0 + kMRows * ty + j;
         const bool ok = gx < w && gy < h;
-        const int64_t g = (int64_t)gy * w + gx;
+        const int64_t g = (int64_t)gy * S.pitch + gx;
         acs_in[j] = ok ? plane_of(S, pair, P_ACS)[g] : 0.0f;
         dcs_in[j] = ok ? plane_of(S, pair, P_DCS)[g] : 0.0f;
-        half_in[j] = (FULL && ok) ? N.half_map_base[(int64_t)pair * N.half_pair + (int64_t)(gy / 2) * N.half_w + gx / 2]
+        half_in[j] = (FULL && ok) ? N.half_map_base[(int64_t)pair * N.half_pair + (int64_t)(gy / 2) * N.half_pitch + gx / 2]
                                   : 0.0f;
     }
     // blurred reference mask on the 40 x 40 region
-    stage_padded<M::SW, M::SWP, M::SW>(L.in, plane_of(S, pair, P_M0), w, h, ox0 - kMF - R, oy0 - kMF - R);
+    stage_padded<M::SW, M::SWP, M::SW>(L.in, plane_of(S, pair, P_M0), w, h, S.pitch, ox0 - kMF - R, oy0 - kMF - R);
     __syncthreads();
     hpass_blocked<R, kMB, M::SWP, M::MW, M::SW>(L.in, L.mid, L.colf, T);
     // the distorted mask's loads, in flight during the column pass
     StageRegs<M::SW1, M::SWP1, M::SW1> m1_regs;
-    stage_load(m1_regs, plane_of(S, pair, P_M1), w, h, ox0 - R, oy0 - R);
+    stage_load(m1_regs, plane_of(S, pair, P_M1), w, h, S.pitch, ox0 - R, oy0 - R);
     __syncthreads();
     for (int job = threadIdx.x; job < kMB * (kMB / 8); job += kThreads) {
         const int c = job % kMB, r0 = (job / kMB) * 8;
@@ -1065,7 +1067,7 @@ __global__ __launch_bounds__(kThreads) void butter_mask_kernel(Scale S, const Ta
         if (gy >= kMErode) store_min3(bc[-kMErode * kMB], m0, m1, m2);
         if (gy < h - kMErode) store_min3(bc[kMErode * kMB], m0, m1, m2);
         const float mask = 0.45f * m0 + 0.3f * m1 + 0.25f * m2;
-        const int64_t g = (int64_t)gy * w + gx;
+        const int64_t g = (int64_t)gy * S.pitch + gx;
         const float db = b0 - b1;
         const float acs = acs_in[j] + 10.0f * db * db;
         const float dcs = dcs_in[j];
@@ -1149,7 +1151,8 @@ __global__ __launch_bounds__(kThreads) void butter_norm_kernel(const double* __r
 
 int cdiv(int a, int b) { return (a + b - 1) / b; }
 
-int64_t plane_floats(int w, int h) { return ((int64_t)w * h + 63) / 64 * 64; }
+int row_pitch(int w) { return (w + 63) / 64 * 64; }
+int64_t plane_floats(int w, int h) { return (int64_t)row_pitch(w) * h; }
 
 // Malta and asymmetric L2 weights as the reference's getdiffmap passes them
 // (main.hpp:57-89) and its launchers scale them (maltaDiff.hpp:663-683,
@@ -1292,8 +1295,9 @@ void Engine::run(int fmt, const PairDesc* pairs, int n, double* norms) {
 }
 
 void Engine::launch(int fmt, int n) {
-    const Scale S0{arena_, w_, h_, plane0_, (int64_t)P_COUNT * plane0_};
-    const Scale S1{arena_ + (int64_t)cap_ * P_COUNT * plane0_, w2_, h2_, plane1_, (int64_t)P_COUNT * plane1_};
+    const Scale S0{arena_, w_, h_, row_pitch(w_), plane0_, (int64_t)P_COUNT * plane0_};
+    const Scale S1{arena_ + (int64_t)cap_ * P_COUNT * plane0_, w2_, h2_, row_pitch(w2_), plane1_,
+                   (int64_t)P_COUNT * plane1_};
     const DiffParams D = diff_params();
     const bool prof = profile_;
     auto mark = [&](int st) {
@@ -1319,7 +1323,7 @@ void Engine::launch(int fmt, int n) {
             N.pairs = desc_d_;
             N.half_map_base = S1.base + (int64_t)P_MAP * S1.plane;
             N.half_pair = S1.pair;
-            N.half_w = S1.w;
+            N.half_pitch = S1.pitch;
             const dim3 gm(cdiv(S.w, kMT), cdiv(S.h, kMT), n);
             if (full)
                 butter_mask_kernel<true><<<gm, blk, 0, stream_>>>(S, taps_d_, N);

@@ -373,29 +373,34 @@ void launch_input_fmt(int fmt, dim3 g, hipStream_t st, const PairDesc* pairs, co
 }
 
 // ---------------------------------------------------------------------------
-// Blur tile kernels: 64 x 32 outputs, 256 threads; thread (tx, ty) owns
-// column tx and the kRows consecutive rows kRows * ty + j.
+// Blur tile kernels: 64 x 32 outputs, 256 threads; thread t owns the kRun
+// consecutive outputs of tile row t / 8, columns 8 (t % 8) .. 8 (t % 8) + 7.
 //
-// Both passes are register-blocked so that LDS reads do not bound them:
-// the row pass gives each job a run of 8 outputs of one staged row, read as
-// 16-byte vectors (rows padded to a multiple of 4 floats), and the column
-// pass gives each thread 8 outputs of one column from 8 + 2R reads. Every
-// output is the same sequence of FMAs over the taps as a plain convolution.
+// The column pass runs first, over the 64 + 2R staged columns of the 32
+// output rows; the row pass then forms the outputs. For this wider-than-tall
+// tile that order costs fewer FMAs than the row pass over 32 + 2R staged
+// rows: (64 + 2R) * 32 + 64 * 32 sums of 2R + 1 taps instead of
+// (32 + 2R) * 64 + 64 * 32. Both passes are register-blocked so that LDS
+// reads do not bound them: a column job sums 8 outputs of one column from
+// 8 + 2R reads, and a row job 8 outputs of one row from 16-byte vector reads
+// (rows padded to a multiple of 4 floats). Every output is the same sequence
+// of FMAs over the taps as a plain convolution.
 // ---------------------------------------------------------------------------
-constexpr int kTW = 64, kTH = 32, kRows = kTH / (kThreads / kTW);
-constexpr int kRun = 8;  // outputs per row-pass job
-static_assert(kRows == kRun, "the column pass computes kRun rows per thread");
+constexpr int kTW = 64, kTH = 32;
+constexpr int kRun = 8;              // outputs per job of either pass
+constexpr int kRuns = kTW / kRun;    // row jobs per tile row
+static_assert(kTH * kRuns == kThreads, "one row job per thread");
 
-// Row strides of the staged input and of the row-pass output are 4 floats
-// past a multiple of 8, so that the 16-byte accesses of the 16 lanes that
-// the LDS serves together (two rows of eight runs) fall in distinct banks.
+// Row strides of the staged input and of the column-pass output are 4
+// floats past a multiple of 8, so that the 16-byte accesses of the 16 lanes
+// that the LDS serves together (two rows of eight runs) fall in distinct
+// banks.
 template <int R>
 struct alignas(16) BlurLds {
     static constexpr int SW = kTW + 2 * R, SH = kTH + 2 * R;
     static constexpr int SWP = (SW + 3) / 8 * 8 + 4;  // padded row, 4 mod 8, >= SW
-    static constexpr int MW = kTW + 4;                 // row-pass output row
     float in[SH * SWP];
-    float mid[SH * MW];
+    float mid[kTH * SWP];  // column pass: the output rows over all staged columns
     float colf[kTW];
     float rowf[kTH];
 };
@@ -470,6 +475,37 @@ __device__ __forceinline__ void stage_padded(float* __restrict__ in, const float
     }
 }
 
+// float4 loads of one row job: its kRun outputs read kRun + 2R inputs
+template <int R>
+constexpr int run_vectors() {
+    return (kRun + 2 * R + 3) / 4;
+}
+
+// row job: acc[i] = sum_k g[k] p[i + k], i < kRun, from 16-byte reads of p
+// (16-byte aligned; 4 * run_vectors<R>() floats are read)
+template <int R>
+__device__ __forceinline__ void hrun(const float* __restrict__ p, const Taps* __restrict__ T, float (&acc)[kRun]) {
+    constexpr int NV = run_vectors<R>();
+    const float4* p4 = reinterpret_cast<const float4*>(p);
+    float v[4 * NV];
+#pragma unroll
+    for (int q = 0; q < NV; q++) {
+        const float4 t = p4[q];
+        v[4 * q] = t.x;
+        v[4 * q + 1] = t.y;
+        v[4 * q + 2] = t.z;
+        v[4 * q + 3] = t.w;
+    }
+#pragma unroll
+    for (int i = 0; i < kRun; i++) acc[i] = 0.0f;
+#pragma unroll
+    for (int k = 0; k <= 2 * R; k++) {
+        const float g = T->g[k];
+#pragma unroll
+        for (int i = 0; i < kRun; i++) acc[i] = fmaf(g, v[i + k], acc[i]);
+    }
+}
+
 // row pass over SH staged rows (stride SWP, 16-byte aligned) into OW
 // columns (stride MW): mid[r][c] = colf[c] * sum_k g[k] in[r][c + k], in
 // jobs of 8 consecutive outputs
@@ -477,30 +513,12 @@ template <int R, int OW, int SWP, int MW, int SH>
 __device__ __forceinline__ void hpass_blocked(const float* __restrict__ in, float* __restrict__ mid,
                                               const float* __restrict__ colf, const Taps* __restrict__ T) {
     constexpr int RUNS = OW / kRun;
-    constexpr int NV = (kRun + 2 * R + 3) / 4;  // float4 loads per run
     static_assert(OW % kRun == 0 && SWP % 4 == 0 && MW % 4 == 0, "vector rows");
-    static_assert((RUNS - 1) * kRun + 4 * NV <= SWP, "run loads stay inside the padded row");
+    static_assert((RUNS - 1) * kRun + 4 * run_vectors<R>() <= SWP, "run loads stay inside the padded row");
     for (int job = threadIdx.x; job < SH * RUNS; job += kThreads) {
         const int r = job / RUNS, c0 = (job - r * RUNS) * kRun;
-        const float4* p = reinterpret_cast<const float4*>(in + r * SWP + c0);
-        float v[4 * NV];
-#pragma unroll
-        for (int q = 0; q < NV; q++) {
-            const float4 t = p[q];
-            v[4 * q] = t.x;
-            v[4 * q + 1] = t.y;
-            v[4 * q + 2] = t.z;
-            v[4 * q + 3] = t.w;
-        }
         float acc[kRun];
-#pragma unroll
-        for (int i = 0; i < kRun; i++) acc[i] = 0.0f;
-#pragma unroll
-        for (int k = 0; k <= 2 * R; k++) {
-            const float g = T->g[k];
-#pragma unroll
-            for (int i = 0; i < kRun; i++) acc[i] = fmaf(g, v[i + k], acc[i]);
-        }
+        hrun<R>(in + r * SWP + c0, T, acc);
         float4* o = reinterpret_cast<float4*>(mid + r * MW + c0);
         o[0] = make_float4(acc[0] * colf[c0], acc[1] * colf[c0 + 1], acc[2] * colf[c0 + 2], acc[3] * colf[c0 + 3]);
         o[1] = make_float4(acc[4] * colf[c0 + 4], acc[5] * colf[c0 + 5], acc[6] * colf[c0 + 6],
@@ -526,27 +544,54 @@ __device__ __forceinline__ void vpass_block(const float* __restrict__ mid, const
     }
 }
 
-// Blur plane src of the tile into out[j] (rows kRows * ty + j); ctr[j]
-// receives the unblurred value at the same pixel. The LDS is free for reuse
-// after the caller's next __syncthreads (the next blur_tile begins with one).
+// Output pixels of a blur-tile thread: row ty of the tile, columns
+// tx .. tx + kRun - 1.
+struct RunPos {
+    int tx, ty;
+};
+__device__ __forceinline__ RunPos run_pos() { return {(int)(threadIdx.x % kRuns) * kRun, (int)threadIdx.x / kRuns}; }
+
+// The kRun values of a thread into a plane at row gy, columns gx .. gx + 7:
+// two 16-byte stores. gx is a multiple of 8 below the width and rows are
+// padded to a multiple of 64 floats, so the run stays inside its row; values
+// past the width land in the row padding, which no kernel reads.
+__device__ __forceinline__ void store_run(float* plane, int pitch, int gx, int gy, const float (&v)[kRun]) {
+    float4* o = reinterpret_cast<float4*>(plane + (int64_t)gy * pitch + gx);
+    o[0] = make_float4(v[0], v[1], v[2], v[3]);
+    o[1] = make_float4(v[4], v[5], v[6], v[7]);
+}
+
+// Blur plane src of the tile into out[i] (the thread's run_pos() pixels);
+// ctr[i] receives the unblurred value at the same pixel. The LDS is free for
+// reuse after the caller's next __syncthreads (the next blur_tile begins
+// with one).
 template <int R>
 __device__ __forceinline__ void blur_tile(BlurLds<R>& L, const float* __restrict__ src, const Taps* __restrict__ T,
-                                          int w, int h, int pitch, int ox0, int oy0, float (&out)[kRows],
-                                          float (&ctr)[kRows]) {
+                                          int w, int h, int pitch, int ox0, int oy0, float (&out)[kRun],
+                                          float (&ctr)[kRun]) {
     constexpr int SW = BlurLds<R>::SW, SWP = BlurLds<R>::SWP, SH = BlurLds<R>::SH;
-    const int tx = threadIdx.x & (kTW - 1), ty = threadIdx.x / kTW;
+    static_assert(SWP % 4 == 0 && (kRuns - 1) * kRun + 4 * run_vectors<R>() <= SWP,
+                  "row-job loads stay inside the padded row");
     __syncthreads();
     stage_padded<SW, SWP, SH>(L.in, src, w, h, pitch, ox0 - R, oy0 - R);
     __syncthreads();
-    constexpr int MW = BlurLds<R>::MW;
-    hpass_blocked<R, kTW, SWP, MW, SH>(L.in, L.mid, L.colf, T);
+    // column pass: job (rb, c) sums rows kRun rb .. kRun rb + kRun - 1 of staged
+    // column c from staged rows kRun rb .. kRun rb + kRun - 1 + 2R
+    for (int job = threadIdx.x; job < SW * (kTH / kRun); job += kThreads) {
+        const int rb = job / SW, c = job - rb * SW;
+        float o[kRun];
+        vpass_block<R, SWP, kRun>(L.in, T, rb * kRun, c, o);
 #pragma unroll
-    for (int j = 0; j < kRows; j++) ctr[j] = L.in[(kRows * ty + j + R) * SWP + tx + R];
+        for (int j = 0; j < kRun; j++) L.mid[(rb * kRun + j) * SWP + c] = o[j] * L.rowf[rb * kRun + j];
+    }
+    const RunPos p = run_pos();
+#pragma unroll
+    for (int i = 0; i < kRun; i++) ctr[i] = L.in[(p.ty + R) * SWP + p.tx + R + i];
     __syncthreads();
-    // column pass: 8 outputs of column tx from rows kRows * ty .. + kRows - 1 + 2R
-    vpass_block<R, MW, kRows>(L.mid, T, kRows * ty, tx, out);
+    // row pass: outputs tx .. tx + 7 of row ty from mid columns tx .. tx + 7 + 2R
+    hrun<R>(L.mid + p.ty * SWP + p.tx, T, out);
 #pragma unroll
-    for (int j = 0; j < kRows; j++) out[j] *= L.rowf[kRows * ty + j];
+    for (int i = 0; i < kRun; i++) out[i] *= L.colf[p.tx + i];
 }
 
 // sigma 1.2 blur of linear RGB + opsin dynamics (colors.hpp:50-91) -> XYB
@@ -559,19 +604,18 @@ __global__ __launch_bounds__(kThreads) void butter_opsin_kernel(Scale S, const T
     const int pair = ti.z >> 1, im = ti.z & 1;
     const int ox0 = ti.x * kTW, oy0 = ti.y * kTH;
     edge_factors<R, kTW, kTH>(L.colf, L.rowf, T, S.w, S.h, ox0, oy0);
-    float bl[3][kRows], ctr[3][kRows];
+    float bl[3][kRun], ctr[3][kRun];
 #pragma unroll
     for (int c = 0; c < 3; c++)
         blur_tile<R>(L, plane_of(S, pair, P_LIN + im * 3 + c), T, S.w, S.h, S.pitch, ox0, oy0, bl[c], ctr[c]);
-    const int tx = threadIdx.x & (kTW - 1), ty = threadIdx.x / kTW;
-    const int gx = ox0 + tx;
-    if (gx >= S.w) return;
+    const RunPos p = run_pos();
+    const int gx = ox0 + p.tx, gy = oy0 + p.ty;
+    if (gx >= S.w || gy >= S.h) return;
+    float x[kRun], y[kRun], z[kRun];
 #pragma unroll
-    for (int j = 0; j < kRows; j++) {
-        const int gy = oy0 + kRows * ty + j;
-        if (gy >= S.h) continue;
+    for (int i = 0; i < kRun; i++) {
         float b0, b1, b2;
-        opsin_absorbance(bl[0][j] * intensity, bl[1][j] * intensity, bl[2][j] * intensity, b0, b1, b2);
+        opsin_absorbance(bl[0][i] * intensity, bl[1][i] * intensity, bl[2][i] * intensity, b0, b1, b2);
         b0 = fmaxf(fmaxf(b0, 1.7557483643287353f), 1e-4f);
         b1 = fmaxf(fmaxf(b1, 1.7557483643287353f), 1e-4f);
         b2 = fmaxf(fmaxf(b2, 12.226454707163354f), 1e-4f);
@@ -579,15 +623,17 @@ __global__ __launch_bounds__(kThreads) void butter_opsin_kernel(Scale S, const T
         const float s1 = fmaxf(opsin_gamma(b1) / b1, 1e-4f);
         const float s2 = fmaxf(opsin_gamma(b2) / b2, 1e-4f);
         float o0, o1, o2;
-        opsin_absorbance(ctr[0][j] * intensity, ctr[1][j] * intensity, ctr[2][j] * intensity, o0, o1, o2);
+        opsin_absorbance(ctr[0][i] * intensity, ctr[1][i] * intensity, ctr[2][i] * intensity, o0, o1, o2);
         o0 = fmaxf(o0 * s0, 1.7557483643287353f);
         o1 = fmaxf(o1 * s1, 1.7557483643287353f);
         o2 = fmaxf(o2 * s2, 12.226454707163354f);
-        const int64_t g = (int64_t)gy * S.pitch + gx;
-        plane_of(S, pair, P_XYB + im * 3 + 0)[g] = o0 - o1;
-        plane_of(S, pair, P_XYB + im * 3 + 1)[g] = o0 + o1;
-        plane_of(S, pair, P_XYB + im * 3 + 2)[g] = o2;
+        x[i] = o0 - o1;
+        y[i] = o0 + o1;
+        z[i] = o2;
     }
+    store_run(plane_of(S, pair, P_XYB + im * 3 + 0), S.pitch, gx, gy, x);
+    store_run(plane_of(S, pair, P_XYB + im * 3 + 1), S.pitch, gx, gy, y);
+    store_run(plane_of(S, pair, P_XYB + im * 3 + 2), S.pitch, gx, gy, z);
 }
 
 // sigma 7.156 blur: low band (stored as values, XybLowFreqToVals) and the
@@ -599,33 +645,30 @@ __global__ __launch_bounds__(kThreads) void butter_low_kernel(Scale S, const Tap
     const TileIdx ti = tile_of_block();
     const int pair = ti.z >> 1, im = ti.z & 1;
     const int ox0 = ti.x * kTW, oy0 = ti.y * kTH;
-    const int tx = threadIdx.x & (kTW - 1), ty = threadIdx.x / kTW;
-    const int gx = ox0 + tx;
+    const RunPos p = run_pos();
+    const int gx = ox0 + p.tx, gy = oy0 + p.ty;
+    const bool ok = gx < S.w && gy < S.h;
     edge_factors<R, kTW, kTH>(L.colf, L.rowf, T, S.w, S.h, ox0, oy0);
-    float lf[3][kRows];
+    float lf[3][kRun];
 #pragma unroll
     for (int c = 0; c < 3; c++) {
-        float ctr[kRows];
+        float ctr[kRun], t[kRun];
         blur_tile<R>(L, plane_of(S, pair, P_XYB + im * 3 + c), T, S.w, S.h, S.pitch, ox0, oy0, lf[c], ctr);
-        float* t = plane_of(S, pair, P_T + im * 3 + c);
 #pragma unroll
-        for (int j = 0; j < kRows; j++) {
-            const int gy = oy0 + kRows * ty + j;
-            if (gx < S.w && gy < S.h) t[(int64_t)gy * S.pitch + gx] = ctr[j] - lf[c][j];
-        }
+        for (int i = 0; i < kRun; i++) t[i] = ctr[i] - lf[c][i];
+        if (ok) store_run(plane_of(S, pair, P_T + im * 3 + c), S.pitch, gx, gy, t);
     }
-    if (gx >= S.w) return;
+    if (!ok) return;
+    float v[3][kRun];
 #pragma unroll
-    for (int j = 0; j < kRows; j++) {
-        const int gy = oy0 + kRows * ty + j;
-        if (gy >= S.h) continue;
-        const int64_t g = (int64_t)gy * S.pitch + gx;
-        float b = lf[2][j] + -0.362267051518f * lf[1][j];
-        b = b * 49.87984651440f;
-        plane_of(S, pair, P_LFV + im * 3 + 0)[g] = lf[0][j] * 33.832837186260f;
-        plane_of(S, pair, P_LFV + im * 3 + 1)[g] = lf[1][j] * 14.458268100570f;
-        plane_of(S, pair, P_LFV + im * 3 + 2)[g] = b;
+    for (int i = 0; i < kRun; i++) {
+        float b = lf[2][i] + -0.362267051518f * lf[1][i];
+        v[0][i] = lf[0][i] * 33.832837186260f;
+        v[1][i] = lf[1][i] * 14.458268100570f;
+        v[2][i] = b * 49.87984651440f;
     }
+#pragma unroll
+    for (int c = 0; c < 3; c++) store_run(plane_of(S, pair, P_LFV + im * 3 + c), S.pitch, gx, gy, v[c]);
 }
 
 // sigma 3.225 blur of the residual: middle band after its range operation,
@@ -638,39 +681,35 @@ __global__ __launch_bounds__(kThreads) void butter_mid_kernel(Scale S, const Tap
     const TileIdx ti = tile_of_block();
     const int pair = ti.z >> 1, im = ti.z & 1;
     const int ox0 = ti.x * kTW, oy0 = ti.y * kTH;
-    const int tx = threadIdx.x & (kTW - 1), ty = threadIdx.x / kTW;
-    const int gx = ox0 + tx;
+    const RunPos p = run_pos();
+    const int gx = ox0 + p.tx, gy = oy0 + p.ty;
+    const bool ok = gx < S.w && gy < S.h;
     edge_factors<R, kTW, kTH>(L.colf, L.rowf, T, S.w, S.h, ox0, oy0);
-    float hfr[2][kRows];
+    float hfr[2][kRun];
     constexpr int kOrder[3] = {1, 0, 2};
 #pragma unroll
     for (int o = 0; o < 3; o++) {
         const int c = kOrder[o];
-        float mf[kRows], ctr[kRows];
+        float mf[kRun], ctr[kRun];
         blur_tile<R>(L, plane_of(S, pair, P_T + im * 3 + c), T, S.w, S.h, S.pitch, ox0, oy0, mf, ctr);
-        float* dst = plane_of(S, pair, P_MF + im * 3 + c);
 #pragma unroll
-        for (int j = 0; j < kRows; j++) {
-            float m = mf[j];
+        for (int i = 0; i < kRun; i++) {
+            const float m = mf[i];
             if (c < 2) {
-                hfr[c][j] = ctr[j] - m;
-                m = c == 0 ? remove_range(m, 0.29f) : amplify_range(m, 0.1f);
+                hfr[c][i] = ctr[i] - m;
+                mf[i] = c == 0 ? remove_range(m, 0.29f) : amplify_range(m, 0.1f);
             }
-            const int gy = oy0 + kRows * ty + j;
-            if (gx < S.w && gy < S.h) dst[(int64_t)gy * S.pitch + gx] = m;
         }
+        if (ok) store_run(plane_of(S, pair, P_MF + im * 3 + c), S.pitch, gx, gy, mf);
     }
-    if (gx >= S.w) return;
+    if (!ok) return;
 #pragma unroll
-    for (int j = 0; j < kRows; j++) {
-        const int gy = oy0 + kRows * ty + j;
-        if (gy >= S.h) continue;
-        const float s = 0.653020556257f, yw = 46.0f, yv = hfr[1][j];
-        const float x = hfr[0][j] * (s + (yw * (1.0f - s)) / (yw + yv * yv));
-        const int64_t g = (int64_t)gy * S.pitch + gx;
-        plane_of(S, pair, P_HFR + im * 2 + 0)[g] = x;
-        plane_of(S, pair, P_HFR + im * 2 + 1)[g] = yv;
+    for (int i = 0; i < kRun; i++) {
+        const float s = 0.653020556257f, yw = 46.0f, yv = hfr[1][i];
+        hfr[0][i] = hfr[0][i] * (s + (yw * (1.0f - s)) / (yw + yv * yv));
     }
+    store_run(plane_of(S, pair, P_HFR + im * 2 + 0), S.pitch, gx, gy, hfr[0]);
+    store_run(plane_of(S, pair, P_HFR + im * 2 + 1), S.pitch, gx, gy, hfr[1]);
 }
 
 // sigma 1.564 blur of the high residual: high and ultra-high bands
@@ -682,18 +721,17 @@ __global__ __launch_bounds__(kThreads) void butter_high_kernel(Scale S, const Ta
     const TileIdx ti = tile_of_block();
     const int pair = ti.z >> 1, im = ti.z & 1;
     const int ox0 = ti.x * kTW, oy0 = ti.y * kTH;
-    const int tx = threadIdx.x & (kTW - 1), ty = threadIdx.x / kTW;
-    const int gx = ox0 + tx;
+    const RunPos p = run_pos();
+    const int gx = ox0 + p.tx, gy = oy0 + p.ty;
+    const bool ok = gx < S.w && gy < S.h;
     edge_factors<R, kTW, kTH>(L.colf, L.rowf, T, S.w, S.h, ox0, oy0);
 #pragma unroll
     for (int c = 0; c < 2; c++) {
-        float hf[kRows], ctr[kRows];
-        blur_tile<R>(L, plane_of(S, pair, P_HFR + im * 2 + c), T, S.w, S.h, S.pitch, ox0, oy0, hf, ctr);
-        float* dh = plane_of(S, pair, P_HF + im * 2 + c);
-        float* du = plane_of(S, pair, P_UHF + im * 2 + c);
+        float hf[kRun], uhf[kRun];
+        blur_tile<R>(L, plane_of(S, pair, P_HFR + im * 2 + c), T, S.w, S.h, S.pitch, ox0, oy0, hf, uhf);
 #pragma unroll
-        for (int j = 0; j < kRows; j++) {
-            float hv = hf[j], uv = ctr[j];
+        for (int i = 0; i < kRun; i++) {
+            float hv = hf[i], uv = uhf[i];
             if (c == 0) {
                 uv -= hv;
                 hv = remove_range(hv, 1.5f);
@@ -706,11 +744,12 @@ __global__ __launch_bounds__(kThreads) void butter_high_kernel(Scale S, const Ta
                 hv *= 2.155f;
                 hv = amplify_range(hv, 0.132f);
             }
-            const int gy = oy0 + kRows * ty + j;
-            if (gx < S.w && gy < S.h) {
-                dh[(int64_t)gy * S.pitch + gx] = hv;
-                du[(int64_t)gy * S.pitch + gx] = uv;
-            }
+            hf[i] = hv;
+            uhf[i] = uv;
+        }
+        if (ok) {
+            store_run(plane_of(S, pair, P_HF + im * 2 + c), S.pitch, gx, gy, hf);
+            store_run(plane_of(S, pair, P_UHF + im * 2 + c), S.pitch, gx, gy, uhf);
         }
     }
 }

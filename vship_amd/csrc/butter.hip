@@ -82,6 +82,29 @@ __device__ __forceinline__ float* plane_of(const Scale& S, int pair, int p) {
     return S.base + (int64_t)pair * S.pair + (int64_t)p * S.plane;
 }
 
+// plane_of for a pair and plane that are the same for the whole workgroup,
+// held in scalar registers: without the read-first-lane the compiler folds
+// the plane offset into every per-lane address as 64-bit vector arithmetic
+// (the offset, not the pointer, goes through it, so that the pointer stays
+// derived from the kernel argument and its loads stay global, not flat)
+__device__ __forceinline__ float* plane_u(const Scale& S, int pair, int p) {
+    const uint64_t v = (uint64_t)((int64_t)pair * S.pair + (int64_t)p * S.plane);
+    const uint32_t lo = __builtin_amdgcn_readfirstlane((uint32_t)v);
+    const uint32_t hi = __builtin_amdgcn_readfirstlane((uint32_t)(v >> 32));
+    return S.base + (int64_t)(((uint64_t)hi << 32) | lo);
+}
+
+// Element off of a plane whose base is the same for the whole workgroup: a
+// 32-bit byte offset from that base, which the compiler emits as the
+// scalar-base + 32-bit vector-offset address form, with no 64-bit vector
+// address arithmetic. Planes hold fewer than 2^30 floats (the engine checks).
+__device__ __forceinline__ float& at(float* base, int off) {
+    return *reinterpret_cast<float*>(reinterpret_cast<char*>(base) + (uint64_t)((uint32_t)off * 4u));
+}
+__device__ __forceinline__ float at(const float* base, int off) {
+    return *reinterpret_cast<const float*>(reinterpret_cast<const char*>(base) + (uint64_t)((uint32_t)off * 4u));
+}
+
 // Tile of this workgroup, XCD-aware: hardware workgroup b (in launch order
 // x fastest, then y, then z) runs on XCD b % 8, so tiles are numbered such
 // that each XCD takes one contiguous range of tile rows, and the halos that
@@ -93,10 +116,13 @@ __device__ __forceinline__ TileIdx tile_of_block() {
     const int nx = gridDim.x, ny = gridDim.y, nz = gridDim.z;
     const int b = blockIdx.x + nx * (blockIdx.y + ny * blockIdx.z);
     const int v = xcd_block(b, nx * ny * nz);
+    // the divisions run on the vector unit; the tile is the workgroup's, so
+    // it is moved to scalar registers, and the plane addresses built from it
+    // stay scalar
     TileIdx t;
-    t.x = v % nx;
-    t.y = (v / nx) % ny;
-    t.z = v / (nx * ny);
+    t.x = __builtin_amdgcn_readfirstlane(v % nx);
+    t.y = __builtin_amdgcn_readfirstlane((v / nx) % ny);
+    t.z = __builtin_amdgcn_readfirstlane(v / (nx * ny));
     return t;
 }
 
@@ -186,19 +212,16 @@ __device__ __forceinline__ float mul_rounded(float a, float b) {
     return a * b;
 }
 
+// Insert v into the three smallest values m0 <= m1 <= m2 (StoreMin3,
+// maskPsycho.hpp:42-56), as a min/max network: the same three values for any
+// non-NaN v as the reference's branches, which the compiler would otherwise
+// lower to a private array written at a computed slot.
 __device__ __forceinline__ void store_min3(float v, float& m0, float& m1, float& m2) {
-    if (v < m2) {
-        if (v < m0) {
-            m2 = m1;
-            m1 = m0;
-            m0 = v;
-        } else if (v < m1) {
-            m2 = m1;
-            m1 = v;
-        } else {
-            m2 = v;
-        }
-    }
+    const float n2 = fminf(m2, fmaxf(m1, v));
+    const float n1 = fminf(m1, fmaxf(m0, v));
+    m0 = fminf(m0, v);
+    m1 = n1;
+    m2 = n2;
 }
 
 // ---------------------------------------------------------------------------
@@ -405,14 +428,50 @@ struct alignas(16) BlurLds {
     float rowf[kTH];
 };
 
-// in[r][c] = src(gx0 + c, gy0 + r) for c < SW, 0 outside the image and in
-// the row padding. Thread (tx, ty) fills columns tx, tx + 64, ... of rows
-// ty, ty + 4, ...
+// Staging of a region [gx0, gx0 + SW) x [gy0, gy0 + SH) of a plane into
+// LDS rows of SWP floats, 0 outside the image and in the row padding. Thread
+// (tx, ty) = (t % 64, t / 64) takes columns tx, tx + 64, ... of rows ty,
+// ty + 4, ...; ty is the wave's, so the row part of an address is scalar.
+//
+// Every load is issued unconditionally, at the address clamped into the
+// image (a 32-bit offset from the plane base, which the caller holds in
+// scalar registers, plane_u), and stored as loaded; in a region that crosses
+// an image edge each thread then overwrites its own elements that lie
+// outside with 0. (Masking each value as it is loaded makes the compiler
+// sink the load into a branch with its own wait, which serialises the loads.)
 template <int SW, int SWP, int SH>
 struct StageRegs {
     static constexpr int NC = (SWP + 63) / 64, NR = (SH + 3) / 4;  // column chunks, rows per thread
     float v[NC][NR];
 };
+
+__device__ __forceinline__ int stage_ty() { return __builtin_amdgcn_readfirstlane(threadIdx.x >> 6); }
+
+__device__ __forceinline__ float stage_at(const float* __restrict__ src, int pitch, int h, int gy, int gxc) {
+    return at(src, min(max(gy, 0), h - 1) * pitch + gxc);
+}
+
+// zero this thread's staged elements that lie outside the image or the
+// region (only called for a region that crosses an image edge)
+template <int SW, int SWP, int SH>
+__device__ __forceinline__ void stage_clear_outside(float* __restrict__ in, int w, int h, int gx0, int gy0) {
+    constexpr int NC = (SWP + 63) / 64, NR = (SH + 3) / 4;
+    const int tx = threadIdx.x & 63, ty = stage_ty();
+#pragma unroll
+    for (int cc = 0; cc < NC; cc++) {
+        const int c = cc * 64 + tx, gx = gx0 + c;
+        const bool col_out = c >= SW || gx < 0 || gx >= w;
+#pragma unroll
+        for (int k = 0; k < NR; k++) {
+            const int r = ty + 4 * k, gy = gy0 + r;
+            if (r < SH && c < SWP && (col_out || gy < 0 || gy >= h)) in[r * SWP + c] = 0.0f;
+        }
+    }
+}
+
+__device__ __forceinline__ bool crosses_edge(int gx0, int gy0, int sw, int sh, int w, int h) {
+    return gx0 < 0 || gy0 < 0 || gx0 + sw > w || gy0 + sh > h;
+}
 
 // the loads of stage_padded, into registers (issued together, so that their
 // latency overlaps whatever the caller does before stage_store)
@@ -420,25 +479,21 @@ template <int SW, int SWP, int SH>
 __device__ __forceinline__ void stage_load(StageRegs<SW, SWP, SH>& S, const float* __restrict__ src, int w, int h,
                                            int pitch, int gx0, int gy0) {
     using SR = StageRegs<SW, SWP, SH>;
-    const int tx = threadIdx.x & 63, ty = threadIdx.x >> 6;
+    const int tx = threadIdx.x & 63, ty = stage_ty();
 #pragma unroll
     for (int cc = 0; cc < SR::NC; cc++) {
-        const int c = cc * 64 + tx;
-        const int gx = gx0 + c;
-        const bool col_ok = c < SW && gx >= 0 && gx < w;
-        const float* col = src + gx;
+        const int gxc = min(max(gx0 + cc * 64 + tx, 0), w - 1);
 #pragma unroll
-        for (int k = 0; k < SR::NR; k++) {
-            const int gy = gy0 + ty + 4 * k;
-            S.v[cc][k] = (col_ok && ty + 4 * k < SH && gy >= 0 && gy < h) ? col[(int64_t)gy * pitch] : 0.0f;
-        }
+        for (int k = 0; k < SR::NR; k++) S.v[cc][k] = stage_at(src, pitch, h, gy0 + ty + 4 * k, gxc);
     }
 }
 
+// the stores of stage_padded, for the same region as stage_load
 template <int SW, int SWP, int SH>
-__device__ __forceinline__ void stage_store(const StageRegs<SW, SWP, SH>& S, float* __restrict__ in) {
+__device__ __forceinline__ void stage_store(const StageRegs<SW, SWP, SH>& S, float* __restrict__ in, int w, int h,
+                                            int gx0, int gy0) {
     using SR = StageRegs<SW, SWP, SH>;
-    const int tx = threadIdx.x & 63, ty = threadIdx.x >> 6;
+    const int tx = threadIdx.x & 63, ty = stage_ty();
 #pragma unroll
     for (int cc = 0; cc < SR::NC; cc++) {
         const int c = cc * 64 + tx;
@@ -447,32 +502,31 @@ __device__ __forceinline__ void stage_store(const StageRegs<SW, SWP, SH>& S, flo
         for (int k = 0; k < SR::NR; k++)
             if (ty + 4 * k < SH) in[(ty + 4 * k) * SWP + c] = S.v[cc][k];
     }
+    if (crosses_edge(gx0, gy0, SW, SH, w, h)) stage_clear_outside<SW, SWP, SH>(in, w, h, gx0, gy0);
 }
 
+// in[r][c] = src(gx0 + c, gy0 + r) for c < SW, 0 outside the image and in
+// the row padding
 template <int SW, int SWP, int SH>
 __device__ __forceinline__ void stage_padded(float* __restrict__ in, const float* __restrict__ src, int w, int h,
                                              int pitch, int gx0, int gy0) {
     // column chunk by column chunk, all loads of a chunk before its stores
     constexpr int NR = (SH + 3) / 4;
-    const int tx = threadIdx.x & 63, ty = threadIdx.x >> 6;
+    const int tx = threadIdx.x & 63, ty = stage_ty();
 #pragma unroll
     for (int c0 = 0; c0 < SWP; c0 += 64) {
         const int c = c0 + tx;
-        const int gx = gx0 + c;
-        const bool col_ok = c < SWP && c < SW && gx >= 0 && gx < w;
-        const float* col = src + gx;
+        const int gxc = min(max(gx0 + c, 0), w - 1);
         float v[NR];
 #pragma unroll
-        for (int k = 0; k < NR; k++) {
-            const int gy = gy0 + ty + 4 * k;
-            v[k] = (col_ok && ty + 4 * k < SH && gy >= 0 && gy < h) ? col[(int64_t)gy * pitch] : 0.0f;
-        }
+        for (int k = 0; k < NR; k++) v[k] = stage_at(src, pitch, h, gy0 + ty + 4 * k, gxc);
         if (c < SWP) {
 #pragma unroll
             for (int k = 0; k < NR; k++)
                 if (ty + 4 * k < SH) in[(ty + 4 * k) * SWP + c] = v[k];
         }
     }
+    if (crosses_edge(gx0, gy0, SW, SH, w, h)) stage_clear_outside<SW, SWP, SH>(in, w, h, gx0, gy0);
 }
 
 // float4 loads of one row job: its kRun outputs read kRun + 2R inputs
@@ -556,7 +610,7 @@ __device__ __forceinline__ RunPos run_pos() { return {(int)(threadIdx.x % kRuns)
 // padded to a multiple of 64 floats, so the run stays inside its row; values
 // past the width land in the row padding, which no kernel reads.
 __device__ __forceinline__ void store_run(float* plane, int pitch, int gx, int gy, const float (&v)[kRun]) {
-    float4* o = reinterpret_cast<float4*>(plane + (int64_t)gy * pitch + gx);
+    float4* o = reinterpret_cast<float4*>(&at(plane, gy * pitch + gx));
     o[0] = make_float4(v[0], v[1], v[2], v[3]);
     o[1] = make_float4(v[4], v[5], v[6], v[7]);
 }
@@ -607,7 +661,7 @@ __global__ __launch_bounds__(kThreads) void butter_opsin_kernel(Scale S, const T
     float bl[3][kRun], ctr[3][kRun];
 #pragma unroll
     for (int c = 0; c < 3; c++)
-        blur_tile<R>(L, plane_of(S, pair, P_LIN + im * 3 + c), T, S.w, S.h, S.pitch, ox0, oy0, bl[c], ctr[c]);
+        blur_tile<R>(L, plane_u(S, pair, P_LIN + im * 3 + c), T, S.w, S.h, S.pitch, ox0, oy0, bl[c], ctr[c]);
     const RunPos p = run_pos();
     const int gx = ox0 + p.tx, gy = oy0 + p.ty;
     if (gx >= S.w || gy >= S.h) return;
@@ -631,9 +685,9 @@ __global__ __launch_bounds__(kThreads) void butter_opsin_kernel(Scale S, const T
         y[i] = o0 + o1;
         z[i] = o2;
     }
-    store_run(plane_of(S, pair, P_XYB + im * 3 + 0), S.pitch, gx, gy, x);
-    store_run(plane_of(S, pair, P_XYB + im * 3 + 1), S.pitch, gx, gy, y);
-    store_run(plane_of(S, pair, P_XYB + im * 3 + 2), S.pitch, gx, gy, z);
+    store_run(plane_u(S, pair, P_XYB + im * 3 + 0), S.pitch, gx, gy, x);
+    store_run(plane_u(S, pair, P_XYB + im * 3 + 1), S.pitch, gx, gy, y);
+    store_run(plane_u(S, pair, P_XYB + im * 3 + 2), S.pitch, gx, gy, z);
 }
 
 // sigma 7.156 blur: low band (stored as values, XybLowFreqToVals) and the
@@ -653,10 +707,10 @@ __global__ __launch_bounds__(kThreads) void butter_low_kernel(Scale S, const Tap
 #pragma unroll
     for (int c = 0; c < 3; c++) {
         float ctr[kRun], t[kRun];
-        blur_tile<R>(L, plane_of(S, pair, P_XYB + im * 3 + c), T, S.w, S.h, S.pitch, ox0, oy0, lf[c], ctr);
+        blur_tile<R>(L, plane_u(S, pair, P_XYB + im * 3 + c), T, S.w, S.h, S.pitch, ox0, oy0, lf[c], ctr);
 #pragma unroll
         for (int i = 0; i < kRun; i++) t[i] = ctr[i] - lf[c][i];
-        if (ok) store_run(plane_of(S, pair, P_T + im * 3 + c), S.pitch, gx, gy, t);
+        if (ok) store_run(plane_u(S, pair, P_T + im * 3 + c), S.pitch, gx, gy, t);
     }
     if (!ok) return;
     float v[3][kRun];
@@ -668,7 +722,7 @@ __global__ __launch_bounds__(kThreads) void butter_low_kernel(Scale S, const Tap
         v[2][i] = b * 49.87984651440f;
     }
 #pragma unroll
-    for (int c = 0; c < 3; c++) store_run(plane_of(S, pair, P_LFV + im * 3 + c), S.pitch, gx, gy, v[c]);
+    for (int c = 0; c < 3; c++) store_run(plane_u(S, pair, P_LFV + im * 3 + c), S.pitch, gx, gy, v[c]);
 }
 
 // sigma 3.225 blur of the residual: middle band after its range operation,
@@ -691,7 +745,7 @@ __global__ __launch_bounds__(kThreads) void butter_mid_kernel(Scale S, const Tap
     for (int o = 0; o < 3; o++) {
         const int c = kOrder[o];
         float mf[kRun], ctr[kRun];
-        blur_tile<R>(L, plane_of(S, pair, P_T + im * 3 + c), T, S.w, S.h, S.pitch, ox0, oy0, mf, ctr);
+        blur_tile<R>(L, plane_u(S, pair, P_T + im * 3 + c), T, S.w, S.h, S.pitch, ox0, oy0, mf, ctr);
 #pragma unroll
         for (int i = 0; i < kRun; i++) {
             const float m = mf[i];
@@ -700,7 +754,7 @@ __global__ __launch_bounds__(kThreads) void butter_mid_kernel(Scale S, const Tap
                 mf[i] = c == 0 ? remove_range(m, 0.29f) : amplify_range(m, 0.1f);
             }
         }
-        if (ok) store_run(plane_of(S, pair, P_MF + im * 3 + c), S.pitch, gx, gy, mf);
+        if (ok) store_run(plane_u(S, pair, P_MF + im * 3 + c), S.pitch, gx, gy, mf);
     }
     if (!ok) return;
 #pragma unroll
@@ -708,8 +762,8 @@ __global__ __launch_bounds__(kThreads) void butter_mid_kernel(Scale S, const Tap
         const float s = 0.653020556257f, yw = 46.0f, yv = hfr[1][i];
         hfr[0][i] = hfr[0][i] * (s + (yw * (1.0f - s)) / (yw + yv * yv));
     }
-    store_run(plane_of(S, pair, P_HFR + im * 2 + 0), S.pitch, gx, gy, hfr[0]);
-    store_run(plane_of(S, pair, P_HFR + im * 2 + 1), S.pitch, gx, gy, hfr[1]);
+    store_run(plane_u(S, pair, P_HFR + im * 2 + 0), S.pitch, gx, gy, hfr[0]);
+    store_run(plane_u(S, pair, P_HFR + im * 2 + 1), S.pitch, gx, gy, hfr[1]);
 }
 
 // sigma 1.564 blur of the high residual: high and ultra-high bands
@@ -728,7 +782,7 @@ __global__ __launch_bounds__(kThreads) void butter_high_kernel(Scale S, const Ta
 #pragma unroll
     for (int c = 0; c < 2; c++) {
         float hf[kRun], uhf[kRun];
-        blur_tile<R>(L, plane_of(S, pair, P_HFR + im * 2 + c), T, S.w, S.h, S.pitch, ox0, oy0, hf, uhf);
+        blur_tile<R>(L, plane_u(S, pair, P_HFR + im * 2 + c), T, S.w, S.h, S.pitch, ox0, oy0, hf, uhf);
 #pragma unroll
         for (int i = 0; i < kRun; i++) {
             float hv = hf[i], uv = uhf[i];
@@ -748,8 +802,8 @@ __global__ __launch_bounds__(kThreads) void butter_high_kernel(Scale S, const Ta
             uhf[i] = uv;
         }
         if (ok) {
-            store_run(plane_of(S, pair, P_HF + im * 2 + c), S.pitch, gx, gy, hf);
-            store_run(plane_of(S, pair, P_UHF + im * 2 + c), S.pitch, gx, gy, uhf);
+            store_run(plane_u(S, pair, P_HF + im * 2 + c), S.pitch, gx, gy, hf);
+            store_run(plane_u(S, pair, P_UHF + im * 2 + c), S.pitch, gx, gy, uhf);
         }
     }
 }
@@ -844,29 +898,24 @@ struct MaltaTerm {
     float n2gt, n2lt, norm1;
 };
 
-// The two weights share one reciprocal of norm1 + |a|/2 + |b|/2 (the
-// reference divides twice; the results differ by rounding only).
+// The two weights share one hardware reciprocal (1 ulp) of
+// norm1 + |a|/2 + |b|/2; the reference divides twice, and the results differ
+// by rounding only.
+//
+// The impact term is formed without branches: with b mirrored to bm = -b when
+// a < 0, the reference's two cases (maltaDiff.hpp:553-572) are the one case
+// a >= 0, and too_small - bm, bm - too_big are exactly its b + too_small,
+// -b - too_big (negation is exact), so the value is bit-identical.
 __device__ __forceinline__ float malta_scaled_diff(float a, float b, const MaltaTerm& M) {
-    const float absval = 0.5f * fabsf(a) + 0.5f * fabsf(b);
-    const float diff = a - b;
-    const float inv = 1.0f / (M.norm1 + absval);
-    const float scaler = M.n2gt * inv;
-    float v = scaler * diff;
-    const float scaler2 = M.n2lt * inv;
     const float fabs0 = fabsf(a);
+    const float absval = 0.5f * fabs0 + 0.5f * fabsf(b);
+    const float diff = a - b;
+    const float inv = __builtin_amdgcn_rcpf(M.norm1 + absval);
+    const float v = (M.n2gt * inv) * diff;
+    const float scaler2 = M.n2lt * inv;
     const float too_small = 0.55f * fabs0, too_big = 1.05f * fabs0;
-    float impact = 0.0f;
-    if (a < 0.0f) {
-        if (b > -too_small)
-            impact = b + too_small;
-        else if (b < -too_big)
-            impact = -b - too_big;
-    } else {
-        if (b < too_small)
-            impact = -b + too_small;
-        else if (b > too_big)
-            impact = b - too_big;
-    }
+    const float bm = a < 0.0f ? -b : b;
+    float impact = bm < too_small ? too_small - bm : (bm > too_big ? bm - too_big : 0.0f);
     impact *= scaler2;
     return diff < 0.0f ? v - impact : v + impact;
 }
@@ -876,23 +925,15 @@ struct DiffParams {
     float asym_gt[2], asym_lt[2];
 };
 
+// asymmetric L2 term (simplerdiff.hpp:60-90), its cases folded as in
+// malta_scaled_diff (bit-identical)
 __device__ __forceinline__ float l2_asym(float a, float b, float wgt, float wlt) {
     const float diff = a - b;
     float r = wgt * diff * diff;
     const float fabs0 = fabsf(a);
     const float too_small = 0.4f * fabs0, too_big = fabs0;
-    float v = 0.0f;
-    if (a < 0.0f) {
-        if (b > -too_small)
-            v = b + too_small;
-        else if (b < -too_big)
-            v = -b - too_big;
-    } else {
-        if (b < too_small)
-            v = too_small - b;
-        else if (b > too_big)
-            v = b - too_big;
-    }
+    const float bm = a < 0.0f ? -b : b;
+    const float v = bm < too_small ? too_small - bm : (bm > too_big ? bm - too_big : 0.0f);
     return r + wlt * v * v;
 }
 
@@ -913,10 +954,12 @@ __global__ __launch_bounds__(kThreads) void butter_diff_kernel(Scale S, DiffPara
     // the previous term they would hold registers across its line sums)
 #pragma unroll 1
     for (int t = 0; t < 6; t++) {
-        const float* a = plane_of(S, pair, src0[t]);
-        const float* b = plane_of(S, pair, src1[t]);
+        const float* a = plane_u(S, pair, src0[t]);
+        const float* b = plane_u(S, pair, src1[t]);
         // stage the scaled differences of the tile and its halo: all of this
         // thread's loads are issued before the first difference is formed
+        // (addresses clamped into the image, so that every load is issued
+        // unconditionally; values outside it are replaced by 0)
         constexpr int NE = (kDSW * kDSH + kThreads - 1) / kThreads;
         float va[NE], vb[NE];
 #pragma unroll
@@ -925,9 +968,10 @@ __global__ __launch_bounds__(kThreads) void butter_diff_kernel(Scale S, DiffPara
             const int r = i / kDSW, c = i - r * kDSW;
             const int gx = ox0 - kDHalo + c, gy = oy0 - kDHalo + r;
             const bool ok = i < kDSW * kDSH && gx >= 0 && gx < w && gy >= 0 && gy < h;
-            const int64_t g = (int64_t)gy * S.pitch + gx;
-            va[k] = ok ? a[g] : 0.0f;
-            vb[k] = ok ? b[g] : 0.0f;
+            const int g = min(max(gy, 0), h - 1) * S.pitch + min(max(gx, 0), w - 1);
+            const float x = at(a, g), y = at(b, g);
+            va[k] = ok ? x : 0.0f;
+            vb[k] = ok ? y : 0.0f;
         }
         __syncthreads();
 #pragma unroll
@@ -948,15 +992,15 @@ __global__ __launch_bounds__(kThreads) void butter_diff_kernel(Scale S, DiffPara
     for (int j = 0; j < kDRows; j++) {
         const int gy = oy0 + kDRows * ty + j;
         if (gy >= h) continue;
-        const int64_t g = (int64_t)gy * S.pitch + gx;
+        const int g = gy * S.pitch + gx;
         float acc = ac[j];
         float hf[2][2], uhf[2][2];
 #pragma unroll
         for (int im = 0; im < 2; im++)
 #pragma unroll
             for (int c = 0; c < 2; c++) {
-                hf[im][c] = plane_of(S, pair, P_HF + im * 2 + c)[g];
-                uhf[im][c] = plane_of(S, pair, P_UHF + im * 2 + c)[g];
+                hf[im][c] = at(plane_u(S, pair, P_HF + im * 2 + c), g);
+                uhf[im][c] = at(plane_u(S, pair, P_UHF + im * 2 + c), g);
             }
         acc += l2_asym(hf[0][0], hf[1][0], D.asym_gt[0], D.asym_lt[0]);
         acc += l2_asym(hf[0][1], hf[1][1], D.asym_gt[1], D.asym_lt[1]);
@@ -965,15 +1009,15 @@ __global__ __launch_bounds__(kThreads) void butter_diff_kernel(Scale S, DiffPara
         float dc = 0.0f;
 #pragma unroll
         for (int c = 0; c < 3; c++) {
-            const float dm = plane_of(S, pair, P_MF + c)[g] - plane_of(S, pair, P_MF + 3 + c)[g];
+            const float dm = at(plane_u(S, pair, P_MF + c), g) - at(plane_u(S, pair, P_MF + 3 + c), g);
             acc += wmf[c] * dm * dm;
-            const float dl = plane_of(S, pair, P_LFV + c)[g] - plane_of(S, pair, P_LFV + 3 + c)[g];
+            const float dl = at(plane_u(S, pair, P_LFV + c), g) - at(plane_u(S, pair, P_LFV + 3 + c), g);
             dc += wlf[c] * dl * dl;
         }
-        plane_of(S, pair, P_ACS)[g] = acc;
-        plane_of(S, pair, P_DCS)[g] = dc;
-        plane_of(S, pair, P_M0)[g] = mask_input(hf[0][0], uhf[0][0], hf[0][1], uhf[0][1]);
-        plane_of(S, pair, P_M1)[g] = mask_input(hf[1][0], uhf[1][0], hf[1][1], uhf[1][1]);
+        at(plane_u(S, pair, P_ACS), g) = acc;
+        at(plane_u(S, pair, P_DCS), g) = dc;
+        at(plane_u(S, pair, P_M0), g) = mask_input(hf[0][0], uhf[0][0], hf[0][1], uhf[0][1]);
+        at(plane_u(S, pair, P_M1), g) = mask_input(hf[1][0], uhf[1][0], hf[1][1], uhf[1][1]);
     }
 }
 
@@ -1051,19 +1095,19 @@ __global__ __launch_bounds__(kThreads) void butter_mask_kernel(Scale S, const Ta
     for (int j = 0; j < kMRows; j++) {
         const int gy = oy0 + kMRows * ty + j;
         const bool ok = gx < w && gy < h;
-        const int64_t g = (int64_t)gy * S.pitch + gx;
-        acs_in[j] = ok ? plane_of(S, pair, P_ACS)[g] : 0.0f;
-        dcs_in[j] = ok ? plane_of(S, pair, P_DCS)[g] : 0.0f;
+        const int g = gy * S.pitch + gx;
+        acs_in[j] = ok ? at(plane_u(S, pair, P_ACS), g) : 0.0f;
+        dcs_in[j] = ok ? at(plane_u(S, pair, P_DCS), g) : 0.0f;
         half_in[j] = (FULL && ok) ? N.half_map_base[(int64_t)pair * N.half_pair + (int64_t)(gy / 2) * N.half_pitch + gx / 2]
                                   : 0.0f;
     }
     // blurred reference mask on the 40 x 40 region
-    stage_padded<M::SW, M::SWP, M::SW>(L.in, plane_of(S, pair, P_M0), w, h, S.pitch, ox0 - kMF - R, oy0 - kMF - R);
+    stage_padded<M::SW, M::SWP, M::SW>(L.in, plane_u(S, pair, P_M0), w, h, S.pitch, ox0 - kMF - R, oy0 - kMF - R);
     __syncthreads();
     hpass_blocked<R, kMB, M::SWP, M::MW, M::SW>(L.in, L.mid, L.colf, T);
     // the distorted mask's loads, in flight during the column pass
     StageRegs<M::SW1, M::SWP1, M::SW1> m1_regs;
-    stage_load(m1_regs, plane_of(S, pair, P_M1), w, h, S.pitch, ox0 - R, oy0 - R);
+    stage_load(m1_regs, plane_u(S, pair, P_M1), w, h, S.pitch, ox0 - R, oy0 - R);
     __syncthreads();
     for (int job = threadIdx.x; job < kMB * (kMB / 8); job += kThreads) {
         const int c = job % kMB, r0 = (job / kMB) * 8;
@@ -1074,7 +1118,7 @@ __global__ __launch_bounds__(kThreads) void butter_mask_kernel(Scale S, const Ta
     }
     __syncthreads();
     // blurred distorted mask on the 32 x 32 tile
-    stage_store(m1_regs, L.in);
+    stage_store(m1_regs, L.in, w, h, ox0 - R, oy0 - R);
     __syncthreads();
     hpass_blocked<R, kMT, M::SWP1, M::MW1, M::SW1>(L.in, L.mid, L.colf + kMF, T);
     __syncthreads();
@@ -1106,13 +1150,13 @@ __global__ __launch_bounds__(kThreads) void butter_mask_kernel(Scale S, const Ta
         if (gy >= kMErode) store_min3(bc[-kMErode * kMB], m0, m1, m2);
         if (gy < h - kMErode) store_min3(bc[kMErode * kMB], m0, m1, m2);
         const float mask = 0.45f * m0 + 0.3f * m1 + 0.25f * m2;
-        const int64_t g = (int64_t)gy * S.pitch + gx;
+        const int g = gy * S.pitch + gx;
         const float db = b0 - b1;
         const float acs = acs_in[j] + 10.0f * db * db;
         const float dcs = dcs_in[j];
         float v = sqrtf(mask_y(mask) * acs + mask_dc_y(mask) * dcs);
         if (!FULL) {
-            plane_of(S, pair, P_MAP)[g] = v;
+            at(plane_u(S, pair, P_MAP), g) = v;
         } else {
             v = (float)((double)v * (1.0 - 0.3 * 0.5));
             v += 0.5f * half_in[j];
@@ -1236,6 +1280,8 @@ Engine::Engine(int width, int height, int capacity, int qnorm, float intensity, 
     h2_ = (h_ - 1) / 2 + 1;
     plane0_ = plane_floats(w_, h_);
     plane1_ = plane_floats(w2_, h2_);
+    // the kernels address a plane with 32-bit byte offsets
+    if (plane0_ >= (int64_t)1 << 30) VSHIP_THROW(E_BadPointer);
     HIP_OR_THROW(hipSetDevice(device_), E_BadDeviceArgument);
     try {
         HIP_OR_THROW(hipStreamCreate(&stream_), E_BadDeviceArgument);

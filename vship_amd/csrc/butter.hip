@@ -959,14 +959,16 @@ __global__ __launch_bounds__(kThreads) void butter_diff_kernel(Scale S, DiffPara
         // stage the scaled differences of the tile and its halo: all of this
         // thread's loads are issued before the first difference is formed
         // (addresses clamped into the image, so that every load is issued
-        // unconditionally; values outside it are replaced by 0)
+        // unconditionally; values outside it are replaced by 0; the last
+        // round's lanes past the region read its last row, whose lines the
+        // region loads anyway)
         constexpr int NE = (kDSW * kDSH + kThreads - 1) / kThreads;
         float va[NE], vb[NE];
 #pragma unroll
         for (int k = 0; k < NE; k++) {
             const int i = threadIdx.x + k * kThreads;
             const int r = i / kDSW, c = i - r * kDSW;
-            const int gx = ox0 - kDHalo + c, gy = oy0 - kDHalo + r;
+            const int gx = ox0 - kDHalo + c, gy = oy0 - kDHalo + min(r, kDSH - 1);
             const bool ok = i < kDSW * kDSH && gx >= 0 && gx < w && gy >= 0 && gy < h;
             const int g = min(max(gy, 0), h - 1) * S.pitch + min(max(gx, 0), w - 1);
             const float x = at(a, g), y = at(b, g);

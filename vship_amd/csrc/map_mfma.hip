@@ -327,34 +327,45 @@ __device__ __forceinline__ void mx_tile(MxLds<SP::kParts>& L, const TileGeo& T, 
             }
             wave_lds_sync();
             // ---- maps of this lane's output pixels: rows y0 + 16 ob + 4g + r, column xo
+            // The eight samples are loaded together, at the position clamped
+            // into the image, and every pixel is computed; a pixel outside the
+            // image adds zero through the selects on ok, which also discard
+            // the NaN that its zero renormalisation total gives. (A load under
+            // the pixel's own condition is compiled into a branch with its
+            // own wait: one global round trip per row.)
             const float4 rf = *reinterpret_cast<const float4*>(&L.rowf[wave][16 * ob + 4 * g]);
             const float rfv[4] = {rf.x, rf.y, rf.z, rf.w};
+            float s1[4], s2[4];
 #pragma unroll
             for (int r = 0; r < 4; r++) {
-                const int y = T.y0 + 16 * ob + 4 * g + r;
-                if (T.xo < W && y < H) {
-                    const float iv = invh * rfv[r];
-                    const float mu1 = o[0][r] * iv, mu2 = o[1][r] * iv;
-                    const float E2 = o[2][r] * (iv * kEOut), Q = o[3][r] * (iv * kQOut);
-                    const float md = mu1 - mu2;
-                    const float md2 = md * md;
-                    const float vd = E2 - md2;  // var(x1 - x2)
-                    const float num_s = fmaf(2.f, Q - mu1 * mu2, 0.0009f);
-                    float d = fmaf(md2, num_s, vd) * __builtin_amdgcn_rcpf(vd + num_s);
-                    d = d > 0.f ? d : 0.f;
-                    const float a = ldf(p1 + (int64_t)y * pitch + T.xo) - cc;
-                    const float b = ldf(p2 + (int64_t)y * pitch + T.xo) - cc;
-                    const float fa = fabsf(a - mu1) + 1.f, fb = fabsf(b - mu2) + 1.f;
-                    const float ev = (fb - fa) * __builtin_amdgcn_rcpf(fa);  // src/ssimu2/score.hpp:150-154
-                    const float art = ev > 0.f ? ev : 0.f, det = ev < 0.f ? -ev : 0.f;
-                    const float d2 = d * d, a2 = art * art, t2 = det * det;
-                    acc[0] += d;
-                    acc[1] += d2 * d2;
-                    acc[2] += art;
-                    acc[3] += a2 * a2;
-                    acc[4] += det;
-                    acc[5] += t2 * t2;
-                }
+                const int64_t off = (int64_t)min(T.y0 + 16 * ob + 4 * g + r, H - 1) * pitch + min(T.xo, W - 1);
+                s1[r] = ldf(p1 + off);
+                s2[r] = ldf(p2 + off);
+            }
+#pragma unroll
+            for (int r = 0; r < 4; r++) {
+                const bool ok = T.xo < W && T.y0 + 16 * ob + 4 * g + r < H;
+                const float iv = invh * rfv[r];
+                const float mu1 = o[0][r] * iv, mu2 = o[1][r] * iv;
+                const float E2 = o[2][r] * (iv * kEOut), Q = o[3][r] * (iv * kQOut);
+                const float md = mu1 - mu2;
+                const float md2 = md * md;
+                const float vd = E2 - md2;  // var(x1 - x2)
+                const float num_s = fmaf(2.f, Q - mu1 * mu2, 0.0009f);
+                float d = fmaf(md2, num_s, vd) * __builtin_amdgcn_rcpf(vd + num_s);
+                d = (ok && d > 0.f) ? d : 0.f;
+                const float a = s1[r] - cc;
+                const float b = s2[r] - cc;
+                const float fa = fabsf(a - mu1) + 1.f, fb = fabsf(b - mu2) + 1.f;
+                const float ev = (fb - fa) * __builtin_amdgcn_rcpf(fa);  // src/ssimu2/score.hpp:150-154
+                const float art = (ok && ev > 0.f) ? ev : 0.f, det = (ok && ev < 0.f) ? -ev : 0.f;
+                const float d2 = d * d, a2 = art * art, t2 = det * det;
+                acc[0] += d;
+                acc[1] += d2 * d2;
+                acc[2] += art;
+                acc[3] += a2 * a2;
+                acc[4] += det;
+                acc[5] += t2 * t2;
             }
         }
         // wave sums of this channel in FP64, fixed order

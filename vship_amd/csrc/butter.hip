@@ -615,19 +615,26 @@ __device__ __forceinline__ void store_run(float* plane, int pitch, int gx, int g
     o[1] = make_float4(v[4], v[5], v[6], v[7]);
 }
 
-// Blur plane src of the tile into out[i] (the thread's run_pos() pixels);
-// ctr[i] receives the unblurred value at the same pixel. The LDS is free for
-// reuse after the caller's next __syncthreads (the next blur_tile begins
-// with one).
+// staging registers of a blur tile's region
 template <int R>
-__device__ __forceinline__ void blur_tile(BlurLds<R>& L, const float* __restrict__ src, const Taps* __restrict__ T,
-                                          int w, int h, int pitch, int ox0, int oy0, float (&out)[kRun],
-                                          float (&ctr)[kRun]) {
-    constexpr int SW = BlurLds<R>::SW, SWP = BlurLds<R>::SWP, SH = BlurLds<R>::SH;
+using BlurRegs = StageRegs<BlurLds<R>::SW, BlurLds<R>::SWP, BlurLds<R>::SH>;
+
+// Blur of the tile region whose staging loads S holds (stage_load of the
+// plane at ox0 - R, oy0 - R) into out[i], the thread's run_pos() pixels; ctr[i]
+// receives the unblurred value at the same pixel. Once S is in LDS, the
+// staging loads of plane next (if not null) are issued into S, so that they
+// are in flight during this blur's passes. The LDS is free for reuse after
+// the caller's next __syncthreads (the next blur_tile begins with one).
+template <int R>
+__device__ __forceinline__ void blur_tile(BlurLds<R>& L, BlurRegs<R>& S, const float* __restrict__ next,
+                                          const Taps* __restrict__ T, int w, int h, int pitch, int ox0, int oy0,
+                                          float (&out)[kRun], float (&ctr)[kRun]) {
+    constexpr int SW = BlurLds<R>::SW, SWP = BlurLds<R>::SWP;
     static_assert(SWP % 4 == 0 && (kRuns - 1) * kRun + 4 * run_vectors<R>() <= SWP,
                   "row-job loads stay inside the padded row");
     __syncthreads();
-    stage_padded<SW, SWP, SH>(L.in, src, w, h, pitch, ox0 - R, oy0 - R);
+    stage_store(S, L.in, w, h, ox0 - R, oy0 - R);
+    if (next) stage_load(S, next, w, h, pitch, ox0 - R, oy0 - R);
     __syncthreads();
     // column pass: job (rb, c) sums rows kRun rb .. kRun rb + kRun - 1 of staged
     // column c from staged rows kRun rb .. kRun rb + kRun - 1 + 2R
@@ -659,9 +666,12 @@ __global__ __launch_bounds__(kThreads) void butter_opsin_kernel(Scale S, const T
     const int ox0 = ti.x * kTW, oy0 = ti.y * kTH;
     edge_factors<R, kTW, kTH>(L.colf, L.rowf, T, S.w, S.h, ox0, oy0);
     float bl[3][kRun], ctr[3][kRun];
+    BlurRegs<R> SR;
+    stage_load(SR, plane_u(S, pair, P_LIN + im * 3), S.w, S.h, S.pitch, ox0 - R, oy0 - R);
 #pragma unroll
     for (int c = 0; c < 3; c++)
-        blur_tile<R>(L, plane_u(S, pair, P_LIN + im * 3 + c), T, S.w, S.h, S.pitch, ox0, oy0, bl[c], ctr[c]);
+        blur_tile<R>(L, SR, c < 2 ? plane_u(S, pair, P_LIN + im * 3 + c + 1) : nullptr, T, S.w, S.h, S.pitch, ox0,
+                     oy0, bl[c], ctr[c]);
     const RunPos p = run_pos();
     const int gx = ox0 + p.tx, gy = oy0 + p.ty;
     if (gx >= S.w || gy >= S.h) return;
@@ -704,10 +714,13 @@ __global__ __launch_bounds__(kThreads) void butter_low_kernel(Scale S, const Tap
     const bool ok = gx < S.w && gy < S.h;
     edge_factors<R, kTW, kTH>(L.colf, L.rowf, T, S.w, S.h, ox0, oy0);
     float lf[3][kRun];
+    BlurRegs<R> SR;
+    stage_load(SR, plane_u(S, pair, P_XYB + im * 3), S.w, S.h, S.pitch, ox0 - R, oy0 - R);
 #pragma unroll
     for (int c = 0; c < 3; c++) {
         float ctr[kRun], t[kRun];
-        blur_tile<R>(L, plane_u(S, pair, P_XYB + im * 3 + c), T, S.w, S.h, S.pitch, ox0, oy0, lf[c], ctr);
+        blur_tile<R>(L, SR, c < 2 ? plane_u(S, pair, P_XYB + im * 3 + c + 1) : nullptr, T, S.w, S.h, S.pitch, ox0,
+                     oy0, lf[c], ctr);
 #pragma unroll
         for (int i = 0; i < kRun; i++) t[i] = ctr[i] - lf[c][i];
         if (ok) store_run(plane_u(S, pair, P_T + im * 3 + c), S.pitch, gx, gy, t);
@@ -741,11 +754,14 @@ __global__ __launch_bounds__(kThreads) void butter_mid_kernel(Scale S, const Tap
     edge_factors<R, kTW, kTH>(L.colf, L.rowf, T, S.w, S.h, ox0, oy0);
     float hfr[2][kRun];
     constexpr int kOrder[3] = {1, 0, 2};
+    BlurRegs<R> SR;
+    stage_load(SR, plane_u(S, pair, P_T + im * 3 + kOrder[0]), S.w, S.h, S.pitch, ox0 - R, oy0 - R);
 #pragma unroll
     for (int o = 0; o < 3; o++) {
         const int c = kOrder[o];
         float mf[kRun], ctr[kRun];
-        blur_tile<R>(L, plane_u(S, pair, P_T + im * 3 + c), T, S.w, S.h, S.pitch, ox0, oy0, mf, ctr);
+        blur_tile<R>(L, SR, o < 2 ? plane_u(S, pair, P_T + im * 3 + kOrder[o < 2 ? o + 1 : o]) : nullptr, T, S.w,
+                     S.h, S.pitch, ox0, oy0, mf, ctr);
 #pragma unroll
         for (int i = 0; i < kRun; i++) {
             const float m = mf[i];
@@ -779,10 +795,13 @@ __global__ __launch_bounds__(kThreads) void butter_high_kernel(Scale S, const Ta
     const int gx = ox0 + p.tx, gy = oy0 + p.ty;
     const bool ok = gx < S.w && gy < S.h;
     edge_factors<R, kTW, kTH>(L.colf, L.rowf, T, S.w, S.h, ox0, oy0);
+    BlurRegs<R> SR;
+    stage_load(SR, plane_u(S, pair, P_HFR + im * 2), S.w, S.h, S.pitch, ox0 - R, oy0 - R);
 #pragma unroll
     for (int c = 0; c < 2; c++) {
         float hf[kRun], uhf[kRun];
-        blur_tile<R>(L, plane_u(S, pair, P_HFR + im * 2 + c), T, S.w, S.h, S.pitch, ox0, oy0, hf, uhf);
+        blur_tile<R>(L, SR, c < 1 ? plane_u(S, pair, P_HFR + im * 2 + c + 1) : nullptr, T, S.w, S.h, S.pitch, ox0,
+                     oy0, hf, uhf);
 #pragma unroll
         for (int i = 0; i < kRun; i++) {
             float hv = hf[i], uv = uhf[i];

@@ -64,10 +64,11 @@ __device__ __forceinline__ void to_xyb(float r, float g, float b, float& X, floa
     l = cbrt_pos(l) + abs_bias;
     m = cbrt_pos(m) + abs_bias;
     s = cbrt_pos(s) + abs_bias;
-    const float x = 0.5f * (l - m);
+    // X = 0.5 (l - m) * 14 + 0.42, as one fma of (l - m) * 7: the halving is
+    // exact, so the product, and the rounded sum, are the same
     const float y = 0.5f * (l + m);
     B = (s - y) + 0.55f;
-    X = x * 14.0f + 0.42f;
+    X = fmaf(l - m, 7.0f, 0.42f);
     Y = y + 0.01f;
 }
 

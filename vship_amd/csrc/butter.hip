@@ -429,7 +429,9 @@ struct alignas(16) BlurLds {
 };
 
 // Staging of a region [gx0, gx0 + SW) x [gy0, gy0 + SH) of a plane into
-// LDS rows of SWP floats, 0 outside the image and in the row padding. Thread
+// LDS rows of SWP floats, 0 outside the image. The row padding (columns SW
+// .. SWP - 1) holds finite image values, or 0 in a region that crosses an
+// edge; no pass uses it (the row jobs' 16-byte reads may load it). Thread
 // (tx, ty) = (t % 64, t / 64) takes columns tx, tx + 64, ... of rows ty,
 // ty + 4, ...; ty is the wave's, so the row part of an address is scalar.
 //
@@ -505,8 +507,8 @@ __device__ __forceinline__ void stage_store(const StageRegs<SW, SWP, SH>& S, flo
     if (crosses_edge(gx0, gy0, SW, SH, w, h)) stage_clear_outside<SW, SWP, SH>(in, w, h, gx0, gy0);
 }
 
-// in[r][c] = src(gx0 + c, gy0 + r) for c < SW, 0 outside the image and in
-// the row padding
+// in[r][c] = src(gx0 + c, gy0 + r) for c < SW, 0 outside the image (the
+// row padding as above)
 template <int SW, int SWP, int SH>
 __device__ __forceinline__ void stage_padded(float* __restrict__ in, const float* __restrict__ src, int w, int h,
                                              int pitch, int gx0, int gy0) {

@@ -4,8 +4,9 @@ FP64 oracle (oracle/butter_oracle.c) and the reference's own output
 
 * the batched entry points (Vship_ButteraugliBatch*): norms within 1e-4
   relative of the oracle, the distortion map within 1e-4 relative + 1e-6
-  absolute, on float and 16-bit inputs, odd sizes, 1x1 and 1920x1080, Q-norms
-  1..5 and several intensities;
+  absolute, on float and 16-bit inputs, odd sizes, 1x1 and 1920x1080, sizes
+  on and just past the kernels' tile boundaries, Q-norms 1..5 and several
+  intensities;
 * the reference's per-frame entry points (Vship_ComputeButteraugli*): the int
   score fields are the norms truncated, the map written with the caller's
   stride;
@@ -77,6 +78,25 @@ def test_batch_matches_oracle(fmt, w, h, q, inten, seed):
     norms, maps = gpu_norms([(ref, dist)], q, inten, distmaps=True)
     want, wmap = bl.butteraugli(ref, dist, q, inten, diffmap=True)
     check_norms(norms[0], want, f"{fmt} {w}x{h}")
+    err = np.abs(maps[0] - wmap) - (MAP_ATOL + MAP_RTOL * np.abs(wmap))
+    say(f"map max abs diff {np.abs(maps[0] - wmap).max():.2e} of max {np.abs(wmap).max():.3f}")
+    assert err.max() <= 0.0
+
+
+# Sizes on and just past the kernels' tiles (blurs 64 x 32, Malta 64 x 16,
+# mask 32 x 32) at full scale and, halved, at half scale: regions that end
+# exactly at an image edge, cross it by one pixel, or are narrower than the
+# widest blur window (21 pixels each side).
+TILE_EDGE_SIZES = [(64, 32), (65, 33), (127, 31), (128, 64), (129, 17), (63, 97), (20, 130)]
+
+
+@pytest.mark.parametrize("w,h", TILE_EDGE_SIZES)
+def test_tile_edges_match_oracle(w, h):
+    say(f"case butter tile edges {w}x{h}")
+    ref, dist = butter_inputs(w, h, "f32", 100 + w + h)
+    norms, maps = gpu_norms([(ref, dist)], distmaps=True)
+    want, wmap = bl.butteraugli(ref, dist, 2, 203.0, diffmap=True)
+    check_norms(norms[0], want, f"{w}x{h}")
     err = np.abs(maps[0] - wmap) - (MAP_ATOL + MAP_RTOL * np.abs(wmap))
     say(f"map max abs diff {np.abs(maps[0] - wmap).max():.2e} of max {np.abs(wmap).max():.3f}")
     assert err.max() <= 0.0

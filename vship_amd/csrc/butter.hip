@@ -365,20 +365,22 @@ __global__ __launch_bounds__(kThreads) void butter_input420_kernel(const PairDes
                 lin[k][1] = linearize((yn - kr * R - kb * B) * ikg);
                 lin[k][2] = linearize(B);
             }
+            // each row's two pixels as one 8-byte store (xa is even and rows are
+            // padded to 64 floats: with no second column, at an odd width, the
+            // second value lands in the row padding, which no kernel reads)
+            const int o0 = ya * S0.pitch + xa, o1 = yb * S0.pitch + xa, oh = (ya / 2) * S1.pitch + xa / 2;
 #pragma unroll
             for (int c = 0; c < 3; c++) {
-                float* L = plane_of(S0, pair, P_LIN + im * 3 + c);
-                L[(int64_t)ya * S0.pitch + xa] = lin[0][c];
-                if (hy) L[(int64_t)yb * S0.pitch + xa] = lin[1][c];
-                if (hx) L[(int64_t)ya * S0.pitch + xb] = lin[2][c];
-                if (hx && hy) L[(int64_t)yb * S0.pitch + xb] = lin[3][c];
+                float* L = plane_u(S0, pair, P_LIN + im * 3 + c);
+                *reinterpret_cast<float2*>(&at(L, o0)) = make_float2(lin[0][c], lin[2][c]);
+                if (hy) *reinterpret_cast<float2*>(&at(L, o1)) = make_float2(lin[1][c], lin[3][c]);
                 // half scale: 2x2 mean, the reference's summation order
                 float m = 0.0f;
                 m += lin[0][c];
                 m += lin[1][c];
                 m += lin[2][c];
                 m += lin[3][c];
-                plane_of(S1, pair, P_LIN + im * 3 + c)[(int64_t)(ya / 2) * S1.pitch + xa / 2] = m * 0.25f;
+                at(plane_u(S1, pair, P_LIN + im * 3 + c), oh) = m * 0.25f;
             }
         }
     }

@@ -764,8 +764,8 @@ __global__ __launch_bounds__(kThreads) void butter_mid_kernel(Scale S, const Tap
     for (int o = 0; o < 3; o++) {
         const int c = kOrder[o];
         float mf[kRun], ctr[kRun];
-        blur_tile<R>(L, SR, o < 2 ? plane_u(S, pair, P_T + im * 3 + kOrder[o < 2 ? o + 1 : o]) : nullptr, T, S.w,
-                     S.h, S.pitch, ox0, oy0, mf, ctr);
+        const float* next = o < 2 ? plane_u(S, pair, P_T + im * 3 + kOrder[(o + 1) % 3]) : nullptr;
+        blur_tile<R>(L, SR, next, T, S.w, S.h, S.pitch, ox0, oy0, mf, ctr);
 #pragma unroll
         for (int i = 0; i < kRun; i++) {
             const float m = mf[i];

@@ -163,8 +163,11 @@ __device__ __forceinline__ float linearize(float a) {
     return copysignf(__builtin_amdgcn_exp2f(2.4f * __builtin_amdgcn_logf(m)), a);
 }
 
+// 19.245 ln(v + 9.971) - 23.160 with the logarithm on the hardware log2 unit
+// (1 ulp; its argument is above 11) and ln 2 folded into the factor
 __device__ __forceinline__ float opsin_gamma(float v) {
-    return fmaf(19.245013259874995f, logf(v + 9.9710635769299145f), -23.16046239805755f);
+    constexpr float kMul = (float)(19.245013259874995 * 0.69314718055994531);
+    return fmaf(kMul, __builtin_amdgcn_logf(v + 9.9710635769299145f), -23.16046239805755f);
 }
 
 __device__ __forceinline__ void opsin_absorbance(float a0, float a1, float a2, float& o0, float& o1, float& o2) {
@@ -186,20 +189,22 @@ __device__ __forceinline__ float maximum_clamp(float v, float maxval) {
 __device__ __forceinline__ float mask_input(float hfx, float uhfx, float hfy, float uhfy) {
     const float xd = (uhfx + hfx) * 2.5f;
     const float yd = uhfy * 0.4f + hfy * 0.4f;
-    const float m = sqrtf(xd * xd + yd * yd);
+    // square roots of non-negative values on the hardware unit (1 ulp)
+    const float m = __builtin_amdgcn_sqrtf(xd * xd + yd * yd);
     const float mul = 6.19424080439f, bias = mul * 12.61050594197f;
-    return sqrtf(mul * fabsf(m) + bias) - sqrtf(bias);
+    return __builtin_amdgcn_sqrtf(mul * fabsf(m) + bias) - __builtin_amdgcn_sqrtf(bias);
 }
 
 constexpr double kMaskNorm = 1.0 / (double)(0.79079917404f * 17.83f);
 
 __device__ __forceinline__ float mask_y(float d) {
-    const float c = 2.5485944793f / ((0.451936922203f * d) + 0.829591754942f);
+    // hardware reciprocal (1 ulp) of a denominator above 0.8
+    const float c = 2.5485944793f * __builtin_amdgcn_rcpf((0.451936922203f * d) + 0.829591754942f);
     const float r = (float)((1.0 + (double)c) * kMaskNorm);
     return r * r;
 }
 __device__ __forceinline__ float mask_dc_y(float d) {
-    const float c = 0.505054525019f / ((3.87449418804f * d) + 0.20025578522f);
+    const float c = 0.505054525019f * __builtin_amdgcn_rcpf((3.87449418804f * d) + 0.20025578522f);
     const float r = (float)((1.0 + (double)c) * kMaskNorm);
     return r * r;
 }
@@ -687,9 +692,10 @@ __global__ __launch_bounds__(kThreads) void butter_opsin_kernel(Scale S, const T
         b0 = fmaxf(fmaxf(b0, 1.7557483643287353f), 1e-4f);
         b1 = fmaxf(fmaxf(b1, 1.7557483643287353f), 1e-4f);
         b2 = fmaxf(fmaxf(b2, 12.226454707163354f), 1e-4f);
-        const float s0 = fmaxf(opsin_gamma(b0) / b0, 1e-4f);
-        const float s1 = fmaxf(opsin_gamma(b1) / b1, 1e-4f);
-        const float s2 = fmaxf(opsin_gamma(b2) / b2, 1e-4f);
+        // b >= 1.75: the hardware reciprocal (1 ulp) for the reference's division
+        const float s0 = fmaxf(opsin_gamma(b0) * __builtin_amdgcn_rcpf(b0), 1e-4f);
+        const float s1 = fmaxf(opsin_gamma(b1) * __builtin_amdgcn_rcpf(b1), 1e-4f);
+        const float s2 = fmaxf(opsin_gamma(b2) * __builtin_amdgcn_rcpf(b2), 1e-4f);
         float o0, o1, o2;
         opsin_absorbance(ctr[0][i] * intensity, ctr[1][i] * intensity, ctr[2][i] * intensity, o0, o1, o2);
         o0 = fmaxf(o0 * s0, 1.7557483643287353f);
@@ -1179,7 +1185,7 @@ __global__ __launch_bounds__(kThreads) void butter_mask_kernel(Scale S, const Ta
         const float db = b0 - b1;
         const float acs = acs_in[j] + 10.0f * db * db;
         const float dcs = dcs_in[j];
-        float v = sqrtf(mask_y(mask) * acs + mask_dc_y(mask) * dcs);
+        float v = __builtin_amdgcn_sqrtf(mask_y(mask) * acs + mask_dc_y(mask) * dcs);  // of a sum >= 0, 1 ulp
         if (!FULL) {
             at(plane_u(S, pair, P_MAP), g) = v;
         } else {

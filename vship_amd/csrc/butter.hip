@@ -589,14 +589,15 @@ __device__ __forceinline__ void hpass_blocked(const float* __restrict__ in, floa
     }
 }
 
-// column pass: N outputs of column c, rows r0 .. r0 + N - 1 of the blur,
-// from mid rows r0 .. r0 + N - 1 + 2R (stride MW), not yet row-normalised
+// column job: N outputs of column c, rows r0 .. r0 + N - 1 of the blur,
+// from rows r0 .. r0 + N - 1 + 2R of src (stride MW: the staged input, or
+// the row pass's output), not yet row-normalised
 template <int R, int MW, int N>
-__device__ __forceinline__ void vpass_block(const float* __restrict__ mid, const Taps* __restrict__ T, int r0, int c,
+__device__ __forceinline__ void vpass_block(const float* __restrict__ src, const Taps* __restrict__ T, int r0, int c,
                                             float (&out)[N]) {
     float v[N + 2 * R];
 #pragma unroll
-    for (int i = 0; i < N + 2 * R; i++) v[i] = mid[(r0 + i) * MW + c];
+    for (int i = 0; i < N + 2 * R; i++) v[i] = src[(r0 + i) * MW + c];
 #pragma unroll
     for (int j = 0; j < N; j++) out[j] = 0.0f;
 #pragma unroll

@@ -108,3 +108,41 @@ def test_oracle_matches_reference_hip(case):
     want = np.array([v for _, _, v in g["map_samples"]])
     np.testing.assert_allclose(got, want, rtol=1e-4, atol=1e-6)
     assert dm.sum() == pytest.approx(g["map_sum"], rel=1e-5)
+
+
+def _guard_refuses(n):
+    """A width or height the reference's tiled blur reads out of bounds for:
+    n or ceil(n / 2) leaves a remainder of 1..5 modulo 32
+    (src/butter/gaussianblur.hpp:127-130,154-157)."""
+    return any(1 <= m % 32 <= 5 for m in (n, (n + 1) // 2))
+
+
+def test_reference_size_guard_python_side():
+    """The test-side guard refuses exactly the sizes the reference's tiled
+    blur reads out of bounds for, at full and half scale."""
+    for w in range(1, 300):
+        for h in (7, 16, 33, 64, 70, 97, 130):
+            assert bl.ref_size_ok(w, h) == (not _guard_refuses(w) and not _guard_refuses(h)), (w, h)
+
+
+@pytest.mark.skipif(not bl.ref_available(), reason="reference shim not built (reference sources absent)")
+def test_reference_size_guard_in_shim():
+    """vref_butter_size_ok, the guard inside the reference shim, refuses every
+    such size, and vref_butter returns its error for one before touching the
+    device (called here without a GPU)."""
+    R = bl.ref_lib()
+    for w in range(1, 300):
+        for h in (7, 16, 33, 64, 70, 97, 130):
+            assert bool(R.vref_butter_size_ok(w, h)) == (not _guard_refuses(w) and not _guard_refuses(h)), (w, h)
+    refused = [n for n in range(1, 200) if _guard_refuses(n)]
+    assert 33 in refused and 97 in refused and 65 in refused  # 65 -> 33 at half scale
+    planes = [np.zeros((64, 97), np.float32) for _ in range(3)]
+    with pytest.raises(ValueError):
+        bl.ref_butteraugli(planes, planes)
+    import ctypes
+    ptrs = bl.P3(*[p.ctypes.data for p in planes])
+    F = ctypes.POINTER(ctypes.c_float)
+    norms = np.zeros(3, np.float32)
+    rc = R.vref_butter(1, ptrs, ptrs, 97 * 4, 97 * 4, 97, 64, 2, 203.0, norms.ctypes.data_as(F),
+                       ctypes.cast(None, F))
+    assert rc == 3

@@ -102,7 +102,11 @@ def test_tile_edges_match_oracle(w, h):
     assert err.max() <= 0.0
 
 
-YUV_CASES = [("420", 8, api.CHROMA_LEFT, api.MATRIX_BT709), ("422", 10, api.CHROMA_CENTER, api.MATRIX_BT2020NCL),
+# 4:2:0 with left-located chroma takes the staged input kernel (8-bit and
+# 16-bit containers); every other layout and location the generic one
+YUV_CASES = [("420", 8, api.CHROMA_LEFT, api.MATRIX_BT709), ("420", 10, api.CHROMA_LEFT, api.MATRIX_BT709),
+             ("420", 12, api.CHROMA_LEFT, api.MATRIX_BT2020NCL),
+             ("422", 10, api.CHROMA_CENTER, api.MATRIX_BT2020NCL),
              ("444", 8, api.CHROMA_LEFT, api.MATRIX_BT601), ("410", 12, api.CHROMA_TOPLEFT, api.MATRIX_BT709)]
 
 
@@ -126,6 +130,79 @@ def test_yuv_input_matches_oracle(layout, bits, loc, matrix):
         dp.free()
     rgb = [ol.yuv_to_rgb_any(im, bits, matrix, sw, sh, loc).astype(np.float32) for im in (ref, dist)]
     check_norms(got, bl.butteraugli(list(rgb[0]), list(rgb[1])), f"YUV{layout}")
+
+
+def test_yuv420_8bit_ignores_bits_field():
+    """For the 8-bit YUV formats the descriptor's bits field is ignored (the
+    header says so): bits = 0 scores the same as bits = 8."""
+    from tests.test_gpu_yuv_layouts import yuv_pair
+
+    w, h = 96, 64
+    say(f"case butter YUV420 8-bit bits=0 {w}x{h}")
+    ref, dist = yuv_pair(w, h, "420", 8, 77)
+    out = []
+    for bits in (8, 0):
+        dp = api.ButterDevicePairs([(ref, dist)], fmt=api.PF_YUV420_U8, bits=bits)
+        try:
+            with api.ButteraugliBatch(w, h, 1) as b:
+                out.append(b.compute(dp)[0])
+        finally:
+            dp.free()
+    assert np.array_equal(out[0], out[1])
+
+
+def test_host_distmap_is_rejected():
+    """A distance map in host memory is refused with BadPointer before any
+    kernel runs (the reference's per-frame ABI writes its map to host memory,
+    the batched ABI takes a device buffer)."""
+    say("case butter host distmap rejected")
+    ref, dist = butter_inputs(64, 48, "f32", 61)
+    dp = api.ButterDevicePairs([(ref, dist)])
+    host_map = np.zeros((48, 64), np.float32)
+    dp.descs[0].distmap = host_map.ctypes.data
+    dp.descs[0].distmap_stride = 64 * 4
+    norms = np.zeros((1, 3))
+    try:
+        with api.ButteraugliBatch(64, 48, 1) as b:
+            rc = api.lib().Vship_ButteraugliBatchCompute(b._h, dp.format, dp.descs, 1, norms.ctypes.data)
+            assert rc == 10  # BadPointer
+            # a device map whose stride is shorter than a row is refused too
+            dp.descs[0].distmap = dp.buffer.ptr
+            dp.descs[0].distmap_stride = 60 * 4
+            rc = api.lib().Vship_ButteraugliBatchCompute(b._h, dp.format, dp.descs, 1, norms.ctypes.data)
+            assert rc == 10
+            # and the handle still works afterwards
+            dp.descs[0].distmap = None
+            got = b.compute(dp)[0]
+    finally:
+        dp.free()
+    check_norms(got, bl.butteraugli(ref, dist), "after rejected maps")
+
+
+@pytest.mark.parametrize("dtype", ["u8", "f16"])
+def test_rgb_u8_f16_inputs_match_oracle(dtype):
+    """Planar RGB u8 (v / 255) and f16 inputs against the oracle on the same
+    values (u8 as the u16 code v * 257, f16 widened to f32, both exact)."""
+    w, h = 83, 57
+    say(f"case butter RGB {dtype} {w}x{h}")
+    ref, dist = butter_inputs(w, h, "f32", 71)
+    if dtype == "u8":
+        gr = [np.round(p * 255).astype(np.uint8) for p in ref]
+        gd = [np.round(p * 255).astype(np.uint8) for p in dist]
+        fmt = api.PF_RGB_U8
+        want = bl.butteraugli([p.astype(np.uint16) * 257 for p in gr], [p.astype(np.uint16) * 257 for p in gd])
+    else:
+        gr = [p.astype(np.float16) for p in ref]
+        gd = [p.astype(np.float16) for p in dist]
+        fmt = api.PF_RGB_F16
+        want = bl.butteraugli([p.astype(np.float32) for p in gr], [p.astype(np.float32) for p in gd])
+    dp = api.ButterDevicePairs([(gr, gd)], fmt=fmt)
+    try:
+        with api.ButteraugliBatch(w, h, 1) as b:
+            got = b.compute(dp)[0]
+    finally:
+        dp.free()
+    check_norms(got, want, f"RGB {dtype}")
 
 
 def test_one_pixel():

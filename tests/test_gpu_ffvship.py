@@ -138,3 +138,24 @@ def test_butteraugli_metric(tmp_path):
                                    .astype(np.float32)) for im in (ref[i], dist[i])])
         assert np.allclose([float(q), float(n3), float(ninf)], w2, rtol=2e-4)
 
+
+
+@needs_cli
+def test_butteraugli_420p10_clip(tmp_path):
+    """-m Butteraugli on a C420p10 clip (left-located chroma in 16-bit
+    containers: the staged 16-bit input kernel) against the FP64 oracle."""
+    from tests import butter_lib as bl
+
+    w, h, n = 144, 80, 3
+    ref, dist = clip(n, w, h, "420", 10, 970)
+    a, b, js = str(tmp_path / "ref.y4m"), str(tmp_path / "dist.y4m"), str(tmp_path / "b.json")
+    write_y4m(a, ref, w, h, "C420p10")
+    write_y4m(b, dist, w, h, "C420p10")
+    want = np.array([bl.butteraugli(*[list(ol.yuv_to_rgb_any(im, 10, ol.SO_MAT_BT601, 1, 1, ol.SO_LOC_LEFT)
+                                          .astype(np.float32)) for im in (ref[i], dist[i])])
+                     for i in range(n)])
+    run("-s", a, "-e", b, "-m", "Butteraugli", "--json", js)
+    got = np.array(json.load(open(js)))
+    say(f"  C420p10 butteraugli max rel {np.max(np.abs(got - want) / want):.1e}")
+    assert got.shape == (n, 3)
+    assert np.all(np.abs(got - want) <= 1e-4 * want + 5e-6 * np.abs(want)), (got, want)

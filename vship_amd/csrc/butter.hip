@@ -961,7 +961,8 @@ __device__ __forceinline__ float l2_asym(float a, float b, float wgt, float wlt)
     const float diff = a - b;
     float r = wgt * diff * diff;
     const float fabs0 = fabsf(a);
-    const float too_small = 0.4f * fabs0, too_big = fabs0;
+    // 0.4 * |a| is a double product rounded to float in simplerdiff.hpp:69
+    const float too_small = (float)(0.4 * (double)fabs0), too_big = fabs0;
     const float bm = a < 0.0f ? -b : b;
     const float v = bm < too_small ? too_small - bm : (bm > too_big ? bm - too_big : 0.0f);
     return r + wlt * v * v;
@@ -1371,17 +1372,21 @@ void Engine::submit(int fmt, const PairDesc* pairs, int n) {
     // reads must lie within the plane's stride
     const int bps = fmt_yuv(fmt) ? (fmt_wide(fmt) ? 2 : 1) : (fmt == IN_RGB_U8 ? 1 : fmt == IN_RGB_F32 ? 4 : 2);
     int64_t row[3];
+    int rows[3];
     for (int c = 0; c < 3; c++) {
-        const int pw = (c == 0 || !fmt_yuv(fmt)) ? w_ : (w_ + (1 << fmt_csw(fmt)) - 1) >> fmt_csw(fmt);
+        const bool sub = c > 0 && fmt_yuv(fmt);
+        const int pw = sub ? (w_ + (1 << fmt_csw(fmt)) - 1) >> fmt_csw(fmt) : w_;
+        rows[c] = sub ? (h_ + (1 << fmt_csh(fmt)) - 1) >> fmt_csh(fmt) : h_;
         row[c] = (int64_t)pw * bps;
     }
+    // every source plane and the distance map must lie inside a device
+    // allocation: a host pointer (the reference's ABI writes its distance map
+    // to host memory) is an error here, before any kernel runs
     for (int i = 0; i < n; i++) {
         for (int im = 0; im < 2; im++)
-            for (int c = 0; c < 3; c++) {
-                if (!pairs[i].src[im][c]) VSHIP_THROW(E_BadPointer);
-                if (pairs[i].stride[im][c] < row[c]) VSHIP_THROW(E_BadPointer);
-            }
-        if (pairs[i].map && pairs[i].map_stride < (int64_t)w_ * 4) VSHIP_THROW(E_BadPointer);
+            for (int c = 0; c < 3; c++)
+                require_device_rows(pairs[i].src[im][c], pairs[i].stride[im][c], rows[c], row[c]);
+        if (pairs[i].map) require_device_rows(pairs[i].map, pairs[i].map_stride, h_, (int64_t)w_ * 4);
         if (fmt_yuv(fmt)) {
             const int bits = pairs[i].bits;
             if (fmt_wide(fmt) ? (bits < 9 || bits > 16) : bits != 8) VSHIP_THROW(E_BadPointer);

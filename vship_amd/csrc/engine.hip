@@ -51,24 +51,27 @@ void validate_planes(const PairInput& p) {
     const int bps = input_bytes_per_sample(p.fmt);
     for (int im = 0; im < 2; im++)
         for (int pl = 0; pl < 3; pl++) {
-            const int pw = plane_w(p, pl);
-            const int ph = plane_h(p, pl);
-            const uint8_t* ptr = p.src[im][pl];
-            const int64_t row = (int64_t)pw * bps;
-            if (!ptr || p.stride[im][pl] < row) VSHIP_THROW(E_BadPointer);
-            hipPointerAttribute_t attr;
-            if (hipPointerGetAttributes(&attr, ptr) != hipSuccess) VSHIP_THROW(E_BadPointer);
-            if (attr.type != hipMemoryTypeDevice && attr.type != hipMemoryTypeManaged) VSHIP_THROW(E_BadPointer);
-            hipDeviceptr_t base = nullptr;
-            size_t size = 0;
-            if (hipMemGetAddressRange(&base, &size, (hipDeviceptr_t)ptr) != hipSuccess) VSHIP_THROW(E_BadPointer);
-            const uint8_t* b = reinterpret_cast<const uint8_t*>(base);
-            const uint8_t* end = ptr + (int64_t)(ph - 1) * p.stride[im][pl] + row;
-            if (ptr < b || end > b + size) VSHIP_THROW(E_BadPointer);
+            const int64_t row = (int64_t)plane_w(p, pl) * bps;
+            if (!p.src[im][pl] || p.stride[im][pl] < row) VSHIP_THROW(E_BadPointer);
+            require_device_rows(p.src[im][pl], p.stride[im][pl], plane_h(p, pl), row);
         }
 }
 
 }  // namespace
+
+void require_device_rows(const void* ptr, int64_t stride, int rows, int64_t row_bytes) {
+    if (!ptr || rows < 1 || row_bytes < 1 || stride < row_bytes) VSHIP_THROW(E_BadPointer);
+    hipPointerAttribute_t attr;
+    if (hipPointerGetAttributes(&attr, ptr) != hipSuccess) VSHIP_THROW(E_BadPointer);
+    if (attr.type != hipMemoryTypeDevice && attr.type != hipMemoryTypeManaged) VSHIP_THROW(E_BadPointer);
+    hipDeviceptr_t base = nullptr;
+    size_t size = 0;
+    if (hipMemGetAddressRange(&base, &size, (hipDeviceptr_t)ptr) != hipSuccess) VSHIP_THROW(E_BadPointer);
+    const uint8_t* b = reinterpret_cast<const uint8_t*>(base);
+    const uint8_t* p = reinterpret_cast<const uint8_t*>(ptr);
+    const uint8_t* end = p + (int64_t)(rows - 1) * stride + row_bytes;
+    if (p < b || end > b + size) VSHIP_THROW(E_BadPointer);
+}
 
 void matrix_coefs(int matrix, float& kr, float& kb) {
     switch (matrix) {

@@ -53,6 +53,11 @@ struct VshipErr : std::exception {
 // 2 BT.2020 non-constant luminance.
 void matrix_coefs(int matrix, float& kr, float& kb);
 
+// Throws E_BadPointer unless `rows` rows of `row_bytes` bytes at a pitch of
+// `stride` bytes from `ptr` lie inside one device allocation, so that a host
+// pointer or a short buffer is an error on the host and never a GPU fault.
+void require_device_rows(const void* ptr, int64_t stride, int rows, int64_t row_bytes);
+
 struct PairInput {
     const uint8_t* src[2][3];  // device pointers [image][plane]
     int64_t stride[2][3];

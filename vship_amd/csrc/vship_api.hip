@@ -560,7 +560,8 @@ Vship_Exception Vship_ButteraugliBatchCompute(Vship_ButteraugliBatchHandler hand
             }
             d[i].map = pairs[i].distmap;
             d[i].map_stride = pairs[i].distmap_stride;
-            d[i].bits = fmt_yuv(format) ? pairs[i].bits : 8;
+            // bits counts only for the 16-bit container formats; 8-bit ones are 8
+            d[i].bits = fmt_wide(format) ? pairs[i].bits : 8;
             d[i].cloc = pairs[i].chroma_location;
             matrix_coefs(pairs[i].matrix, d[i].kr, d[i].kb);
         }
@@ -573,6 +574,9 @@ Vship_Exception Vship_ButteraugliBatchGetStats(Vship_ButteraugliBatchHandler han
     std::shared_ptr<ButterBatchHandle> b = butter_batch_table().get(handler.id);
     if (!b) return Vship_BadHandler;
     if (!st) return Vship_BadPointer;
+    // the engine's timing state belongs to a compute in flight on this handle
+    BusyGuard g(b->busy);
+    if (!g.ok) return Vship_BadHandler;
     std::memset(st, 0, sizeof(*st));
     const bt::Engine& e = *b->engine;
     float ms[bt::ST_COUNT] = {};

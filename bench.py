@@ -22,9 +22,16 @@ blocks; vship_amd/synth/synth.c holds the generator):
           the FP64 CPU oracle as baseline.
 
 Prints one JSON line. Besides throughput: the dominant kernel's roofline
-position (HIP events on the library's stream), the per-frame path, p50/p99
-latency, and the CPU baseline (the FP32 CPU restatement timed on the host
-cores on a bounded sample) with |dscore| against the FP64 oracle on a sample.
+position (HIP events on the library's stream; vector-ALU issue from the
+committed counter record over the live duration, with the HBM position
+beside it), the per-frame path, p50/p99 latency, and the CPU baseline (the
+FP32 CPU restatement timed on the host's CPU share on a bounded sample, and
+on one thread, with the host's CPU count and model) with |dscore| against the
+FP64 oracle on a sample.
+
+The default 1080p line also carries `sub_records` for the 4k10, mixed and
+served workloads (each with its own throughput, latency, kernel times and
+|dscore| against the FP64 oracle on sampled pairs).
 """
 import argparse
 import json
@@ -45,7 +52,7 @@ CONFIGS = ["1080p", "rgb256", "4k10", "mixed", "served", "butter1080p"]
 def parse():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--steps", type=int, default=200)
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--config", default="1080p", choices=CONFIGS)
     ap.add_argument("--map-th", type=int, default=0)
@@ -58,6 +65,9 @@ def parse():
                     help="first run device-to-device copies of this many MiB (HBM counter calibration)")
     ap.add_argument("--arrival-seconds", type=float, default=2.0, help="served: length of the arrival window")
     ap.add_argument("--rate", type=float, default=200.0, help="served: clip arrival rate (clips/s)")
+    ap.add_argument("--no-sub", action="store_true",
+                    help="1080p: skip the 4K, mixed, served and host-input sub-records")
+    ap.add_argument("--sub-steps", type=int, default=20, help="timed steps of each sub-record")
     return ap.parse_args()
 
 
@@ -129,40 +139,83 @@ def per_frame_rate(descs, n, w, h, fp16):
 
 
 TRAFFIC_FILE = os.path.join(ROOT, "profiles", "hbm_traffic.json")
+VALU_FILE = os.path.join(ROOT, "profiles", "valu_issue.json")
+# MI355X vector ALUs: 256 CUs x 4 SIMDs x 16 lanes at the 2.4 GHz peak engine
+# clock; a wave64 VALU instruction is 64 lane-ops (4 cycles of its SIMD), a
+# transcendental one 4 times that (quarter rate)
+VALU_PEAK_TOPS = 256 * 4 * 16 * 2.4e9 / 1e12
+
+
+def _record(path, config, kernel):
+    try:
+        with open(path) as f:
+            return json.load(f)["workloads"][config][kernel]
+    except (OSError, KeyError, ValueError):
+        return None
 
 
 def measured_traffic(config, kernel):
     """HBM bytes per launch of `kernel` on workload `config` from the committed
     record of the FETCH_SIZE/WRITE_SIZE runs (calibrated on a device copy), or
     None where no run of this kernel on this workload is recorded."""
-    try:
-        with open(TRAFFIC_FILE) as f:
-            rec = json.load(f)["workloads"][config][kernel]
-    except (OSError, KeyError, ValueError):
+    rec = _record(TRAFFIC_FILE, config, kernel)
+    return None if rec is None else rec["traffic_bytes"]
+
+
+def valu_position(config, kernel, ms):
+    """Vector-ALU issue of one launch of `kernel`: its lane-ops (from the
+    committed SQ_INSTS_VALU / SQ_INSTS_VALU_TRANS_F32 / SQ_INSTS_MFMA record of
+    this workload) over the launch's live duration, against the peak issue
+    rate. None where no counter run of this kernel on this workload exists."""
+    rec = _record(VALU_FILE, config, kernel)
+    if rec is None:
         return None
-    return rec["traffic_bytes"]
+    lane_ops = (rec["valu_insts"] - rec["mfma_insts"] + 3 * rec["trans_insts"]) * 64
+    achieved = lane_ops / (ms * 1e-3) / 1e12
+    return {"achieved": round(achieved, 2), "peak": round(VALU_PEAK_TOPS, 2), "unit": "T VALU lane-ops/s",
+            "frac": round(achieved / VALU_PEAK_TOPS, 4), "lane_ops_per_launch": int(lane_ops)}
 
 
-def roofline(st, map_ms, fp16, config):
-    achieved = st["map_bytes_fp32"] / (map_ms * 1e-3) / 1e9
-    elem = "2 B (scales 0-1) or 4 B (scales 2-5)" if fp16 else "4 B"
+def map_kernel_name(st, fp16):
     # with FP16 storage the FP32 class is two launches, one per plane group
-    # (FP16 scales 0-1, FP32 scales 2-5); the time and the traffic cover both
+    # (FP16 scales 0-1, FP32 scales 2-5); the time and the counters cover both
     st_name = "__half+float" if fp16 else "float"
     if st["map_tile_height"] == 1:
         kernel = f"map_mx_kernel<{st_name}, SplitF16x2>"
-        label = f"{kernel} (matrix cores, 64x64 tiles)"
-    else:
-        kernel = label = f"map_kernel<{st['map_tile_height']}, {st_name}, float>"
+        return kernel, f"{kernel} (matrix cores, 64x64 tiles)"
+    kernel = f"map_kernel<{st['map_tile_height']}, {st_name}, float>"
+    return kernel, kernel
+
+
+def roofline(st, map_ms, prep_ms, fp16, config):
+    """The dominant kernel's position. The map kernel is bound by vector-ALU
+    issue: `bound` is "valu" when its counter record exists, with the HBM
+    position (algorithmic bytes over the live duration) beside it."""
+    achieved = st["map_bytes_fp32"] / (map_ms * 1e-3) / 1e9
+    elem = "2 B (scales 0-1) or 4 B (scales 2-5)" if fp16 else "4 B"
+    kernel, label = map_kernel_name(st, fp16)
     traffic = measured_traffic(config, kernel)
-    return {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-            "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic,
-            "traffic_source": ("profiles/hbm_traffic.json: rocprofv3 FETCH_SIZE x 2.0 + WRITE_SIZE x 1.0 per launch"
-                               if traffic is not None else "no PMC run of this kernel on this workload"),
-            "kernel": label,
-            "algorithmic_bytes_per_launch": st["map_bytes_fp32"],
-            "note": f"2 images x 3 XYB planes x {elem} read per pixel of every scale in the FP32 class; "
-                    "the kernel is bound by vector-ALU issue (operand splitting and the maps), not HBM"}
+    hbm = {"achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+           "frac": round(achieved / HBM_PEAK_GBS, 4)}
+    valu = valu_position(config, kernel, map_ms)
+    out = {"bound": "valu" if valu else "hbm"}
+    out.update(valu if valu else hbm)
+    out.update({"traffic": traffic,
+                "traffic_source": ("profiles/hbm_traffic.json: rocprofv3 FETCH_SIZE x 2.0 + WRITE_SIZE x 1.0 per launch"
+                                   if traffic is not None else "no PMC run of this kernel on this workload"),
+                "kernel": label, "hbm": hbm,
+                "algorithmic_bytes_per_launch": st["map_bytes_fp32"],
+                "note": f"2 images x 3 XYB planes x {elem} read per pixel of every scale in the FP32 class; "
+                        "the kernel is bound by vector-ALU issue (operand splitting and the maps), not HBM"})
+    if valu:
+        out["valu_source"] = "profiles/valu_issue.json: rocprofv3 SQ_INSTS_VALU, SQ_INSTS_VALU_TRANS_F32, " \
+                             "SQ_INSTS_MFMA per launch over the live HIP-event duration"
+    # the other large kernel of the step
+    prep_kernel = f"prep_kernel<{'__half' if fp16 else 'float'}, false>"
+    pv = valu_position(config, prep_kernel, prep_ms)
+    if pv:
+        out["prep"] = {"kernel": prep_kernel, "valu": pv}
+    return out
 
 
 def record(a, value, elapsed, steps, dtype, workload, extra_cfg):
@@ -183,9 +236,13 @@ def record(a, value, elapsed, steps, dtype, workload, extra_cfg):
     }
 
 
-def run_batched(a, pairs, bits, fp16, workload, tune=False, graph=False):
-    """Times `a.steps` passes over the device-resident pairs; returns (line, scores)."""
+def run_batched(a, pairs, bits, fp16, workload, tune=False, graph=False, steps=None, config=None, per_frame=True):
+    """Times `steps` (default --steps) passes over the device-resident pairs;
+    returns (line, scores)."""
     from vship_amd import api
+
+    steps = a.steps if steps is None else steps
+    config = a.config if config is None else config
 
     npairs = len(pairs)
     w = max(p[0][0].shape[1] for p in pairs)
@@ -205,22 +262,22 @@ def run_batched(a, pairs, bits, fp16, workload, tune=False, graph=False):
               flush=True)
     # eager handle with per-kernel HIP events
     batch = api.SSIMU2Batch(npairs, w, h, fp16=fp16, map_tile_height=th, persistent_grid=grid, profile=True)
-    scores, elapsed, step_ms, k, st = timed_steps(batch, dp.descs, npairs, a.steps, a.warmup)
-    value, graph_used = npairs * a.steps / elapsed, False
+    scores, elapsed, step_ms, k, st = timed_steps(batch, dp.descs, npairs, steps, a.warmup)
+    value, graph_used = npairs * steps / elapsed, False
     out_extra = {}
     if graph:
         # the same shape captured once in a hipGraph and replayed each step
         gb = api.SSIMU2Batch(npairs, w, h, fp16=fp16, map_tile_height=th, persistent_grid=grid, use_graph=True)
-        gscores, gel, gstep, _, _ = timed_steps(gb, dp.descs, npairs, a.steps, a.warmup)
+        gscores, gel, gstep, _, _ = timed_steps(gb, dp.descs, npairs, steps, a.warmup)
         gb.close()
         if not np.array_equal(gscores, scores):
             raise RuntimeError("graph replay scores differ from eager launches")
-        gval = npairs * a.steps / gel
+        gval = npairs * steps / gel
         out_extra["eager_value"] = round(value, 2)
         out_extra["graph_value"] = round(gval, 2)
         value, elapsed, step_ms, graph_used = gval, gel, gstep, True
     map_ms = float(np.mean(k["map"]))
-    out = record(a, value, elapsed, a.steps, "f16-storage/f32" if fp16 else "f32", workload,
+    out = record(a, value, elapsed, steps, "f16-storage/f32" if fp16 else "f32", workload,
                  {"pairs_per_step": npairs, "map_tile_height": st["map_tile_height"],
                   "persistent_grid": grid, "hip_graph": graph_used})
     out.update(out_extra)
@@ -231,12 +288,12 @@ def run_batched(a, pairs, bits, fp16, workload, tune=False, graph=False):
                          "finalize": round(float(np.mean(k["fin"])), 3)}
     if tuned is not None:
         out["autotune"] = tuned
-    if not a.no_per_frame:
+    if per_frame and not a.no_per_frame:
         pf = per_frame_rate(dp.descs, npairs, w, h, fp16)
         out["per_frame_path"] = {"value": round(pf, 2), "unit": "frame-pairs/s",
                                  "note": "one pair per launch sequence, eager, same device inputs",
                                  "batched_over_per_frame": round(value / pf, 2)}
-    out["roofline"] = roofline(st, map_ms, fp16, a.config)
+    out["roofline"] = roofline(st, map_ms, float(np.mean(k["prep"])), fp16, config)
     batch.close()
     dp.free()
     return out, scores
@@ -416,13 +473,23 @@ def butter_cpu_baseline(pairs, norms, a):
     return base, {"pairs": list(range(n)), "max_rel_norm_diff": max(rel)}
 
 
-def cpu_baseline(pairs, bits, gpu_scores, a):
-    """FP32 CPU restatement timed on the host cores (bounded sample) and the FP64
-    oracle on a few pairs for |dscore|. bits: 0 for RGB8, else per-pair list or int."""
+def host_cpu():
+    """The host the CPU baseline ran on: logical CPUs and the model name."""
+    model = None
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    model = line.split(":", 1)[1].strip()
+                    break
+    except OSError:
+        pass
+    return {"nproc": os.cpu_count(), "model": model}
+
+
+def _score_fn(pairs, bits):
     from tests import oracle_lib as ol
 
-    threads = min(os.cpu_count() or 1, 16)
-    ol.set_threads(threads)
     bl = bits if isinstance(bits, list) else [bits] * len(pairs)
 
     def score(i, prec):
@@ -432,19 +499,120 @@ def cpu_baseline(pairs, bits, gpu_scores, a):
         m = ol.SO_MAT_BT709 if bl[i] == 8 else ol.SO_MAT_BT2020NCL
         return ol.score_yuv420(r, d, bl[i], m, prec)[0]
 
-    n, t0 = 0, time.perf_counter()
-    while n < len(pairs) and time.perf_counter() - t0 < a.cpu_seconds:
-        score(n, "f32")
-        n += 1
+    return score
+
+
+def oracle_delta(pairs, bits, gpu_scores, idx, threads):
+    """|GPU score - FP64 oracle score| on the pairs `idx`."""
+    from tests import oracle_lib as ol
+
+    ol.set_threads(threads)
+    score = _score_fn(pairs, bits)
+    diffs = [abs(float(gpu_scores[i]) - score(i, "f64")) for i in idx]
+    return {"pairs": [int(i) for i in idx], "abs": [float(x) for x in diffs], "max_abs": float(np.max(diffs)),
+            "mean_abs": float(np.mean(diffs))}
+
+
+def cpu_threads():
+    # the CPU share this process is given (16 threads per GPU on the MI355X
+    # boxes, set through OMP_NUM_THREADS), else every logical CPU
+    try:
+        return max(1, int(os.environ.get("OMP_NUM_THREADS", "")))
+    except ValueError:
+        return os.cpu_count() or 1
+
+
+def cpu_baseline(pairs, bits, gpu_scores, a):
+    """FP32 CPU restatement timed on the host's CPU share (bounded sample):
+    one pair on one thread first, then pairs scored concurrently, one per
+    thread (frame pairs are independent); and the FP64 oracle on a few pairs
+    for |dscore|. bits: 0 for RGB8, else per-pair list or int."""
+    import concurrent.futures as cf
+
+    from tests import oracle_lib as ol
+
+    threads = cpu_threads()
+    score = _score_fn(pairs, bits)
+    ol.set_threads(1)
+    t1 = time.perf_counter()
+    score(0, "f32")
+    one_rate = 1.0 / (time.perf_counter() - t1)
+    # enough pairs for about cpu_seconds of work on every thread, at least one per thread
+    n = int(min(len(pairs), max(threads, round(one_rate * threads * a.cpu_seconds))))
+    def one(i):
+        ol.set_threads(1)  # per-thread OpenMP setting: no nested teams
+        return score(i, "f32")
+
+    t0 = time.perf_counter()
+    with cf.ThreadPoolExecutor(max_workers=threads) as ex:
+        list(ex.map(one, range(n)))
     cpu_rate = n / (time.perf_counter() - t0)
     idx = sorted(set(np.linspace(0, len(pairs) - 1, a.oracle_pairs).astype(int).tolist()))
-    diffs = [abs(float(gpu_scores[i]) - score(i, "f64")) for i in idx]
+    delta = oracle_delta(pairs, bits, gpu_scores, idx, threads)
     base = {"value": round(cpu_rate, 4), "unit": "frame-pairs/s", "cores": threads, "kind": "port",
-            "sample": f"first {n} pairs of the workload, FP32 CPU restatement (OpenMP over rows), "
-                      f"{a.cpu_seconds:.0f}s budget"}
-    delta = {"pairs": idx, "abs": [float(x) for x in diffs], "max_abs": float(np.max(diffs)),
-             "mean_abs": float(np.mean(diffs))}
+            "sample": f"first {n} pairs of the workload, FP32 CPU restatement, one pair per thread on "
+                      f"{threads} threads",
+            "single_thread": {"value": round(one_rate, 4), "unit": "frame-pairs/s", "sample": "pair 0, 1 thread"},
+            "host": dict(host_cpu(), threads_used=threads)}
     return base, delta
+
+
+def sub_4k10(a):
+    """3840x2160 YUV420P10 BT.2020, FP16 planes for scales 0-1, 120 pairs."""
+    pairs = video_pairs(3840, 2160, 10, 120, seed=2)
+    out, scores = run_batched(a, pairs, 10, True, "ssimulacra2 3840x2160 YUV420P10 BT.2020 limited, FP16 planes "
+                              "for scales 0-1, 120 pairs per step, inputs resident in HBM", steps=a.sub_steps,
+                              config="4k10")
+    out["score_vs_fp64_oracle"] = oracle_delta(pairs, 10, scores, [0, 119], cpu_threads())
+    return out
+
+
+def mixed_pairs():
+    from vship_amd import synth
+
+    res = synth.batch_resolutions(64, seed=3)
+    clips, pairs = {}, []
+    for i, (w, h) in enumerate(res):
+        if (w, h) not in clips:
+            clips[(w, h)] = synth.video_clip(w, h, 8, 64, seed=w + h)
+        pairs.append(clips[(w, h)].frame(i))
+    for c in clips.values():
+        c.close()
+    return res, pairs
+
+
+def sub_mixed(a, steps=None, per_frame=False):
+    res, pairs = mixed_pairs()
+    out, scores = run_batched(a, pairs, 8, False, "ssimulacra2 uneven batch of 64 pairs, 640x360..3840x2160 "
+                              "YUV420P8 BT.709 limited, inputs resident in HBM", tune=True, graph=True,
+                              steps=steps, config="mixed", per_frame=per_frame)
+    out["config"]["resolutions"] = {f"{w}x{h}": res.count((w, h)) for (w, h) in sorted(set(res))}
+    idx = sorted({0, res.index((3840, 2160)), 63})
+    out["score_vs_fp64_oracle"] = oracle_delta(pairs, 8, scores, idx, cpu_threads())
+    return out, pairs, scores
+
+
+def sub_served(a):
+    out, host_pool, alone = run_served(a)
+    pairs, scores, bits = [], [], []
+    for key, pp in host_pool.items():
+        pairs += pp[:1]
+        scores.append(alone[key][0])
+        bits.append(key[2])
+    # the first pair of the 1920x1080 8-bit, 3840x2160 10-bit and 640x360 10-bit pools
+    keys = list(host_pool)
+    idx = [keys.index(k) for k in ((1920, 1080, 8), (3840, 2160, 10), (640, 360, 10)) if k in keys]
+    out["score_vs_fp64_oracle"] = oracle_delta(pairs, bits, scores, idx, cpu_threads())
+    out["score_vs_fp64_oracle"]["note"] = "first pair of the 1080p 8-bit, 4K 10-bit and 640x360 10-bit pools, " \
+                                          "scores equal to the served ones (clips_equal_alone)"
+    return out
+
+
+def brief(rec):
+    keep = ("metric", "value", "unit", "steps", "ms_per_step", "dtype", "config", "latency_ms", "kernels_ms",
+            "per_frame_path", "roofline", "score_vs_fp64_oracle", "clips_equal_alone", "server", "eager_value",
+            "graph_value", "host_input")
+    return {k: rec[k] for k in keep if k in rec}
 
 
 def main():
@@ -465,7 +633,20 @@ def main():
         print(f"# generated in {time.time() - t0:.1f}s", flush=True)
         out, scores = run_batched(a, pairs, 8, False, "ssimulacra2 1920x1080 YUV420P8 BT.709 limited, "
                                   "240 pairs per step, inputs resident in HBM")
-    elif a.config == "rgb256":
+        if not a.no_cpu:
+            out["cpu_baseline"], out["score_vs_fp64_oracle"] = cpu_baseline(pairs, bits, scores, a)
+        if not a.no_sub:
+            del pairs
+            subs = {}
+            for name, fn in (("4k10", sub_4k10), ("mixed", lambda x: sub_mixed(x, x.sub_steps)[0]),
+                             ("served", sub_served)):
+                t1 = time.time()
+                subs[name] = brief(fn(a))
+                print(f"# sub-record {name}: {subs[name]['value']} pairs/s ({time.time() - t1:.0f}s)", flush=True)
+            out["sub_records"] = subs
+        print(json.dumps(out), flush=True)
+        return
+    if a.config == "rgb256":
         pairs, bits = [synth.config0()], 0
         out, scores = run_batched(a, pairs, 0, False, "ssimulacra2 256x256 planar RGB8 full range, 1 pair per step")
     elif a.config == "4k10":
@@ -474,17 +655,8 @@ def main():
         out, scores = run_batched(a, pairs, 10, True, "ssimulacra2 3840x2160 YUV420P10 BT.2020 limited, FP16 "
                                   "planes for scales 0-1, 120 pairs per step, inputs resident in HBM")
     elif a.config == "mixed":
-        res = synth.batch_resolutions(64, seed=3)
-        clips, pairs = {}, []
-        for i, (w, h) in enumerate(res):
-            if (w, h) not in clips:
-                clips[(w, h)] = synth.video_clip(w, h, 8, 64, seed=w + h)
-            pairs.append(clips[(w, h)].frame(i))
         bits = 8
-        print(f"# generated in {time.time() - t0:.1f}s", flush=True)
-        out, scores = run_batched(a, pairs, 8, False, "ssimulacra2 uneven batch of 64 pairs, 640x360..3840x2160 "
-                                  "YUV420P8 BT.709 limited, inputs resident in HBM", tune=True, graph=True)
-        out["config"]["resolutions"] = {f"{w}x{h}": res.count((w, h)) for (w, h) in sorted(set(res))}
+        out, pairs, scores = sub_mixed(a, per_frame=True)
     elif a.config == "butter1080p":
         out, pairs, norms = run_butter(a)
         if not a.no_cpu:

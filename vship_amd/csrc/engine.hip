@@ -27,6 +27,20 @@ int input_bytes_per_sample(int fmt) {
 int plane_w(const PairInput& p, int pl) { return pl > 0 ? (p.w + (1 << fmt_csw(p.fmt)) - 1) >> fmt_csw(p.fmt) : p.w; }
 int plane_h(const PairInput& p, int pl) { return pl > 0 ? (p.h + (1 << fmt_csh(p.fmt)) - 1) >> fmt_csh(p.fmt) : p.h; }
 
+// Every plane must lie inside one device allocation with rows that hold the
+// image: a bad pointer or stride becomes an error here instead of a GPU fault.
+void validate_planes(const PairInput& p) {
+    const int bps = input_bytes_per_sample(p.fmt);
+    for (int im = 0; im < 2; im++)
+        for (int pl = 0; pl < 3; pl++) {
+            const int64_t row = (int64_t)plane_w(p, pl) * bps;
+            if (!p.src[im][pl] || p.stride[im][pl] < row) VSHIP_THROW(E_BadPointer);
+            require_device_rows(p.src[im][pl], p.stride[im][pl], plane_h(p, pl), row);
+        }
+}
+
+}  // namespace
+
 // Normalised 17-tap Gaussian, sigma 1.5, radius 8 (src/ssimu2/gaussianblur.hpp:7-18,
 // src/util/preprocessor.hpp:89-92), recomputed in double and checked against
 // the compile-time taps the kernels use.
@@ -44,20 +58,6 @@ Taps make_taps() {
     }
     return t;
 }
-
-// Every plane must lie inside one device allocation with rows that hold the
-// image: a bad pointer or stride becomes an error here instead of a GPU fault.
-void validate_planes(const PairInput& p) {
-    const int bps = input_bytes_per_sample(p.fmt);
-    for (int im = 0; im < 2; im++)
-        for (int pl = 0; pl < 3; pl++) {
-            const int64_t row = (int64_t)plane_w(p, pl) * bps;
-            if (!p.src[im][pl] || p.stride[im][pl] < row) VSHIP_THROW(E_BadPointer);
-            require_device_rows(p.src[im][pl], p.stride[im][pl], plane_h(p, pl), row);
-        }
-}
-
-}  // namespace
 
 void require_device_rows(const void* ptr, int64_t stride, int rows, int64_t row_bytes) {
     if (!ptr || rows < 1 || row_bytes < 1 || stride < row_bytes) VSHIP_THROW(E_BadPointer);
@@ -140,18 +140,11 @@ Engine::Engine(const EngineOptions& opt) : opt_(opt) {
     taps_ = make_taps();
     try {
         HIP_OR_THROW(hipStreamCreateWithFlags(&stream_, hipStreamNonBlocking), E_OutOfVRAM);
-        const size_t nseg = (size_t)kScales * opt_.max_pairs;
-        off_pairs_ = round_up(sizeof(BatchHdr), 64);
-        off_prep_ = off_pairs_ + round_up(sizeof(PairDev) * opt_.max_pairs, 64);
-        off_segstart_ = off_prep_ + round_up(sizeof(int32_t) * (opt_.max_pairs + 1), 64);
-        off_segcount_ = off_segstart_ + round_up(sizeof(int32_t) * nseg, 64);
-        off_order_ = off_segcount_ + round_up(sizeof(int32_t) * nseg, 64);
-        off_ostart_ = off_order_ + round_up(sizeof(int32_t) * nseg, 64);
-        desc_bytes_ = off_ostart_ + round_up(sizeof(int32_t) * (nseg + 1), 64);
-        HIP_OR_THROW(hipHostMalloc(reinterpret_cast<void**>(&desc_h_), desc_bytes_, hipHostMallocDefault),
+        desc_ = desc_layout(opt_.max_pairs);
+        HIP_OR_THROW(hipHostMalloc(reinterpret_cast<void**>(&desc_h_), desc_.bytes, hipHostMallocDefault),
                      E_OutOfRAM);
-        std::memset(desc_h_, 0, desc_bytes_);
-        HIP_OR_THROW(hipMalloc(reinterpret_cast<void**>(&desc_d_), desc_bytes_), E_OutOfVRAM);
+        std::memset(desc_h_, 0, desc_.bytes);
+        HIP_OR_THROW(hipMalloc(reinterpret_cast<void**>(&desc_d_), desc_.bytes), E_OutOfVRAM);
         arena_bytes_ = opt_.arena_bytes / 256 * 256;
         if (arena_bytes_ <= 0) VSHIP_THROW(E_OutOfVRAM);
         HIP_OR_THROW(hipMalloc(&arena_d_, (size_t)arena_bytes_), E_OutOfVRAM);
@@ -198,15 +191,42 @@ void Engine::release() {
     stream_ = nullptr;
 }
 
-void Engine::plan(const PairInput* in, int n) {
-    if (n < 1 || n > opt_.max_pairs || !in) VSHIP_THROW(E_BadPointer);
-    BatchHdr* hdr = reinterpret_cast<BatchHdr*>(desc_h_);
-    PairDev* pd = reinterpret_cast<PairDev*>(desc_h_ + off_pairs_);
-    int32_t* prep_start = reinterpret_cast<int32_t*>(desc_h_ + off_prep_);
-    int32_t* seg_start = reinterpret_cast<int32_t*>(desc_h_ + off_segstart_);
-    int32_t* seg_count = reinterpret_cast<int32_t*>(desc_h_ + off_segcount_);
-    int32_t* order_seg = reinterpret_cast<int32_t*>(desc_h_ + off_order_);
-    int32_t* order_start = reinterpret_cast<int32_t*>(desc_h_ + off_ostart_);
+DescLayout desc_layout(int max_pairs) {
+    DescLayout L;
+    const size_t nseg = (size_t)kScales * max_pairs;
+    L.off_pairs = round_up(sizeof(BatchHdr), 64);
+    L.off_prep = L.off_pairs + round_up(sizeof(PairDev) * max_pairs, 64);
+    L.off_segstart = L.off_prep + round_up(sizeof(int32_t) * (max_pairs + 1), 64);
+    L.off_segcount = L.off_segstart + round_up(sizeof(int32_t) * nseg, 64);
+    L.off_order = L.off_segcount + round_up(sizeof(int32_t) * nseg, 64);
+    L.off_ostart = L.off_order + round_up(sizeof(int32_t) * nseg, 64);
+    L.bytes = L.off_ostart + round_up(sizeof(int32_t) * (nseg + 1), 64);
+    return L;
+}
+
+DescView desc_view(const uint8_t* base, const DescLayout& L) {
+    DescView d;
+    d.hdr = reinterpret_cast<const BatchHdr*>(base);
+    d.pairs = reinterpret_cast<const PairDev*>(base + L.off_pairs);
+    d.prep_start = reinterpret_cast<const int32_t*>(base + L.off_prep);
+    d.seg_start = reinterpret_cast<const int32_t*>(base + L.off_segstart);
+    d.seg_count = reinterpret_cast<const int32_t*>(base + L.off_segcount);
+    d.order_seg = reinterpret_cast<const int32_t*>(base + L.off_order);
+    d.order_start = reinterpret_cast<const int32_t*>(base + L.off_ostart);
+    return d;
+}
+
+PlanInfo plan_batch(const PairInput* in, int n, int max_pairs, bool fp16, int map_th, int64_t arena_bytes,
+                    int64_t max_map_tiles, uint8_t* block, const DescLayout& DL) {
+    if (n < 1 || n > max_pairs || !in) VSHIP_THROW(E_BadPointer);
+    PlanInfo P;
+    BatchHdr* hdr = reinterpret_cast<BatchHdr*>(block);
+    PairDev* pd = reinterpret_cast<PairDev*>(block + DL.off_pairs);
+    int32_t* prep_start = reinterpret_cast<int32_t*>(block + DL.off_prep);
+    int32_t* seg_start = reinterpret_cast<int32_t*>(block + DL.off_segstart);
+    int32_t* seg_count = reinterpret_cast<int32_t*>(block + DL.off_segcount);
+    int32_t* order_seg = reinterpret_cast<int32_t*>(block + DL.off_order);
+    int32_t* order_start = reinterpret_cast<int32_t*>(block + DL.off_ostart);
     int64_t arena_off = 0;
     int64_t ptiles = 0;
     uint64_t mbytes = 0, pbytes = 0;
@@ -216,7 +236,7 @@ void Engine::plan(const PairInput* in, int n) {
         if (fmt_wide(p.fmt) && (p.bits < 9 || p.bits > 16)) VSHIP_THROW(E_BadPointer);
         if (p.chroma_loc < CLOC_LEFT || p.chroma_loc > CLOC_TOP) VSHIP_THROW(E_BadPointer);
         validate_planes(p);
-        const PairLayout L = pair_layout(p.w, p.h, opt_.fp16);
+        const PairLayout L = pair_layout(p.w, p.h, fp16);
         PairDev& d = pd[i];
         std::memset(&d, 0, sizeof(d));
         for (int im = 0; im < 2; im++)
@@ -242,20 +262,18 @@ void Engine::plan(const PairInput* in, int n) {
             d.sh[s] = L.sh[s];
         }
         arena_off += 2 * L.img_bytes;
-        if (arena_off > arena_bytes_) VSHIP_THROW(E_OutOfVRAM);
+        if (arena_off > arena_bytes) VSHIP_THROW(E_OutOfVRAM);
         prep_start[i] = (int32_t)ptiles;
         ptiles += prep_tiles(p.w, p.h);
-        mbytes += map_algorithmic_bytes(p.w, p.h, opt_.fp16);
-        pbytes += prep_algorithmic_bytes(p, opt_.fp16);
+        mbytes += map_algorithmic_bytes(p.w, p.h, fp16);
+        pbytes += prep_algorithmic_bytes(p, fp16);
     }
     // map segments, group by group: class 0 (FP32 moments) before class 1
     // (FP64), FP32 planes before FP16 planes within a class
     int64_t mtiles = 0;
     int k = 0;
-    int nseg_grp[kMapGroups] = {}, tiles_grp[kMapGroups] = {};
-    for (int& c : ntiles_cls_) c = 0;
-    for (uint64_t& b : cls_bytes_) b = 0;
-    const bool mfma = opt_.map_th == kMapMfma;
+    int nseg_grp[kMapGroups] = {};
+    const bool mfma = map_th == kMapMfma;
     for (int g = 0; g < kMapGroups; g++) {
         const int cls = g < 4 ? g / 2 : 0;
         const bool half = g & 1;
@@ -266,12 +284,12 @@ void Engine::plan(const PairInput* in, int n) {
                 const bool float_in = pd[i].fmt == IN_RGB_F16 || pd[i].fmt == IN_RGB_F32;
                 if ((mfma && cls == 0 && float_in) != exact) continue;
                 if (((int64_t)sw * sh < kSmallSegment ? 1 : 0) != cls) continue;
-                if ((opt_.fp16 && s < kHalfScales) != half) continue;
-                const int th = cls ? kSmallTH : opt_.map_th;
+                if ((fp16 && s < kHalfScales) != half) continue;
+                const int th = cls ? kSmallTH : map_th;
                 const int tw = map_tile_w(th), tht = map_tile_h(th);
                 const int64_t cnt = (int64_t)((sw + tw - 1) / tw) * ((sh + tht - 1) / tht);
                 const int seg = i * kScales + s;
-                cls_bytes_[cls] += (uint64_t)sw * sh * 2 * 3 * scale_elem_bytes(s, opt_.fp16);
+                P.cls_bytes[cls] += (uint64_t)sw * sh * 2 * 3 * scale_elem_bytes(s, fp16);
                 seg_start[seg] = (int32_t)mtiles;
                 seg_count[seg] = (int32_t)cnt;
                 order_seg[k] = seg;
@@ -279,8 +297,8 @@ void Engine::plan(const PairInput* in, int n) {
                 k++;
                 mtiles += cnt;
                 nseg_grp[g]++;
-                tiles_grp[g] += (int)cnt;
-                if (mtiles > max_map_tiles_ || mtiles > INT32_MAX) VSHIP_THROW(E_OutOfVRAM);
+                P.ntiles_grp[g] += (int)cnt;
+                if (mtiles > max_map_tiles || mtiles > INT32_MAX) VSHIP_THROW(E_OutOfVRAM);
             }
     }
     order_start[k] = (int32_t)mtiles;
@@ -291,79 +309,88 @@ void Engine::plan(const PairInput* in, int n) {
     hdr->npairs = n;
     hdr->prep_tiles = (int32_t)ptiles;
     hdr->prep_generic_pairs = ngen;
-    nprep_generic_ = ngen;
     hdr->map_tiles = (int32_t)mtiles;
-    hdr->map_th = opt_.map_th;
+    hdr->map_th = map_th;
     for (int g = 0; g < kMapGroups; g++) {
         hdr->nseg[g] = nseg_grp[g];
-        hdr->tiles[g] = tiles_grp[g];
-        ntiles_grp_[g] = tiles_grp[g];
-        ntiles_cls_[g < 4 ? g / 2 : 0] += tiles_grp[g];
+        hdr->tiles[g] = P.ntiles_grp[g];
+        P.ntiles_cls[g < 4 ? g / 2 : 0] += P.ntiles_grp[g];
     }
-    npairs_ = n;
-    ntiles_prep_ = (int)ptiles;
-    ntiles_map_ = (int)mtiles;
-    map_bytes_ = mbytes;
-    prep_bytes_ = pbytes;
+    P.nprep_generic = ngen;
+    P.npairs = n;
+    P.ntiles_prep = (int)ptiles;
+    P.ntiles_map = (int)mtiles;
+    P.map_bytes = mbytes;
+    P.prep_bytes = pbytes;
+    return P;
 }
 
-void Engine::enqueue_body() {
-    const bool graph = opt_.use_graph;
-    const BatchHdr* hdr = reinterpret_cast<const BatchHdr*>(desc_d_);
-    const PairDev* pd = reinterpret_cast<const PairDev*>(desc_d_ + off_pairs_);
-    const int32_t* prep_start = reinterpret_cast<const int32_t*>(desc_d_ + off_prep_);
-    const int32_t* seg_start = reinterpret_cast<const int32_t*>(desc_d_ + off_segstart_);
-    const int32_t* seg_count = reinterpret_cast<const int32_t*>(desc_d_ + off_segcount_);
-    const int32_t* order_seg = reinterpret_cast<const int32_t*>(desc_d_ + off_order_);
-    const int32_t* order_start = reinterpret_cast<const int32_t*>(desc_d_ + off_ostart_);
-    const bool prof = opt_.profile && !graph;
-    const int fixed = opt_.persistent_grid > 0 ? opt_.persistent_grid : 4 * num_cus_;
+void Engine::plan(const PairInput* in, int n) {
+    plan_ = plan_batch(in, n, opt_.max_pairs, opt_.fp16, opt_.map_th, arena_bytes_, max_map_tiles_, desc_h_, desc_);
+}
+
+void launch_batch(const DescView& d, const PlanInfo& P, const LaunchShape& S, void* arena, double* partials,
+                  double* scores, double* features, const Taps& taps, hipStream_t st, hipEvent_t* ev) {
+    const bool graph = S.graph;
+    const int fixed = S.persistent_grid > 0 ? S.persistent_grid : 4 * S.num_cus;
     auto grid_for = [&](int work, int cap_in_graph) {
         if (graph) return cap_in_graph;
-        if (opt_.persistent_grid > 0) return std::max(1, std::min(work, fixed));
+        if (S.persistent_grid > 0) return std::max(1, std::min(work, fixed));
         return std::max(1, work);
     };
-    const int g_prep = grid_for(ntiles_prep_, fixed);
-    const int g_fin = grid_for(npairs_, std::min(opt_.max_pairs, fixed));
-    const size_t copy_bytes = desc_bytes_;
-    if (prof) HIP_OR_THROW(hipEventRecord(ev_[0], stream_), E_BadDeviceCode);
-    HIP_OR_THROW(hipMemcpyAsync(desc_d_, desc_h_, copy_bytes, hipMemcpyHostToDevice, stream_), E_BadDeviceCode);
-    if (prof) HIP_OR_THROW(hipEventRecord(ev_[1], stream_), E_BadDeviceCode);
-    HIP_OR_THROW(launch_prep(opt_.fp16, false, g_prep, pd, prep_start, hdr, arena_d_, stream_), E_BadDeviceCode);
+    const int g_prep = grid_for(P.ntiles_prep, fixed);
+    const int g_fin = grid_for(P.npairs, std::min(S.max_pairs, fixed));
+    HIP_OR_THROW(launch_prep(S.fp16, false, g_prep, d.pairs, d.prep_start, d.hdr, arena, st), E_BadDeviceCode);
     // pairs on the generic chroma path have their own prep kernel; a captured
     // graph always holds it (it returns at once when the batch has none)
-    if (graph || nprep_generic_ > 0)
-        HIP_OR_THROW(launch_prep(opt_.fp16, true, g_prep, pd, prep_start, hdr, arena_d_, stream_), E_BadDeviceCode);
-    if (prof) HIP_OR_THROW(hipEventRecord(ev_[2], stream_), E_BadDeviceCode);
+    if (graph || P.nprep_generic > 0)
+        HIP_OR_THROW(launch_prep(S.fp16, true, g_prep, d.pairs, d.prep_start, d.hdr, arena, st), E_BadDeviceCode);
+    if (ev) HIP_OR_THROW(hipEventRecord(ev[0], st), E_BadDeviceCode);
     // one launch per map group; FP16-plane groups exist only with FP16 storage,
     // and an eager launch is skipped when its group is empty
-    const bool mfma = opt_.map_th == kMapMfma;
+    const bool mfma = S.map_th == kMapMfma;
     for (int cls = 0; cls < 2; cls++) {
         // on the matrix cores, class 0 also has the exact-split groups of float-input pairs
         const int nexact = (cls == 0 && mfma) ? 2 : 1;
         for (int exact = 0; exact < nexact; exact++)
             for (int half = 0; half < 2; half++) {
                 const int g = map_group(cls, half, exact);
-                if (half && !opt_.fp16) continue;
-                if (!graph && ntiles_grp_[g] == 0) continue;
-                const int grid = grid_for(ntiles_grp_[g], cls == 0 ? fixed : std::max(1, fixed / 4));
+                if (half && !S.fp16) continue;
+                if (!graph && P.ntiles_grp[g] == 0) continue;
+                const int grid = grid_for(P.ntiles_grp[g], cls == 0 ? fixed : std::max(1, fixed / 4));
                 if (cls == 0 && mfma)
-                    HIP_OR_THROW(launch_map_mx(half, exact, grid, pd, order_seg, order_start, hdr, arena_d_,
-                                               partials_d_, stream_),
+                    HIP_OR_THROW(launch_map_mx(half, exact, grid, d.pairs, d.order_seg, d.order_start, d.hdr, arena,
+                                               partials, st),
                                  E_BadDeviceCode);
                 else
-                    HIP_OR_THROW(launch_map(half, cls, cls ? kSmallTH : opt_.map_th, grid, pd, order_seg, order_start,
-                                            hdr, arena_d_, partials_d_, taps_, stream_),
+                    HIP_OR_THROW(launch_map(half, cls, cls ? kSmallTH : S.map_th, grid, d.pairs, d.order_seg,
+                                            d.order_start, d.hdr, arena, partials, taps, st),
                                  E_BadDeviceCode);
             }
-        if (prof) HIP_OR_THROW(hipEventRecord(ev_[3 + cls], stream_), E_BadDeviceCode);
+        if (ev) HIP_OR_THROW(hipEventRecord(ev[1 + cls], st), E_BadDeviceCode);
     }
+    HIP_OR_THROW(launch_finalize(g_fin, d.pairs, d.seg_start, d.seg_count, d.hdr, partials, scores, features, st),
+                 E_BadDeviceCode);
+    if (ev) HIP_OR_THROW(hipEventRecord(ev[3], st), E_BadDeviceCode);
+}
+
+void Engine::enqueue_body() {
+    const bool prof = opt_.profile && !opt_.use_graph;
+    LaunchShape S;
+    S.fp16 = opt_.fp16;
+    S.map_th = opt_.map_th;
+    S.persistent_grid = opt_.persistent_grid;
+    S.graph = opt_.use_graph;
+    S.max_pairs = opt_.max_pairs;
+    S.num_cus = num_cus_;
+    if (prof) HIP_OR_THROW(hipEventRecord(ev_[0], stream_), E_BadDeviceCode);
+    HIP_OR_THROW(hipMemcpyAsync(desc_d_, desc_h_, desc_.bytes, hipMemcpyHostToDevice, stream_), E_BadDeviceCode);
+    if (prof) HIP_OR_THROW(hipEventRecord(ev_[1], stream_), E_BadDeviceCode);
     double* scores_d = out_d_;
     double* feats_d = out_d_ + opt_.max_pairs;
-    HIP_OR_THROW(launch_finalize(g_fin, pd, seg_start, seg_count, hdr, partials_d_, scores_d, feats_d, stream_),
-                 E_BadDeviceCode);
-    if (prof) HIP_OR_THROW(hipEventRecord(ev_[5], stream_), E_BadDeviceCode);
-    const int np = graph ? opt_.max_pairs : npairs_;
+    launch_batch(desc_view(desc_d_, desc_), plan_, S, arena_d_, partials_d_, scores_d, feats_d, taps_, stream_,
+                 prof ? ev_ + 2 : nullptr);
+    const int np = opt_.use_graph ? opt_.max_pairs : plan_.npairs;
     HIP_OR_THROW(hipMemcpyAsync(out_h_, scores_d, sizeof(double) * np, hipMemcpyDeviceToHost, stream_),
                  E_BadDeviceCode);
     HIP_OR_THROW(hipMemcpyAsync(out_h_ + opt_.max_pairs, feats_d, sizeof(double) * 108 * np,
@@ -401,8 +428,8 @@ void Engine::wait(double* scores, double* features) {
     if (!pending_) VSHIP_THROW(E_BadHandler);
     pending_ = false;
     HIP_OR_THROW(hipStreamSynchronize(stream_), E_BadDeviceCode);
-    if (scores) std::memcpy(scores, out_h_, sizeof(double) * npairs_);
-    if (features) std::memcpy(features, out_h_ + opt_.max_pairs, sizeof(double) * 108 * npairs_);
+    if (scores) std::memcpy(scores, out_h_, sizeof(double) * plan_.npairs);
+    if (features) std::memcpy(features, out_h_ + opt_.max_pairs, sizeof(double) * 108 * plan_.npairs);
     if (opt_.profile && !opt_.use_graph) {
         (void)hipEventElapsedTime(&timings_.prep_ms, ev_[1], ev_[2]);
         (void)hipEventElapsedTime(&timings_.map32_ms, ev_[2], ev_[3]);

@@ -100,6 +100,55 @@ int map_tiles(int w, int h, int th);
 uint64_t map_algorithmic_bytes(int w, int h, bool fp16);
 uint64_t prep_algorithmic_bytes(const PairInput& p, bool fp16);
 
+// Descriptor block of a batch, one allocation (host copy, then device copy):
+//   [BatchHdr][PairDev x max_pairs][prep_start x (max_pairs+1)]
+//   [seg_start x 6 max_pairs][seg_count x 6 max_pairs][order_seg x 6 max_pairs][order_start x (6 max_pairs+1)]
+struct DescLayout {
+    size_t off_pairs = 0, off_prep = 0, off_segstart = 0, off_segcount = 0, off_order = 0, off_ostart = 0;
+    size_t bytes = 0;
+};
+DescLayout desc_layout(int max_pairs);
+
+// Pointers into a descriptor block at `base`.
+struct DescView {
+    const BatchHdr* hdr;
+    const PairDev* pairs;
+    const int32_t *prep_start, *seg_start, *seg_count, *order_seg, *order_start;
+};
+DescView desc_view(const uint8_t* base, const DescLayout& L);
+
+// What planning a batch found: tile counts per kernel and group, bytes.
+struct PlanInfo {
+    int npairs = 0, ntiles_prep = 0, ntiles_map = 0, nprep_generic = 0;
+    int ntiles_cls[2] = {0, 0}, ntiles_grp[kMapGroups] = {};
+    uint64_t map_bytes = 0, prep_bytes = 0, cls_bytes[2] = {0, 0};
+};
+
+// Plans pairs [0, n) into the host descriptor block `block` (layout L for at
+// most max_pairs pairs): checks every pair (bad sizes, formats and planes
+// throw E_BadPointer; an arena or tile table too small throws E_OutOfVRAM)
+// and lays the pairs out in an arena of arena_bytes.
+PlanInfo plan_batch(const PairInput* in, int n, int max_pairs, bool fp16, int map_th, int64_t arena_bytes,
+                    int64_t max_map_tiles, uint8_t* block, const DescLayout& L);
+
+// Launch shape of a batch's kernels.
+struct LaunchShape {
+    bool fp16 = false;
+    int map_th = kMapMfma;
+    int persistent_grid = 0;  // 0: one workgroup per tile
+    bool graph = false;       // fixed grids (capture): every group launched, sized by max_pairs
+    int max_pairs = 1;
+    int num_cus = 256;
+};
+
+// Enqueues the prep, map and finalize kernels of a planned batch on st (no
+// copies). ev, if not null, holds 4 events recorded after the prep kernels,
+// after each map class and after finalize.
+void launch_batch(const DescView& d, const PlanInfo& P, const LaunchShape& S, void* arena, double* partials,
+                  double* scores, double* features, const Taps& taps, hipStream_t st, hipEvent_t* ev);
+
+Taps make_taps();
+
 class Engine {
 public:
     explicit Engine(const EngineOptions& opt);
@@ -115,13 +164,13 @@ public:
     const EngineOptions& options() const { return opt_; }
     hipStream_t stream() const { return stream_; }
     const Timings& timings() const { return timings_; }
-    int last_npairs() const { return npairs_; }
-    int last_prep_tiles() const { return ntiles_prep_; }
-    int last_map_tiles() const { return ntiles_map_; }
-    int last_map_tiles_class(int c) const { return ntiles_cls_[c]; }
-    uint64_t last_map_bytes() const { return map_bytes_; }
-    uint64_t last_map_bytes_class(int c) const { return cls_bytes_[c]; }
-    uint64_t last_prep_bytes() const { return prep_bytes_; }
+    int last_npairs() const { return plan_.npairs; }
+    int last_prep_tiles() const { return plan_.ntiles_prep; }
+    int last_map_tiles() const { return plan_.ntiles_map; }
+    int last_map_tiles_class(int c) const { return plan_.ntiles_cls[c]; }
+    uint64_t last_map_bytes() const { return plan_.map_bytes; }
+    uint64_t last_map_bytes_class(int c) const { return plan_.cls_bytes[c]; }
+    uint64_t last_prep_bytes() const { return plan_.prep_bytes; }
     int num_cus() const { return num_cus_; }
 
 private:
@@ -134,10 +183,7 @@ private:
     int num_cus_ = 0;
     Taps taps_{};
 
-    // descriptor block: [BatchHdr][PairDev x max_pairs][prep_start x (max_pairs+1)]
-    //   [seg_start x 6 max_pairs][seg_count x 6 max_pairs][order_seg x 6 max_pairs][order_start x (6 max_pairs+1)]
-    size_t desc_bytes_ = 0;
-    size_t off_pairs_ = 0, off_prep_ = 0, off_segstart_ = 0, off_segcount_ = 0, off_order_ = 0, off_ostart_ = 0;
+    DescLayout desc_;
     uint8_t* desc_h_ = nullptr;  // pinned
     uint8_t* desc_d_ = nullptr;
     void* arena_d_ = nullptr;
@@ -150,9 +196,7 @@ private:
     hipGraph_t graph_ = nullptr;
     hipGraphExec_t graph_exec_ = nullptr;
 
-    int nprep_generic_ = 0;  // pairs of the batch on the generic chroma path
-    int npairs_ = 0, ntiles_prep_ = 0, ntiles_map_ = 0, ntiles_cls_[2] = {0, 0}, ntiles_grp_[kMapGroups] = {};
-    uint64_t map_bytes_ = 0, prep_bytes_ = 0, cls_bytes_[2] = {0, 0};
+    PlanInfo plan_;  // the last planned batch
     bool pending_ = false;
     Timings timings_;
 };

@@ -97,6 +97,14 @@ struct SplitBf16x3 {
             w[2][k] = pack2(l0, l1);
         }
     }
+    // split of x * s (row-pass results, re-split at the data scale)
+    template <int N>
+    static __device__ __forceinline__ void split_scaled(const float* v, float s, uint32_t (&w)[3][N / 2]) {
+        float x[N];
+#pragma unroll
+        for (int k = 0; k < N; k++) x[k] = v[k] * s;
+        split<N>(x, w);
+    }
     // c + a.b from the six partial products above 2^-24, smallest first
     static __device__ __forceinline__ f32x4 mma(const Op& a, const Op& b, f32x4 c) {
         c = mfma_bf16(a.p[2], b.p[0], c);
@@ -139,9 +147,25 @@ struct SplitF16x2 {
             w[1][k] = __builtin_bit_cast(uint32_t, l);
         }
     }
-    // c + a.b: all four products of the two-part representations
+    // split of x * s: h = fp16(x s), l = fp16(fma(x, s, -h)) (the residual
+    // x s - h is exact in FP32 when s is a power of two times a float that
+    // keeps x s in range, and fma computes it in one rounding otherwise)
+    template <int N>
+    static __device__ __forceinline__ void split_scaled(const float* v, float s, uint32_t (&w)[2][N / 2]) {
+#pragma unroll
+        for (int k = 0; k < N / 2; k++) {
+            const f32x2 y = {v[2 * k] * s, v[2 * k + 1] * s};
+            const f16x2 h = __builtin_convertvector(y, f16x2);
+            const f32x2 r = {fmaf(v[2 * k], s, -(float)h[0]), fmaf(v[2 * k + 1], s, -(float)h[1])};
+            const f16x2 l = __builtin_convertvector(r, f16x2);
+            w[0][k] = __builtin_bit_cast(uint32_t, h);
+            w[1][k] = __builtin_bit_cast(uint32_t, l);
+        }
+    }
+    // c + a.b from the three products above the split's own error: with
+    // a = ah + al + ea, |al| <= 2^-12 |a|, |ea| <= 2^-23 |a| (and the same for
+    // b), the dropped al.bl is at most 2^-24 |a b|, below ea.b and a.eb
     static __device__ __forceinline__ f32x4 mma(const Op& a, const Op& b, f32x4 c) {
-        c = mfma_f16(a.p[1], b.p[1], c);
         c = mfma_f16(a.p[1], b.p[0], c);
         c = mfma_f16(a.p[0], b.p[1], c);
         c = mfma_f16(a.p[0], b.p[0], c);
@@ -230,25 +254,44 @@ __device__ __forceinline__ typename SP::Op tap_operand(int g, int j) {
 }
 
 // The three channels of one tile; leaves the 18 wave sums in red[wave].
-template <typename SP, typename ST>
-__device__ __forceinline__ void mx_tile(MxLds<SP::kParts>& L, const TileGeo& T, const ST* base1, const ST* base2,
-                                        const float* cum, int lane) {
+//
+// EDGE is false for tiles whose 80 x 80 input region lies inside the image:
+// no element is masked, every output pixel is in the image and every
+// renormalisation total is the full sum of the taps, so the factors fold into
+// constants. Edge tiles clamp their loads into the image, give out-of-image
+// elements the centre value (all four moments zero) and renormalise by the
+// in-bounds taps of each output row and column.
+//
+// Scales: the row pass multiplies operands at the data scale (kScaleAB for a
+// and b, kScaleDD^2 for (x1 - x2)^2, kScaleAB^2 kScaleQ for ab) by taps at
+// kTapScale; its results are re-split at the data scale times the column
+// factor (kmid); the column pass multiplies by the taps at kTapScale again.
+// An output o of moment m is therefore the renormalised blur times
+// scale_m kTapScale / (row factor), and k1 (per output row) undoes that for a
+// and b.
+template <typename SP, typename ST, bool EDGE>
+__device__ __forceinline__ void mx_tile(MxLds<SP::kParts>& L, const TileGeo& T, const typename SP::Op& G,
+                                        const ST* base1, const ST* base2, const float* cum, int lane) {
     constexpr int NP = SP::kParts;
     const int W = T.W, H = T.H, g = T.g, j = T.j, wave = T.wave, swz = T.swz;
-    const typename SP::Op G = tap_operand<SP>(g, j);
     const int64_t pitch = T.pitch;
-    // renormalisation: the factor of this lane's output column, with the scale
-    // of the products folded in, and a table of the tile's 64 row factors
-    // (lane l: row y0 + l) that the maps read four at a time
-    constexpr float kOutScale = 1.f / (SP::kScaleAB * SP::kTapScale);
-    const float invh = __builtin_amdgcn_rcpf(tap_total(cum, T.xo, W)) * kOutScale;
-    L.rowf[wave][lane] = __builtin_amdgcn_rcpf(tap_total(cum, min(T.y0 + lane, H - 1), H));
-    wave_lds_sync();
-    // E2 carries kScaleDD^2, Q carries kScaleAB^2 kScaleQ, where a, b carry kScaleAB
-    constexpr float kEOut = SP::kScaleAB / (SP::kScaleDD * SP::kScaleDD);
-    constexpr float kQOut = 1.f / (SP::kScaleAB * SP::kScaleQ);
-    // the row-pass results are re-split at the data scale
-    constexpr float kMid = 1.f / SP::kTapScale;
+    // moments relative to a and b: (x1 - x2)^2 and ab carry kScaleDD^2 and
+    // kScaleAB^2 kScaleQ where a and b carry kScaleAB
+    constexpr float kE = SP::kScaleAB / (SP::kScaleDD * SP::kScaleDD);
+    constexpr float kQ = 1.f / (SP::kScaleAB * SP::kScaleQ);
+    constexpr float kOut = 1.f / (SP::kScaleAB * SP::kTapScale);
+    // the full tap sum as the split represents it (interior renormalisation)
+    const float inv_total = __builtin_amdgcn_rcpf(cum[17]);
+    // re-split scale of this lane's row-pass results (output column T.xo)
+    const float kmid =
+        (EDGE ? __builtin_amdgcn_rcpf(tap_total(cum, T.xo, W)) : inv_total) * (1.f / SP::kTapScale);
+    if (EDGE) {
+        // per-wave table of k1 for the tile's 64 output rows (lane l: row y0 + l)
+        L.rowf[wave][lane] = __builtin_amdgcn_rcpf(tap_total(cum, min(T.y0 + lane, H - 1), H)) * kOut;
+        wave_lds_sync();
+    }
+    const float k1i = inv_total * kOut;
+    const int xc = EDGE ? min(T.xo, W - 1) : T.xo;
 
 #pragma unroll 1
     for (int c = 0; c < 3; c++) {
@@ -265,23 +308,28 @@ __device__ __forceinline__ void mx_tile(MxLds<SP::kParts>& L, const TileGeo& T, 
             const int yr = T.y0 - 8 + 16 * mb + j;
             typename SP::Op A[4];
             {
-                // one vector load per image for every lane: the row is clamped
-                // into the image and the columns outside it are masked to zero
                 float u1[8], u2[8];
-                const bool row_ok = yr >= 0 && yr < H;
-                const int64_t ro = (int64_t)min(max(yr, 0), H - 1) * pitch + T.xl;
+                const int64_t ro = (int64_t)(EDGE ? min(max(yr, 0), H - 1) : yr) * pitch + T.xl;
                 Load8<ST>::run(p1 + ro, u1);
                 Load8<ST>::run(p2 + ro, u2);
-                bool in[8];
+                if (EDGE) {
+                    // one vector load per image for every lane: the row is
+                    // clamped into the image and an element outside it takes
+                    // the value cc in both images, so that all four of its
+                    // moments are exactly zero (ccs = cc kScaleAB is exact:
+                    // the scale is a power of two)
+                    const bool row_ok = yr >= 0 && yr < H;
 #pragma unroll
-                for (int e = 0; e < 8; e++) in[e] = row_ok && ((T.cmask >> e) & 1);
-                // an element outside the image takes the value cc in both
-                // images, so that all four of its moments are exactly zero
-                // (ccs = cc kScaleAB is exact: the scale is a power of two)
+                    for (int e = 0; e < 8; e++) {
+                        const bool in = row_ok && ((T.cmask >> e) & 1);
+                        u1[e] = in ? u1[e] : cc;
+                        u2[e] = in ? u2[e] : cc;
+                    }
+                }
                 float v[4][8];
 #pragma unroll
                 for (int e = 0; e < 8; e++) {
-                    const float x1 = in[e] ? u1[e] : cc, x2 = in[e] ? u2[e] : cc;
+                    const float x1 = u1[e], x2 = u2[e];
                     const float a = fmaf(x1, SP::kScaleAB, -ccs);
                     const float b = fmaf(x2, SP::kScaleAB, -ccs);
                     const float dd = (x1 - x2) * SP::kScaleDD;
@@ -304,9 +352,9 @@ __device__ __forceinline__ void mx_tile(MxLds<SP::kParts>& L, const TileGeo& T, 
             for (int m = 0; m < 4; m++) {
                 const f32x4 hr = SP::mma(A[m], G, z);
                 // lane (g, j) holds rows 4g .. 4g+3 of the block, column j: store the split parts
-                const float rv[4] = {hr[0] * kMid, hr[1] * kMid, hr[2] * kMid, hr[3] * kMid};
+                const float rv[4] = {hr[0], hr[1], hr[2], hr[3]};
                 uint32_t w[NP][2];
-                SP::template split<4>(rv, w);
+                SP::template split_scaled<4>(rv, kmid, w);
 #pragma unroll
                 for (int p = 0; p < NP; p++)
                     *reinterpret_cast<uint2*>(&L.ring[wave][slot][m][p][j][(2 * g) ^ swz]) = make_uint2(w[p][0], w[p][1]);
@@ -327,45 +375,62 @@ __device__ __forceinline__ void mx_tile(MxLds<SP::kParts>& L, const TileGeo& T, 
             }
             wave_lds_sync();
             // ---- maps of this lane's output pixels: rows y0 + 16 ob + 4g + r, column xo
-            // The eight samples are loaded together, at the position clamped
-            // into the image, and every pixel is computed; a pixel outside the
-            // image adds zero through the selects on ok, which also discard
-            // the NaN that its zero renormalisation total gives. (A load under
-            // the pixel's own condition is compiled into a branch with its
-            // own wait: one global round trip per row.)
-            const float4 rf = *reinterpret_cast<const float4*>(&L.rowf[wave][16 * ob + 4 * g]);
-            const float rfv[4] = {rf.x, rf.y, rf.z, rf.w};
+            // The eight samples are loaded together (at the position clamped
+            // into the image on edge tiles, where a pixel outside the image
+            // adds zero through the selects on ok, which also discard the NaN
+            // that its zero renormalisation total gives).
+            float k1[4];
+            if (EDGE) {
+                const float4 rf = *reinterpret_cast<const float4*>(&L.rowf[wave][16 * ob + 4 * g]);
+                k1[0] = rf.x; k1[1] = rf.y; k1[2] = rf.z; k1[3] = rf.w;
+            } else {
+#pragma unroll
+                for (int r = 0; r < 4; r++) k1[r] = k1i;
+            }
+            const int yo = T.y0 + 16 * ob + 4 * g;
             float s1[4], s2[4];
 #pragma unroll
             for (int r = 0; r < 4; r++) {
-                const int64_t off = (int64_t)min(T.y0 + 16 * ob + 4 * g + r, H - 1) * pitch + min(T.xo, W - 1);
+                const int64_t off = (int64_t)(EDGE ? min(yo + r, H - 1) : yo + r) * pitch + xc;
                 s1[r] = ldf(p1 + off);
                 s2[r] = ldf(p2 + off);
             }
 #pragma unroll
             for (int r = 0; r < 4; r++) {
-                const bool ok = T.xo < W && T.y0 + 16 * ob + 4 * g + r < H;
-                const float iv = invh * rfv[r];
-                const float mu1 = o[0][r] * iv, mu2 = o[1][r] * iv;
-                const float E2 = o[2][r] * (iv * kEOut), Q = o[3][r] * (iv * kQOut);
+                const float mu1 = o[0][r] * k1[r], mu2 = o[1][r] * k1[r];
+                const float E2 = o[2][r] * (k1[r] * kE), Q = o[3][r] * (k1[r] * kQ);
+                // SSIM: d = (Vd + md^2 num) / (Vd + num) with md = mu1 - mu2,
+                // Vd = var(x1 - x2), num = 2 cov + C2 (src/ssimu2/score.hpp:141-148)
                 const float md = mu1 - mu2;
                 const float md2 = md * md;
-                const float vd = E2 - md2;  // var(x1 - x2)
-                const float num_s = fmaf(2.f, Q - mu1 * mu2, 0.0009f);
-                float d = fmaf(md2, num_s, vd) * __builtin_amdgcn_rcpf(vd + num_s);
-                d = (ok && d > 0.f) ? d : 0.f;
-                const float a = s1[r] - cc;
-                const float b = s2[r] - cc;
-                const float fa = fabsf(a - mu1) + 1.f, fb = fabsf(b - mu2) + 1.f;
-                const float ev = (fb - fa) * __builtin_amdgcn_rcpf(fa);  // src/ssimu2/score.hpp:150-154
-                const float art = (ok && ev > 0.f) ? ev : 0.f, det = (ok && ev < 0.f) ? -ev : 0.f;
+                const float vd = E2 - md2;
+                const float num_s = fmaf(2.f, fmaf(-mu1, mu2, Q), 0.0009f);
+                const float n_s = fmaf(md2, num_s, vd);
+                const float den = vd + num_s;
+                // edge ratio (src/ssimu2/score.hpp:150-154)
+                const float fa = fabsf((s1[r] - cc) - mu1) + 1.f;
+                const float fb = fabsf((s2[r] - cc) - mu2) + 1.f;
+                // one reciprocal for both quotients: d = n / den and
+                // (fb - fa) / fa = (fb - fa) den / (den fa); fb - fa is
+                // exactly 0 for identical inputs, so both maps stay 0
+                const float rr = __builtin_amdgcn_rcpf(den * fa);
+                float d = (n_s * fa) * rr;
+                const float ev = ((fb - fa) * den) * rr;
+                float art = fmaxf(ev, 0.f), det = fmaxf(-ev, 0.f);
+                d = fmaxf(d, 0.f);
+                if (EDGE) {
+                    const bool ok = T.xo < W && yo + r < H;
+                    d = ok ? d : 0.f;
+                    art = ok ? art : 0.f;
+                    det = ok ? det : 0.f;
+                }
                 const float d2 = d * d, a2 = art * art, t2 = det * det;
                 acc[0] += d;
-                acc[1] += d2 * d2;
+                acc[1] = fmaf(d2, d2, acc[1]);
                 acc[2] += art;
-                acc[3] += a2 * a2;
+                acc[3] = fmaf(a2, a2, acc[3]);
                 acc[4] += det;
-                acc[5] += t2 * t2;
+                acc[5] = fmaf(t2, t2, acc[5]);
             }
         }
         // wave sums of this channel in FP64, fixed order
@@ -404,6 +469,8 @@ __global__ __launch_bounds__(256, SP::kParts == 2 ? 4 : 2) void map_mx_kernel(co
         }
     }
     __syncthreads();
+
+    const typename SP::Op G = tap_operand<SP>(g, j);
 
     int seg_lo = 0, t_begin = 0;
 #pragma unroll
@@ -455,7 +522,10 @@ __global__ __launch_bounds__(256, SP::kParts == 2 ? 4 : 2) void map_mx_kernel(co
         T.cmask = gin ? (int)((1u << min(T.W - xi, 8)) - 1u) : 0;
         const ST* base1 = reinterpret_cast<const ST*>(arena + P.arena_off + P.scale_off[s]);
         const ST* base2 = reinterpret_cast<const ST*>(arena + P.arena_off + P.img_bytes + P.scale_off[s]);
-        mx_tile<SP>(L, T, base1, base2, L.cum, lane);
+        // the 80 x 80 input region in the image: no masks, constant renormalisation
+        const bool interior = T.x0 >= 8 && T.x0 + kMxTile + 8 <= T.W && T.y0 >= 8 && T.y0 + kMxTile + 8 <= T.H;
+        if (interior) mx_tile<SP, ST, false>(L, T, G, base1, base2, L.cum, lane);
+        else mx_tile<SP, ST, true>(L, T, G, base1, base2, L.cum, lane);
 
         __syncthreads();
         if (tid < kFeat) {

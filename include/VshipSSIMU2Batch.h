@@ -230,6 +230,28 @@ EXPORTPREPROCESS Vship_Exception Vship_MemcpyHtoD(void* dst, const void* src, ui
 EXPORTPREPROCESS Vship_Exception Vship_MemcpyDtoH(void* dst, const void* src, uint64_t size);
 EXPORTPREPROCESS Vship_Exception Vship_MemcpyDtoD(void* dst, const void* src, uint64_t size);
 
+/* ---- allocation-free entry point -------------------------------------------
+ * No counterpart in the reference. For callers that cannot allocate device
+ * memory: code under HIP graph capture, tight serving loops, harnesses that
+ * forbid hipMalloc. Scores one device-resident pair with the kernels and
+ * launch shape of Vship_SSIMU2BatchCompute at its defaults (matrix-core map
+ * kernel, one workgroup per tile, FP32 planes), so the results are that entry
+ * point's. It allocates nothing: it works in a fixed scratch area of
+ * VSHIP_NOALLOC_SCRATCH_BYTES that each GPU the library runs on holds until
+ * the process ends.
+ *
+ * Asynchronous: enqueues on `stream` (a hipStream_t; NULL for the null
+ * stream) and returns; *score (and features[108], reference feature order,
+ * if not NULL) are device memory written when the stream reaches that work.
+ * Frames whose XYB planes exceed the scratch (above about 1920 x 1080)
+ * return Vship_OutOfVRAM; bad pointers or sizes return Vship_BadPointer,
+ * both before anything is enqueued. Every call on one GPU shares that GPU's
+ * scratch: calls that overlap on one GPU race, so order them (one stream,
+ * or events). */
+#define VSHIP_NOALLOC_SCRATCH_BYTES (2048ull + 4096ull * 18ull * 8ull + (64ull << 20))
+EXPORTPREPROCESS Vship_Exception Vship_SSIMU2ComputeNoAlloc(const Vship_SSIMU2Pair* pair, double* score,
+                                                            double* features, void* stream);
+
 #ifdef __cplusplus
 } /* extern "C" */
 #endif

@@ -191,6 +191,9 @@ def lib():
     L.Vship_MemcpyDtoH.argtypes = [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_uint64]
     L.Vship_MemcpyDtoD.restype = E
     L.Vship_MemcpyDtoD.argtypes = [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_uint64]
+    L.Vship_SSIMU2ComputeNoAlloc.restype = E
+    L.Vship_SSIMU2ComputeNoAlloc.argtypes = [ctypes.POINTER(Pair), ctypes.c_void_p, ctypes.c_void_p,
+                                             ctypes.c_void_p]
     L.Vship_SSIMU2ServerInit.restype = E
     L.Vship_SSIMU2ServerInit.argtypes = [ctypes.POINTER(Handler), ctypes.POINTER(ServerConfig)]
     L.Vship_SSIMU2ServerFree.restype = E

@@ -69,7 +69,10 @@ void require_device_rows(const void* ptr, int64_t stride, int rows, int64_t row_
     if (hipMemGetAddressRange(&base, &size, (hipDeviceptr_t)ptr) != hipSuccess) VSHIP_THROW(E_BadPointer);
     const uint8_t* b = reinterpret_cast<const uint8_t*>(base);
     const uint8_t* p = reinterpret_cast<const uint8_t*>(ptr);
-    const uint8_t* end = p + (int64_t)(rows - 1) * stride + row_bytes;
+    const int64_t extent = (int64_t)(rows - 1) * stride + row_bytes;
+    // the kernels address a plane with 32-bit byte offsets
+    if (extent >= (int64_t(1) << 32)) VSHIP_THROW(E_BadPointer);
+    const uint8_t* end = p + extent;
     if (p < b || end > b + size) VSHIP_THROW(E_BadPointer);
 }
 
@@ -250,6 +253,7 @@ PlanInfo plan_batch(const PairInput* in, int n, int max_pairs, bool fp16, int ma
         d.csh = fmt_csh(p.fmt);
         d.cloc = fmt_yuv(p.fmt) ? p.chroma_loc : CLOC_LEFT;
         matrix_coefs(p.matrix, d.kr, d.kb);
+        d.ikg = 1.0f / (1.0f - d.kr - d.kb);
         d.w = p.w;
         d.h = p.h;
         d.arena_off = arena_off;

@@ -54,8 +54,9 @@ struct VshipErr : std::exception {
 void matrix_coefs(int matrix, float& kr, float& kb);
 
 // Throws E_BadPointer unless `rows` rows of `row_bytes` bytes at a pitch of
-// `stride` bytes from `ptr` lie inside one device allocation, so that a host
-// pointer or a short buffer is an error on the host and never a GPU fault.
+// `stride` bytes from `ptr` lie inside one device allocation and span less
+// than 4 GiB (the kernels address planes with 32-bit byte offsets), so that a
+// host pointer or a short buffer is an error on the host and never a GPU fault.
 void require_device_rows(const void* ptr, int64_t stride, int rows, int64_t row_bytes);
 
 struct PairInput {

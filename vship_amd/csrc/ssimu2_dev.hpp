@@ -78,6 +78,7 @@ struct PairDev {
     int32_t csw, csh;           // log2 chroma subsampling (YUV)
     int32_t cloc;               // ChromaLoc (YUV)
     float kr, kb;               // YUV matrix coefficients
+    float ikg;                  // 1 / (1 - kr - kb)
     int32_t w, h;               // scale-0 size
     int64_t arena_off;          // byte offset of the pair's region
     int64_t img_bytes;          // bytes of one image region

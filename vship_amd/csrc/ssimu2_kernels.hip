@@ -40,18 +40,14 @@ __device__ __forceinline__ float linearize(float v) {
     return copysignf(__builtin_amdgcn_exp2f(2.4f * __builtin_amdgcn_logf(a)), v);
 }
 
-// cube root of x >= 0 without a division: r = 2^(-log2(x)/3) ~ x^(-1/3) on
-// the hardware units, y = x r^2, then one Newton step on y^3 = x with the
-// derivative's reciprocal 1/(3 y^2) taken as r^2/3 (within 0.75 ulp of the
-// exact cube root where the hardware units are within 2 ulp). x <= 0 (and
-// NaN) is clamped to 1e-30, whose cube root 1e-10 is below half an ulp of
-// the opsin bias it is added to: the same XYB as cbrt(0) = 0, with no branch.
+// cube root of x >= 0 as 2^(log2(x) / 3) on the hardware units (within
+// about 3 ulp: the units are within about 1 ulp each, and log2(x) / 3 carries
+// their error into the exponent). x <= 0 (and NaN) is clamped to 1e-30, whose
+// cube root 1e-10 is below half an ulp of the opsin bias it is added to: the
+// same XYB as cbrt(0) = 0, with no branch.
 __device__ __forceinline__ float cbrt_pos(float x) {
     x = fmaxf(x, 1e-30f);
-    const float r = __builtin_amdgcn_exp2f(__builtin_amdgcn_logf(x) * (-1.0f / 3.0f));
-    const float rr = r * r;
-    const float y = x * rr;
-    return fmaf(-fmaf(y * y, y, -x), rr * (1.0f / 3.0f), y);
+    return __builtin_amdgcn_exp2f(__builtin_amdgcn_logf(x) * (1.0f / 3.0f));
 }
 
 // Linear RGB -> positive XYB, src/ssimu2/makeXYB.hpp:4-50.
@@ -83,15 +79,53 @@ struct PrepLds {
     float l4[3][4][5];
 };
 
+// non-temporal (streaming) stores of one XYB value
+__device__ __forceinline__ void stf_nt(float* p, float v) { __builtin_nontemporal_store(v, p); }
+__device__ __forceinline__ void stf_nt(__half* p, float v) {
+    __builtin_nontemporal_store(__half_as_ushort(__float2half_rn(v)), reinterpret_cast<unsigned short*>(p));
+}
+
+// X, Y, B of pixel (x, y) of scale s: the plane bases are the same for every
+// lane (scalar registers) and the pixel's offset is 32-bit, so each store is
+// one instruction on a scalar base with a vector offset.
 template <typename T>
 __device__ __forceinline__ void store_planes(uint8_t* arena, const PairDev& P, int img, int s, int x, int y, float X,
                                              float Y, float B) {
-    T* base = reinterpret_cast<T*>(arena + P.arena_off + img * P.img_bytes + P.scale_off[s]) +
-              (int64_t)y * P.pitch[s] + x;
-    const int64_t pe = P.plane_elems[s];
-    stf(base, X);
-    stf(base + pe, Y);
-    stf(base + 2 * pe, B);
+    uint8_t* b0 = arena + P.arena_off + img * P.img_bytes + P.scale_off[s];
+    const int64_t pb = P.plane_elems[s] * (int64_t)sizeof(T);
+    uint8_t* b1 = b0 + pb;
+    uint8_t* b2 = b1 + pb;
+    // byte offset within the plane (planes are far below 4 GiB)
+    const uint32_t o = ((uint32_t)y * (uint32_t)P.pitch[s] + (uint32_t)x) * (uint32_t)sizeof(T);
+    stf_nt(reinterpret_cast<T*>(b0 + o), X);
+    stf_nt(reinterpret_cast<T*>(b1 + o), Y);
+    stf_nt(reinterpret_cast<T*>(b2 + o), B);
+}
+
+// Four consecutive scale-0 pixels x .. x+3 (x a multiple of 4) of plane c of
+// row y in one 16-byte (FP16 planes: 8-byte) store. The planes are read back
+// only by the map kernels, after this kernel has written far more than the
+// caches hold, so the stores are non-temporal (streaming): measured at
+// 1920x1080, 240 pairs, 5.36 -> 4.40 ms for the whole kernel.
+template <typename T>
+__device__ __forceinline__ void store_row4(uint8_t* arena, const PairDev& P, int img, int c, int x, int y, float4 v);
+template <>
+__device__ __forceinline__ void store_row4<float>(uint8_t* arena, const PairDev& P, int img, int c, int x, int y,
+                                                  float4 v) {
+    uint8_t* b = arena + P.arena_off + img * P.img_bytes + P.scale_off[0] + c * P.plane_elems[0] * 4;
+    const uint32_t o = ((uint32_t)y * (uint32_t)P.pitch[0] + (uint32_t)x) * 4u;
+    typedef float f4 __attribute__((ext_vector_type(4)));
+    __builtin_nontemporal_store((f4){v.x, v.y, v.z, v.w}, reinterpret_cast<f4*>(b + o));
+}
+template <>
+__device__ __forceinline__ void store_row4<__half>(uint8_t* arena, const PairDev& P, int img, int c, int x, int y,
+                                                   float4 v) {
+    uint8_t* b = arena + P.arena_off + img * P.img_bytes + P.scale_off[0] + c * P.plane_elems[0] * 2;
+    const uint32_t o = ((uint32_t)y * (uint32_t)P.pitch[0] + (uint32_t)x) * 2u;
+    const __half2 a = __floats2half2_rn(v.x, v.y), d = __floats2half2_rn(v.z, v.w);
+    typedef uint32_t u2 __attribute__((ext_vector_type(2)));
+    __builtin_nontemporal_store((u2){*reinterpret_cast<const uint32_t*>(&a), *reinterpret_cast<const uint32_t*>(&d)},
+                                reinterpret_cast<u2*>(b + o));
 }
 
 // XYB of one pixel of scale s into the arena: planes of type ST for the two
@@ -133,15 +167,18 @@ __device__ __forceinline__ void pyr_level(const float* par, float* chi, uint8_t*
     }
 }
 
-// Reads sample x of row y of an input plane of RGB format FMT as float.
+// Reads sample x of row y of an input plane of RGB format FMT as float (a
+// 32-bit byte offset from the plane base: the host refuses planes of 4 GiB
+// or more).
 template <int FMT>
 __device__ __forceinline__ float rgb_sample(const uint8_t* plane, int64_t stride, int x, int y) {
-    const uint8_t* row = plane + (int64_t)y * stride;
-    if constexpr (FMT == IN_RGB_U8) return (float)as_global(row)[x] * (1.0f / 255.0f);
-    else if constexpr (FMT == IN_RGB_U16) return (float)as_global(reinterpret_cast<const uint16_t*>(row))[x] * (1.0f / 65535.0f);
+    constexpr uint32_t bps = FMT == IN_RGB_U8 ? 1 : (FMT == IN_RGB_F32 ? 4 : 2);
+    const uint8_t* p = plane + ((uint32_t)y * (uint32_t)stride + bps * (uint32_t)x);
+    if constexpr (FMT == IN_RGB_U8) return (float)*as_global(p) * (1.0f / 255.0f);
+    else if constexpr (FMT == IN_RGB_U16) return (float)*as_global(reinterpret_cast<const uint16_t*>(p)) * (1.0f / 65535.0f);
     else if constexpr (FMT == IN_RGB_F16)
-        return __half2float(__ushort_as_half(as_global(reinterpret_cast<const unsigned short*>(row))[x]));
-    else return as_global(reinterpret_cast<const float*>(row))[x];
+        return __half2float(__ushort_as_half(*as_global(reinterpret_cast<const unsigned short*>(p))));
+    else return *as_global(reinterpret_cast<const float*>(p));
 }
 
 // Scales 0 and 1 of one 64x64 tile of an image of input format FMT: a thread
@@ -169,17 +206,21 @@ __device__ __forceinline__ void prep_quads(PrepLds& L, const PairDev& P, uint8_t
         __syncthreads();
     }
 
-    // limited-range YUV -> R'G'B' constants (uniform)
-    const float sc = yuv ? (float)(1 << (P.bits - 8)) : 1.0f;
-    const float ymul = 1.0f / (219.0f * sc), yadd = -16.0f / 219.0f;
+    // limited-range YUV -> R'G'B' constants (uniform): 1 / (219 s) and the
+    // others with s = 2^(bits - 8) as exact power-of-two scalings of the
+    // correctly rounded 8-bit constants
+    const int e8 = yuv ? 8 - P.bits : 0;
+    const float ymul = ldexpf(1.0f / 219.0f, e8), yadd = -16.0f / 219.0f;
     // the staged 4:2:0 path sums its vertical taps with integer weights of
     // total 128, so its chroma carries a factor 128 that cmul takes out;
     // the generic path's weights are normalised (cmul1)
-    const float cmul = 1.0f / (128.0f * 224.0f * sc), cmul1 = 1.0f / (224.0f * sc), cadd = -128.0f / 224.0f;
-    const float kr = P.kr, kb = P.kb, ikg = 1.0f / (1.0f - P.kr - P.kb);
+    const float cmul = ldexpf(1.0f / (128.0f * 224.0f), e8), cmul1 = ldexpf(1.0f / 224.0f, e8),
+                cadd = -128.0f / 224.0f;
+    const float kr = P.kr, kb = P.kb, ikg = P.ikg;
     const float rcr = 2.0f * (1.0f - kr), bcb = 2.0f * (1.0f - kb);
 
     const int nv1x = min(32, P.sw[1] - tx * 32), nv1y = min(32, P.sh[1] - ty * 32);
+    const bool full = x0 + kPrepTile <= W && y0 + kPrepTile <= H;
 #pragma unroll 1
     for (int i = 0; i < 4; i++) {
         const int q = tid + 256 * i;
@@ -218,7 +259,8 @@ __device__ __forceinline__ void prep_quads(PrepLds& L, const PairDev& P, uint8_t
                 }
             } else {
 #pragma unroll
-                for (int pl = 0; pl < 2; pl++) chroma420_quad(L.ch[pl], qx, qy, hx, hy, cmul, cadd, cbcr[pl]);
+                for (int pl = 0; pl < 2; pl++)
+                    chroma420_quad(L.ch[pl], qx, qy, hx, hy, cmul, cadd, cbcr[pl]);
             }
 #pragma unroll
             for (int k = 0; k < 4; k++) {
@@ -235,10 +277,29 @@ __device__ __forceinline__ void prep_quads(PrepLds& L, const PairDev& P, uint8_t
         for (int k = 0; k < 4; k++)
 #pragma unroll
             for (int p = 0; p < 3; p++) rgb[k][p] = linearize(rgb[k][p]);
-        store_xyb<ST>(arena, P, img, 0, xa, ya, rgb[0][0], rgb[0][1], rgb[0][2]);
-        if (hy) store_xyb<ST>(arena, P, img, 0, xa, yb, rgb[1][0], rgb[1][1], rgb[1][2]);
-        if (hx) store_xyb<ST>(arena, P, img, 0, xb, ya, rgb[2][0], rgb[2][1], rgb[2][2]);
-        if (hx && hy) store_xyb<ST>(arena, P, img, 0, xb, yb, rgb[3][0], rgb[3][1], rgb[3][2]);
+        if (full) {
+            // every lane of the wave is here (no quad of a full tile is out of
+            // the image); lanes 2k and 2k+1 hold horizontally adjacent quads:
+            // the even lane writes both quads' first row, the odd lane their
+            // second row, four consecutive pixels per plane in one store
+            float v[3][4];
+#pragma unroll
+            for (int k = 0; k < 4; k++) to_xyb(rgb[k][0], rgb[k][1], rgb[k][2], v[0][k], v[1][k], v[2][k]);
+            const bool odd = qx & 1;
+            const int sx = odd ? xa - 2 : xa, sy = odd ? yb : ya;
+#pragma unroll
+            for (int c = 0; c < 3; c++) {
+                const float s0 = odd ? v[c][0] : v[c][1], s1 = odd ? v[c][2] : v[c][3];
+                const float r0 = __shfl_xor(s0, 1, 64), r1 = __shfl_xor(s1, 1, 64);
+                const float4 o = odd ? make_float4(r0, r1, v[c][1], v[c][3]) : make_float4(v[c][0], v[c][2], r0, r1);
+                store_row4<ST>(arena, P, img, c, sx, sy, o);
+            }
+        } else {
+            store_xyb<ST>(arena, P, img, 0, xa, ya, rgb[0][0], rgb[0][1], rgb[0][2]);
+            if (hy) store_xyb<ST>(arena, P, img, 0, xa, yb, rgb[1][0], rgb[1][1], rgb[1][2]);
+            if (hx) store_xyb<ST>(arena, P, img, 0, xb, ya, rgb[2][0], rgb[2][1], rgb[2][2]);
+            if (hx && hy) store_xyb<ST>(arena, P, img, 0, xb, yb, rgb[3][0], rgb[3][1], rgb[3][2]);
+        }
         // scale-1 child (src/ssimu2/downsample.hpp:3-19 summation order)
         if (qx < nv1x && qy < nv1y) {
             float v[3];

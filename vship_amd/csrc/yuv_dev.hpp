@@ -11,11 +11,16 @@
 namespace vship {
 
 // One YUV sample; 16-bit-container samples above the declared depth's
-// largest code are clamped to it.
+// largest code are clamped to it. The byte offset is 32-bit (the host
+// refuses planes of 4 GiB or more), so a plane base in scalar registers and a
+// vector offset address it.
 __device__ __forceinline__ float sample_u(const uint8_t* plane, int64_t stride, int x, int y, bool wide,
                                           int maxcode) {
-    if (wide) return (float)min((int)as_global(reinterpret_cast<const uint16_t*>(plane + (int64_t)y * stride))[x], maxcode);
-    return (float)as_global(plane)[(int64_t)y * stride + x];
+    if (wide) {
+        const uint32_t o = (uint32_t)y * (uint32_t)stride + 2u * (uint32_t)x;
+        return (float)min((int)*as_global(reinterpret_cast<const uint16_t*>(plane + o)), maxcode);
+    }
+    return (float)*as_global(plane + ((uint32_t)y * (uint32_t)stride + (uint32_t)x));
 }
 
 // Catmull-Rom weights of samples c-1 .. c+2 at fraction t of [c, c+1]: the

@@ -287,7 +287,7 @@ PlanInfo plan_batch(const PairInput* in, int n, int max_pairs, bool fp16, int ma
                 const int sw = pd[i].sw[s], sh = pd[i].sh[s];
                 const bool float_in = pd[i].fmt == IN_RGB_F16 || pd[i].fmt == IN_RGB_F32;
                 if ((mfma && cls == 0 && float_in) != exact) continue;
-                if (((int64_t)sw * sh < kSmallSegment ? 1 : 0) != cls) continue;
+                if (((int64_t)sw * sh < (mfma ? kSmallSegmentMx : kSmallSegment) ? 1 : 0) != cls) continue;
                 if ((fp16 && s < kHalfScales) != half) continue;
                 const int th = cls ? kSmallTH : map_th;
                 const int tw = map_tile_w(th), tht = map_tile_h(th);

@@ -1,7 +1,8 @@
 // Map kernel on the gfx950 matrix cores, for the FP32-moment segments.
 //
 // Same result as map_kernel<TH, ST, float> (renormalised 17-tap Gaussian of
-// the four moment planes a, b, (x1 - x2)^2, ab, then the SSIM, artifact and
+// the four moment planes a, b, (a - b)^2, ab of the samples centred per image
+// on the tile-centre values (a = x1 - c1, b = x2 - c2), then the SSIM, artifact and
 // detail-loss maps and their 1-/4-norm sums; src/ssimu2/gaussianblur.hpp:24-92,
 // src/ssimu2/score.hpp:95-196), with both blur passes done as matrix products
 // on the 16x16x32 MFMA instructions:
@@ -19,16 +20,16 @@
 //     last 8); the six products above 2^-24 are accumulated (hh + hm + mh +
 //     hl + mm + lh). FP32 accuracy, FP32 range.
 //   SplitF16x2 (8- to 16-bit RGB and YUV input): operands are scaled by a
-//     power of two (a, b: 2^11, (x1 - x2)^2: 2^6, ab: 2^6, taps: 2^14) and
-//     split into two fp16 parts h + l (22 significant bits); all four
-//     products hh + hl + lh + ll are accumulated, so each product of the two
-//     representations is exact. The taps are the values the fp16 pair
+//     power of two (a, b: 2^11, (a - b)^2: 2^4, ab: 2^6, taps: 2^14) and
+//     split into two fp16 parts h + l (22 significant bits); the products
+//     hh + hl + lh are accumulated, each exact (the lo x lo product, below
+//     the split's own error, is dropped). The taps are the values the fp16 pair
 //     represents exactly (within 2^-22 of the FP32 taps), and the
 //     renormalisation sums use the same values. The scales keep every value
 //     inside the fp16 range for any input of these formats: their R'G'B' lies
 //     in [-0.97, 1.98] (limited-range YUV with samples clamped to the
 //     declared depth), so XYB lies in [-11.7, 12.5] x [-0.15, 1.58] x
-//     [-1.18, 2.28], |a|, |b|, |x1 - x2| <= 24.2 and |ab| <= 586. A CPU
+//     [-1.18, 2.28], |a|, |b| <= 24.2, |a - b| <= 48.4 and |ab| <= 586. A CPU
 //     emulation of this arithmetic on real frames puts the score within
 //     1e-5 of the FP64 maps (typically 1e-6), at about half the splitting
 //     work of the exact split and two thirds of its matrix products.
@@ -129,7 +130,7 @@ __device__ __forceinline__ f32x4 mfma_f16(u32x4 a, u32x4 b, f32x4 c) {
 struct SplitF16x2 {
     static constexpr int kParts = 2;
     static constexpr float kScaleAB = 2048.f;        // a, b: 2^11
-    static constexpr float kScaleDD = 8.f;           // (x1 - x2)^2 as (8 (x1 - x2))^2: 2^6
+    static constexpr float kScaleDD = 4.f;           // (a - b)^2 as (4 (a - b))^2: 2^4
     static constexpr float kScaleQ = 1.f / 65536.f;  // ab from the scaled a, b: 2^22 / 2^16 = 2^6
     static constexpr float kTapScale = 16384.f;      // taps: 2^14
     struct Op {
@@ -275,7 +276,7 @@ __device__ __forceinline__ void mx_tile(MxLds<SP::kParts>& L, const TileGeo& T, 
     constexpr int NP = SP::kParts;
     const int W = T.W, H = T.H, g = T.g, j = T.j, wave = T.wave, swz = T.swz;
     const int64_t pitch = T.pitch;
-    // moments relative to a and b: (x1 - x2)^2 and ab carry kScaleDD^2 and
+    // moments relative to a and b: (a - b)^2 and ab carry kScaleDD^2 and
     // kScaleAB^2 kScaleQ where a and b carry kScaleAB
     constexpr float kE = SP::kScaleAB / (SP::kScaleDD * SP::kScaleDD);
     constexpr float kQ = 1.f / (SP::kScaleAB * SP::kScaleQ);
@@ -298,8 +299,14 @@ __device__ __forceinline__ void mx_tile(MxLds<SP::kParts>& L, const TileGeo& T, 
         const ST* p1 = base1 + c * T.pe;
         const ST* p2 = base2 + c * T.pe;
         const int cxp = min(T.x0 + kMxTile / 2, W - 1), cyp = min(T.y0 + kMxTile / 2, H - 1);
+        // each image centred on its own value at the tile centre: a = x1 - cc,
+        // b = x2 - cc2; the difference moment is then (a - b)^2, which stays
+        // near 0 where the images differ by a locally constant offset, so
+        // Vd = E[(a - b)^2] - (mu_a - mu_b)^2 does not cancel two large terms
         const float cc = ldf(p1 + (int64_t)cyp * pitch + cxp);
-        const float ccs = cc * SP::kScaleAB;
+        const float cc2 = ldf(p2 + (int64_t)cyp * pitch + cxp);
+        const float ccs = cc * SP::kScaleAB, ccs2 = cc2 * SP::kScaleAB;
+        const float dc = cc - cc2;
         float acc[6] = {0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
 
 #pragma unroll 1
@@ -315,7 +322,7 @@ __device__ __forceinline__ void mx_tile(MxLds<SP::kParts>& L, const TileGeo& T, 
                 if (EDGE) {
                     // one vector load per image for every lane: the row is
                     // clamped into the image and an element outside it takes
-                    // the value cc in both images, so that all four of its
+                    // the centre values cc and cc2, so that all four of its
                     // moments are exactly zero (ccs = cc kScaleAB is exact:
                     // the scale is a power of two)
                     const bool row_ok = yr >= 0 && yr < H;
@@ -323,7 +330,7 @@ __device__ __forceinline__ void mx_tile(MxLds<SP::kParts>& L, const TileGeo& T, 
                     for (int e = 0; e < 8; e++) {
                         const bool in = row_ok && ((T.cmask >> e) & 1);
                         u1[e] = in ? u1[e] : cc;
-                        u2[e] = in ? u2[e] : cc;
+                        u2[e] = in ? u2[e] : cc2;
                     }
                 }
                 float v[4][8];
@@ -331,8 +338,8 @@ __device__ __forceinline__ void mx_tile(MxLds<SP::kParts>& L, const TileGeo& T, 
                 for (int e = 0; e < 8; e++) {
                     const float x1 = u1[e], x2 = u2[e];
                     const float a = fmaf(x1, SP::kScaleAB, -ccs);
-                    const float b = fmaf(x2, SP::kScaleAB, -ccs);
-                    const float dd = (x1 - x2) * SP::kScaleDD;
+                    const float b = fmaf(x2, SP::kScaleAB, -ccs2);
+                    const float dd = (a - b) * (SP::kScaleDD / SP::kScaleAB);
                     v[0][e] = a;
                     v[1][e] = b;
                     v[2][e] = dd * dd;
@@ -399,17 +406,20 @@ __device__ __forceinline__ void mx_tile(MxLds<SP::kParts>& L, const TileGeo& T, 
             for (int r = 0; r < 4; r++) {
                 const float mu1 = o[0][r] * k1[r], mu2 = o[1][r] * k1[r];
                 const float E2 = o[2][r] * (k1[r] * kE), Q = o[3][r] * (k1[r] * kQ);
-                // SSIM: d = (Vd + md^2 num) / (Vd + num) with md = mu1 - mu2,
-                // Vd = var(x1 - x2), num = 2 cov + C2 (src/ssimu2/score.hpp:141-148)
-                const float md = mu1 - mu2;
+                // SSIM: d = (Vd + md^2 num) / (Vd + num) with md the difference
+                // of the two means, Vd = var(x1 - x2), num = 2 cov + C2
+                // (src/ssimu2/score.hpp:141-148); mu1, mu2 are the centred
+                // means, so md = mu1 - mu2 + (cc - cc2)
+                const float mdc = mu1 - mu2;
+                const float vd = fmaf(-mdc, mdc, E2);
+                const float md = mdc + dc;
                 const float md2 = md * md;
-                const float vd = E2 - md2;
                 const float num_s = fmaf(2.f, fmaf(-mu1, mu2, Q), 0.0009f);
                 const float n_s = fmaf(md2, num_s, vd);
                 const float den = vd + num_s;
                 // edge ratio (src/ssimu2/score.hpp:150-154)
                 const float fa = fabsf((s1[r] - cc) - mu1) + 1.f;
-                const float fb = fabsf((s2[r] - cc) - mu2) + 1.f;
+                const float fb = fabsf((s2[r] - cc2) - mu2) + 1.f;
                 // one reciprocal for both quotients: d = n / den and
                 // (fb - fa) / fa = (fb - fa) den / (den fa); fb - fa is
                 // exactly 0 for identical inputs, so both maps stay 0

@@ -112,6 +112,10 @@ __host__ __device__ constexpr int map_group(int cls, bool half, bool exact = fal
 }
 
 constexpr int64_t kSmallSegment = 32768;
+// The matrix-core map kernel centres each image on its own tile-centre value
+// and keeps Var(x1 - x2) free of large cancelling terms, so its FP32 class
+// reaches down to segments of kSmallSegmentMx pixels.
+constexpr int64_t kSmallSegmentMx = 4096;
 constexpr int kSmallTH = 48;
 // Tile height value that selects the matrix-core map kernel for the FP32
 // class; its tiles are kMxTile x kMxTile.

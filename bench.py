@@ -141,8 +141,7 @@ def per_frame_rate(descs, n, w, h, fp16):
 TRAFFIC_FILE = os.path.join(ROOT, "profiles", "hbm_traffic.json")
 VALU_FILE = os.path.join(ROOT, "profiles", "valu_issue.json")
 # MI355X vector ALUs: 256 CUs x 4 SIMDs x 16 lanes at the 2.4 GHz peak engine
-# clock; a wave64 VALU instruction is 64 lane-ops (4 cycles of its SIMD), a
-# transcendental one 4 times that (quarter rate)
+# clock; a wave64 VALU instruction is 64 lane-ops (4 cycles of its SIMD)
 VALU_PEAK_TOPS = 256 * 4 * 16 * 2.4e9 / 1e12
 
 
@@ -164,16 +163,19 @@ def measured_traffic(config, kernel):
 
 def valu_position(config, kernel, ms):
     """Vector-ALU issue of one launch of `kernel`: its lane-ops (from the
-    committed SQ_INSTS_VALU / SQ_INSTS_VALU_TRANS_F32 / SQ_INSTS_MFMA record of
-    this workload) over the launch's live duration, against the peak issue
-    rate. None where no counter run of this kernel on this workload exists."""
+    committed SQ_INSTS_VALU / SQ_INSTS_VALU_MFMA_F16 record of this workload;
+    every VALU instruction, transcendentals included, counted once) over the
+    launch's live duration, against the peak issue rate. None where no counter
+    run of this kernel on this workload exists."""
     rec = _record(VALU_FILE, config, kernel)
     if rec is None:
         return None
-    lane_ops = (rec["valu_insts"] - rec["mfma_insts"] + 3 * rec["trans_insts"]) * 64
+    lane_ops = (rec["valu_insts"] - rec["mfma_insts"]) * 64
     achieved = lane_ops / (ms * 1e-3) / 1e12
     return {"achieved": round(achieved, 2), "peak": round(VALU_PEAK_TOPS, 2), "unit": "T VALU lane-ops/s",
-            "frac": round(achieved / VALU_PEAK_TOPS, 4), "lane_ops_per_launch": int(lane_ops)}
+            "frac": round(achieved / VALU_PEAK_TOPS, 4), "lane_ops_per_launch": int(lane_ops),
+            "transcendental_share": round(rec["trans_insts"] / max(1, rec["valu_insts"] - rec["mfma_insts"]), 4),
+            "issue_of_active_cycles": rec.get("valu_issue_of_active_cycles")}
 
 
 def map_kernel_name(st, fp16):
@@ -208,8 +210,8 @@ def roofline(st, map_ms, prep_ms, fp16, config):
                 "note": f"2 images x 3 XYB planes x {elem} read per pixel of every scale in the FP32 class; "
                         "the kernel is bound by vector-ALU issue (operand splitting and the maps), not HBM"})
     if valu:
-        out["valu_source"] = "profiles/valu_issue.json: rocprofv3 SQ_INSTS_VALU, SQ_INSTS_VALU_TRANS_F32, " \
-                             "SQ_INSTS_MFMA per launch over the live HIP-event duration"
+        out["valu_source"] = "profiles/valu_issue.json: rocprofv3 SQ_INSTS_VALU minus SQ_INSTS_VALU_MFMA_F16 per " \
+                             "launch over the live HIP-event duration"
     # the other large kernel of the step
     prep_kernel = f"prep_kernel<{'__half' if fp16 else 'float'}, false>"
     pv = valu_position(config, prep_kernel, prep_ms)

@@ -98,6 +98,13 @@ struct SplitBf16x3 {
             w[2][k] = pack2(l0, l1);
         }
     }
+    // split of the eight products x[e] y[e] (y = nullptr: of x[e])
+    static __device__ __forceinline__ void split_prod8(const float* x, const float* y, uint32_t (&w)[3][4]) {
+        float p[8];
+#pragma unroll
+        for (int k = 0; k < 8; k++) p[k] = y ? x[k] * y[k] : x[k];
+        split<8>(p, w);
+    }
     // split of x * s (row-pass results, re-split at the data scale)
     template <int N>
     static __device__ __forceinline__ void split_scaled(const float* v, float s, uint32_t (&w)[3][N / 2]) {
@@ -147,6 +154,44 @@ struct SplitF16x2 {
             w[0][k] = __builtin_bit_cast(uint32_t, h);
             w[1][k] = __builtin_bit_cast(uint32_t, l);
         }
+    }
+    // split of the eight products x[e] y[e] (y = nullptr: of x[e]) for a
+    // matrix-core operand: h = fp16(x y) by the compiler; the low parts
+    // l = fp16(x y - h), each one v_fma_mixlo/mixhi_f16 that forms x y - h
+    // exactly and rounds once, written by inline assembly (the compiler
+    // splits them into a multiply-subtract and a conversion). The block ends
+    // with s_nop 1: the compiler counts no VALU wait states for inline
+    // assembly, and a matrix-core instruction reading a register a VALU
+    // instruction wrote needs two.
+    static __device__ __forceinline__ void split_prod8(const float* x, const float* y, uint32_t (&w)[2][4]) {
+        float p[8];
+#pragma unroll
+        for (int k = 0; k < 8; k++) p[k] = y ? x[k] * y[k] : x[k];
+#pragma unroll
+        for (int k = 0; k < 4; k++)
+            w[0][k] = __builtin_bit_cast(uint32_t, __builtin_convertvector((f32x2){p[2 * k], p[2 * k + 1]}, f16x2));
+        const float one = 1.0f;
+        float q[8];
+#pragma unroll
+        for (int k = 0; k < 8; k++) q[k] = y ? y[k] : one;
+        uint32_t l0, l1, l2, l3;
+        asm("v_fma_mixlo_f16 %0, %4, %12, -%20 op_sel_hi:[0,0,1]\n\t"
+            "v_fma_mixhi_f16 %0, %5, %13, -%20 op_sel:[0,0,1] op_sel_hi:[0,0,1]\n\t"
+            "v_fma_mixlo_f16 %1, %6, %14, -%21 op_sel_hi:[0,0,1]\n\t"
+            "v_fma_mixhi_f16 %1, %7, %15, -%21 op_sel:[0,0,1] op_sel_hi:[0,0,1]\n\t"
+            "v_fma_mixlo_f16 %2, %8, %16, -%22 op_sel_hi:[0,0,1]\n\t"
+            "v_fma_mixhi_f16 %2, %9, %17, -%22 op_sel:[0,0,1] op_sel_hi:[0,0,1]\n\t"
+            "v_fma_mixlo_f16 %3, %10, %18, -%23 op_sel_hi:[0,0,1]\n\t"
+            "v_fma_mixhi_f16 %3, %11, %19, -%23 op_sel:[0,0,1] op_sel_hi:[0,0,1]\n\t"
+            "s_nop 1"
+            : "=&v"(l0), "=&v"(l1), "=&v"(l2), "=&v"(l3)
+            : "v"(x[0]), "v"(x[1]), "v"(x[2]), "v"(x[3]), "v"(x[4]), "v"(x[5]), "v"(x[6]), "v"(x[7]),
+              "v"(q[0]), "v"(q[1]), "v"(q[2]), "v"(q[3]), "v"(q[4]), "v"(q[5]), "v"(q[6]), "v"(q[7]),
+              "v"(w[0][0]), "v"(w[0][1]), "v"(w[0][2]), "v"(w[0][3]));
+        w[1][0] = l0;
+        w[1][1] = l1;
+        w[1][2] = l2;
+        w[1][3] = l3;
     }
     // split of x * s: h = fp16(x s), l = fp16(fma(x, s, -h)) (the residual
     // x s - h is exact in FP32 when s is a power of two times a float that
@@ -214,6 +259,18 @@ struct Load8<__half> {
         }
     }
 };
+
+// Byte offset of element x of row y of a plane (planes are far below 4 GiB),
+// and the element at a byte offset from a plane base: the base stays in
+// scalar registers and each load takes one 32-bit vector offset.
+template <typename ST>
+__device__ __forceinline__ uint32_t plane_off(int y, int64_t pitch, int x) {
+    return ((uint32_t)y * (uint32_t)pitch + (uint32_t)x) * (uint32_t)sizeof(ST);
+}
+template <typename ST>
+__device__ __forceinline__ const ST* at(const ST* base, uint32_t byte_off) {
+    return reinterpret_cast<const ST*>(reinterpret_cast<const uint8_t*>(base) + byte_off);
+}
 
 // Shared memory of one workgroup, for a split into NP parts.
 template <int NP>
@@ -316,9 +373,9 @@ __device__ __forceinline__ void mx_tile(MxLds<SP::kParts>& L, const TileGeo& T, 
             typename SP::Op A[4];
             {
                 float u1[8], u2[8];
-                const int64_t ro = (int64_t)(EDGE ? min(max(yr, 0), H - 1) : yr) * pitch + T.xl;
-                Load8<ST>::run(p1 + ro, u1);
-                Load8<ST>::run(p2 + ro, u2);
+                const uint32_t ro = plane_off<ST>(EDGE ? min(max(yr, 0), H - 1) : yr, pitch, T.xl);
+                Load8<ST>::run(at(p1, ro), u1);
+                Load8<ST>::run(at(p2, ro), u2);
                 if (EDGE) {
                     // one vector load per image for every lane: the row is
                     // clamped into the image and an element outside it takes
@@ -333,22 +390,21 @@ __device__ __forceinline__ void mx_tile(MxLds<SP::kParts>& L, const TileGeo& T, 
                         u2[e] = in ? u2[e] : cc2;
                     }
                 }
-                float v[4][8];
+                // the moments as products of two factors: a, b, dd dd, (a kScaleQ) b
+                float a[8], b[8], dd[8], aq[8];
 #pragma unroll
                 for (int e = 0; e < 8; e++) {
-                    const float x1 = u1[e], x2 = u2[e];
-                    const float a = fmaf(x1, SP::kScaleAB, -ccs);
-                    const float b = fmaf(x2, SP::kScaleAB, -ccs2);
-                    const float dd = (a - b) * (SP::kScaleDD / SP::kScaleAB);
-                    v[0][e] = a;
-                    v[1][e] = b;
-                    v[2][e] = dd * dd;
-                    v[3][e] = (a * SP::kScaleQ) * b;
+                    a[e] = fmaf(u1[e], SP::kScaleAB, -ccs);
+                    b[e] = fmaf(u2[e], SP::kScaleAB, -ccs2);
+                    dd[e] = (a[e] - b[e]) * (SP::kScaleDD / SP::kScaleAB);
+                    aq[e] = a[e] * SP::kScaleQ;
                 }
+                const float* fx[4] = {a, b, dd, aq};
+                const float* fy[4] = {nullptr, nullptr, dd, b};
 #pragma unroll
                 for (int m = 0; m < 4; m++) {
                     uint32_t w[NP][4];
-                    SP::template split<8>(v[m], w);
+                    SP::split_prod8(fx[m], fy[m], w);
 #pragma unroll
                     for (int p = 0; p < NP; p++) A[m].p[p] = u32x4{w[p][0], w[p][1], w[p][2], w[p][3]};
                 }
@@ -398,9 +454,9 @@ __device__ __forceinline__ void mx_tile(MxLds<SP::kParts>& L, const TileGeo& T, 
             float s1[4], s2[4];
 #pragma unroll
             for (int r = 0; r < 4; r++) {
-                const int64_t off = (int64_t)(EDGE ? min(yo + r, H - 1) : yo + r) * pitch + xc;
-                s1[r] = ldf(p1 + off);
-                s2[r] = ldf(p2 + off);
+                const uint32_t off = plane_off<ST>(EDGE ? min(yo + r, H - 1) : yo + r, pitch, xc);
+                s1[r] = ldf(at(p1, off));
+                s2[r] = ldf(at(p2, off));
             }
 #pragma unroll
             for (int r = 0; r < 4; r++) {

@@ -98,6 +98,14 @@ def test_mixed_64_tuned_and_graph(mixed64):
         for rep in range(3):
             got = g.compute(dp.descs)
             assert np.array_equal(got, eager), rep
+    # persistent grids (per-XCD dynamic tile queue), eager and captured: the
+    # same scores bit for bit
+    for pg in (8, 512, 2048):
+        with api.SSIMU2Batch(64, 3840, 2160, map_tile_height=th, persistent_grid=pg) as b:
+            assert np.array_equal(b.compute(dp.descs), eager), pg
+        with api.SSIMU2Batch(64, 3840, 2160, map_tile_height=th, persistent_grid=pg, use_graph=True) as g:
+            for rep in range(2):
+                assert np.array_equal(g.compute(dp.descs), eager), (pg, rep)
     # each pair alone under the same shape
     one = (api.Pair * 1)()
     with api.SSIMU2Batch(1, 3840, 2160, map_tile_height=th, persistent_grid=grid) as b1:

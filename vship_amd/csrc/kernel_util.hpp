@@ -29,6 +29,27 @@ __device__ __forceinline__ int xcd_block(int b, int nwg) {
     return x < r ? x * (q + 1) + j : r * (q + 1) + (x - r) * q + j;
 }
 
+// Next tile of a persistent grid's dynamic queue, or -1 when none is left.
+// The launch's tiles [0, n) are split into 8 contiguous ranges, one per XCD
+// (workgroup b runs on XCD b % 8, each XCD with its own L2), so that
+// neighbouring tiles, whose halos overlap, are taken on one XCD. A workgroup
+// takes the next tile of its own XCD's range from that range's counter
+// (q[0..7], one vector atomic), then tiles of the other ranges once its own
+// is used up. The counters are 0 at the start of every launch (they live in
+// the batch's descriptor block, which every batch writes afresh). One lane
+// calls it; it only hands out indices, so relaxed ordering suffices.
+__device__ __forceinline__ int queue_next(int32_t* q, int n) {
+    const int x = blockIdx.x & 7;
+    for (int k = 0; k < 8; k++) {
+        const int r = (x + k) & 7;
+        const int lo = (int)(((int64_t)r * n) >> 3), hi = (int)(((int64_t)(r + 1) * n) >> 3);
+        if (lo >= hi) continue;
+        const int i = __hip_atomic_fetch_add(q + r, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        if (i < hi - lo) return lo + i;
+    }
+    return -1;
+}
+
 // Sum of the in-bounds normalised taps at position p of an axis of length n,
 // from the cumulative tap table (renormalisation of src/ssimu2/gaussianblur.hpp:58-60).
 template <typename AT>

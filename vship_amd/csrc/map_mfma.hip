@@ -546,13 +546,32 @@ __global__ __launch_bounds__(256, SP::kParts == 2 ? 4 : 2) void map_mx_kernel(co
     }
     const int seg_hi = seg_lo + hdr->nseg[grp] - 1;
     const int t_end = t_begin + hdr->tiles[grp];
+    // A grid of one workgroup per tile (or more) numbers its workgroups with
+    // xcd_block, so that the workgroups of one XCD hold one contiguous range
+    // of tiles. A smaller, persistent grid takes its tiles from the per-XCD
+    // dynamic queue.
     const int nwg = gridDim.x;
-    const int vb = xcd_block(blockIdx.x, nwg);
     const int ntl = t_end - t_begin;
-    const int t_lo = t_begin + (int)(((int64_t)vb * ntl) / nwg);
+    const bool persistent = nwg < ntl;
+    int32_t* queue = const_cast<int32_t*>(hdr->queue[grp]);
+    __shared__ int s_tile;
+    // (with more workgroups than tiles, tile floor(v n / nwg) of virtual
+    // workgroup v, so that every XCD still gets tiles)
+    const int vb = xcd_block(blockIdx.x, nwg);
+    int next = t_begin + (int)(((int64_t)vb * ntl) / nwg);
     const int t_hi = t_begin + (int)(((int64_t)(vb + 1) * ntl) / nwg);
-
-    for (int t = t_lo; t < t_hi; t++) {
+    for (;;) {
+        int t;
+        if (persistent) {
+            if (tid == 0) s_tile = queue_next(queue, ntl);
+            __syncthreads();
+            t = s_tile;  // read by every thread before the tile's closing barrier
+            if (t < 0) break;
+            t += t_begin;
+        } else {
+            if (next >= t_hi) break;
+            t = next++;
+        }
         int lo = seg_lo, hi = seg_hi;
         while (lo < hi) {
             const int mid = (lo + hi + 1) >> 1;

@@ -105,6 +105,9 @@ struct BatchHdr {
     // numbered group by group.
     int32_t nseg[6];
     int32_t tiles[6];
+    // per map group, the 8 per-XCD tile counters of a persistent grid's
+    // dynamic queue (queue_next); written as 0 with the rest of the header
+    int32_t queue[6][8];
 };
 
 __host__ __device__ constexpr int map_group(int cls, bool half, bool exact = false) {

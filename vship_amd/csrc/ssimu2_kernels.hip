@@ -481,16 +481,33 @@ __global__ __launch_bounds__(256, (sizeof(AT) == 4 && TH <= 64) ? 2 : 1) void ma
 #define gr(k) ((AT)tap(k))
 
     // XCD-aware tile order: workgroups are dispatched round-robin over the 8
-    // XCDs, each with its own L2. Renumber them so that each XCD owns one
-    // contiguous range of virtual workgroups, and give each workgroup a
-    // contiguous range of tiles: neighbouring tiles, whose halos overlap, then
-    // run on the same XCD and share its L2.
+    // XCDs, each with its own L2. A grid of one workgroup per tile (or more)
+    // renumbers them so that each XCD owns one contiguous range of tiles:
+    // neighbouring tiles, whose halos overlap, then run on the same XCD and
+    // share its L2. A smaller, persistent grid takes its tiles from the
+    // per-XCD dynamic queue, which keeps the same ranges.
     const int nwg = gridDim.x;
-    const int vb = xcd_block(blockIdx.x, nwg);
     const int ntl = t_end - t_begin;
-    const int t_lo = t_begin + (int)(((int64_t)vb * ntl) / nwg);
+    const bool persistent = nwg < ntl;
+    int32_t* queue = const_cast<int32_t*>(hdr->queue[grp]);
+    __shared__ int s_tile;
+    // (with more workgroups than tiles, tile floor(v n / nwg) of virtual
+    // workgroup v, so that every XCD still gets tiles)
+    const int vb = xcd_block(blockIdx.x, nwg);
+    int next = t_begin + (int)(((int64_t)vb * ntl) / nwg);
     const int t_hi = t_begin + (int)(((int64_t)(vb + 1) * ntl) / nwg);
-    for (int t = t_lo; t < t_hi; t++) {
+    for (;;) {
+        int t;
+        if (persistent) {
+            if (tid == 0) s_tile = queue_next(queue, ntl);
+            __syncthreads();
+            t = s_tile;  // read by every thread before the tile's closing barrier
+            if (t < 0) break;
+            t += t_begin;
+        } else {
+            if (next >= t_hi) break;
+            t = next++;
+        }
         int lo = seg_lo, hi = seg_hi;
         while (lo < hi) {
             const int mid = (lo + hi + 1) >> 1;

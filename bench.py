@@ -610,10 +610,126 @@ def sub_served(a):
     return out
 
 
+def h2d_copy_rate(nbytes):
+    """GB/s of one synchronous page-locked host-to-device copy of nbytes (best of 3)."""
+    from vship_amd import api
+
+    src, dst = api.PinnedBuffer(nbytes), api.DeviceBuffer(nbytes)
+    src.array[:] = 1
+    best = 0.0
+    for _ in range(3):
+        t0 = time.perf_counter()
+        api._check(api.lib().Vship_MemcpyHtoD(dst.ptr, src.ptr, nbytes), "Vship_MemcpyHtoD")
+        best = max(best, nbytes / (time.perf_counter() - t0) / 1e9)
+    src.free()
+    dst.free()
+    return best
+
+
+def sub_host_input(a, pairs, want):
+    """The 1080p workload with its inputs in HOST memory: every step uploads
+    the 240 pairs through Vship_SSIMU2BatchSubmitHost, two steps in flight,
+    so that step k+1's upload runs while step k's kernels do. Page-locked
+    planes go by DMA directly; pageable ones through the staging ring."""
+    from vship_amd import api
+
+    n, steps = len(pairs), a.sub_steps
+    fmt, matrix = fmt_of(8)
+    out = {"metric": "ssimulacra2 frame-pairs/s, host input", "unit": "frame-pairs/s",
+           "config": {"workload": "ssimulacra2 1920x1080 YUV420P8 BT.709 limited, 240 pairs per step, planes in "
+                                  "host memory, uploaded every step (host-to-device copies inside the timed region)"}}
+    for kind in ("page_locked", "pageable"):
+        hp = api.HostPairs(pairs, fmt, 8, matrix, pinned=kind == "page_locked")
+        b = api.SSIMU2Batch(n, 1920, 1080, map_tile_height=a.map_th, persistent_grid=a.grid)
+        for _ in range(max(1, a.warmup)):
+            b.submit_host(hp.descs)
+            got = b.wait()
+        if not np.array_equal(got, want):
+            raise RuntimeError(f"host-input ({kind}) scores differ from the device-input ones")
+        t0 = time.perf_counter()
+        b.submit_host(hp.descs)
+        for k in range(steps):
+            if k + 1 < steps:
+                b.submit_host(hp.descs)
+            got = b.wait()
+        el = time.perf_counter() - t0
+        if not np.array_equal(got, want):
+            raise RuntimeError(f"host-input ({kind}) scores differ from the device-input ones")
+        out[kind] = {"value": round(n * steps / el, 2), "ms_per_step": round(el / steps * 1e3, 3),
+                     "upload_GBps": round(hp.nbytes * steps / el / 1e9, 2), "steps": steps}
+        if kind == "pageable":
+            out[kind]["copy_threads"] = 4
+        b.close()
+        hp.free()
+    nbytes = sum(p.nbytes for r, d in pairs for p in list(r) + list(d))
+    rate = h2d_copy_rate(nbytes)
+    out["input_bytes_per_step"] = nbytes
+    out["h2d_copy_GBps"] = round(rate, 2)
+    out["h2d_bound_pairs_per_s"] = round(n / (nbytes / (rate * 1e9)), 2)
+    out["value"] = out["page_locked"]["value"]
+    out["note"] = ("value: page-locked planes; h2d_bound_pairs_per_s: the rate a synchronous page-locked copy of "
+                   "the same bytes allows; scores equal to the device-input ones bit for bit")
+    return out
+
+
+def rgb_float_pairs(w, h, n, seed):
+    """Planar float RGB pairs in [0, 1] from the synthetic YUV420P8 clip
+    (BT.709 limited range, chroma repeated 2x2)."""
+    from vship_amd import synth
+
+    clip = synth.video_clip(w, h, 8, n, seed)
+    out = []
+    for f in range(n):
+        imgs = []
+        for yuv in clip.frame(f):
+            y = (yuv[0].astype(np.float32) - 16) / 219
+            u = (np.repeat(np.repeat(yuv[1], 2, 0), 2, 1)[:h, :w].astype(np.float32) - 128) / 224
+            v = (np.repeat(np.repeat(yuv[2], 2, 0), 2, 1)[:h, :w].astype(np.float32) - 128) / 224
+            r = y + 1.5748 * v
+            g = y - 0.1873 * u - 0.4681 * v
+            b = y + 1.8556 * u
+            imgs.append([np.ascontiguousarray(np.clip(c, 0, 1), np.float32) for c in (r, g, b)])
+        out.append(tuple(imgs))
+    clip.close()
+    return out
+
+
+def sub_vs_filter(a):
+    """vship.SSIMULACRA2 through the mock VapourSynth host: 48 frames of
+    1920x1080 float RGB in pageable memory (16 distinct pairs, each requested
+    three times), 8 request threads, 1/3/8 engines (numStream)."""
+    from vship_amd import api
+    from tests import vs_mock
+
+    if not os.path.exists(vs_mock.MOCK):
+        return {"value": None, "note": "mock host not built"}
+    w, h, uniq, reps = 1920, 1080, 16, 3
+    pairs = rgb_float_pairs(w, h, uniq, seed=9)
+    ref = [pairs[i % uniq][0] for i in range(uniq * reps)]
+    dist = [pairs[i % uniq][1] for i in range(uniq * reps)]
+    with api.SSIMU2Handler(w, h) as hd:
+        direct = np.array([hd.compute_float(r, d) for r, d in pairs])
+    per = {}
+    for ns in (1, 3, 8):
+        rc, scores, matches, err, ms = vs_mock.run_ssimu2(ref, dist, w, h, w * 4, num_stream=ns, gpu_id=0,
+                                                          core_threads=8, nthreads=8, timed=True)
+        if rc != 0 or matches != uniq * reps or not np.array_equal(scores, np.tile(direct, reps)):
+            raise RuntimeError(f"vs filter numStream {ns}: rc {rc} {err}")
+        per[str(ns)] = round(uniq * reps / ms * 1e3, 2)
+    best = max(per, key=lambda k: per[k])
+    return {"metric": "ssimulacra2 frame-pairs/s through vship.SSIMULACRA2", "unit": "frame-pairs/s",
+            "value": per[best], "num_stream": int(best), "per_num_stream": per,
+            "config": {"workload": f"{uniq * reps} frames {w}x{h} float RGB (RGBS) in pageable host memory, "
+                                   "8 request threads, mock VapourSynth host"},
+            "upload_GBps": round(per[best] * w * h * 4 * 6 / 1e9, 2),
+            "note": "frame requests only (filter creation excluded); scores equal to Vship_ComputeSSIMU2Floatv2's"}
+
+
 def brief(rec):
     keep = ("metric", "value", "unit", "steps", "ms_per_step", "dtype", "config", "latency_ms", "kernels_ms",
             "per_frame_path", "roofline", "score_vs_fp64_oracle", "clips_equal_alone", "server", "eager_value",
-            "graph_value", "host_input")
+            "graph_value", "page_locked", "pageable", "input_bytes_per_step", "h2d_copy_GBps",
+            "h2d_bound_pairs_per_s", "per_num_stream", "num_stream", "upload_GBps", "note")
     return {k: rec[k] for k in keep if k in rec}
 
 
@@ -638,8 +754,16 @@ def main():
         if not a.no_cpu:
             out["cpu_baseline"], out["score_vs_fp64_oracle"] = cpu_baseline(pairs, bits, scores, a)
         if not a.no_sub:
-            del pairs
             subs = {}
+            t1 = time.time()
+            subs["host_input"] = brief(sub_host_input(a, pairs, scores))
+            print(f"# sub-record host_input: {subs['host_input']['value']} pairs/s ({time.time() - t1:.0f}s)",
+                  flush=True)
+            del pairs
+            t1 = time.time()
+            subs["vs_filter"] = brief(sub_vs_filter(a))
+            print(f"# sub-record vs_filter: {subs['vs_filter']['value']} pairs/s ({time.time() - t1:.0f}s)",
+                  flush=True)
             for name, fn in (("4k10", sub_4k10), ("mixed", lambda x: sub_mixed(x, x.sub_steps)[0]),
                              ("served", sub_served)):
                 t1 = time.time()

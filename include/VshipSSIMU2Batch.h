@@ -106,14 +106,32 @@ EXPORTPREPROCESS Vship_Exception Vship_SSIMU2BatchInit(Vship_SSIMU2BatchHandler*
                                                        const Vship_SSIMU2BatchConfig* config);
 EXPORTPREPROCESS Vship_Exception Vship_SSIMU2BatchFree(Vship_SSIMU2BatchHandler handler);
 
-/* Enqueue a batch on the handler's stream (asynchronous). */
+/* Enqueue a batch on the handler's stream (asynchronous). Two batches may be
+ * outstanding (submitted and not yet waited for) at once; a third Submit
+ * returns Vship_BadHandler. */
 EXPORTPREPROCESS Vship_Exception Vship_SSIMU2BatchSubmit(Vship_SSIMU2BatchHandler handler,
                                                          const Vship_SSIMU2Pair* pairs, int npairs);
-/* Wait for the last submitted batch; scores[npairs] and, if not NULL,
+/* Submit for pairs whose plane pointers are HOST memory (the reference's
+ * per-frame copies, src/ssimu2/main.hpp:116-135, for a whole batch). Each
+ * plane is uploaded as one block of (rows - 1) strides plus one row on the
+ * handler's copy stream into one of two device input buffers: pageable memory
+ * through a ring of page-locked chunks filled by CPU threads, page-locked
+ * memory (Vship_PinnedMalloc) by one DMA per plane. The kernels wait for the
+ * upload, so batch k+1's upload overlaps batch k's kernels when batch k+1 is
+ * submitted before batch k is waited for. Returns once the caller's planes
+ * may be reused. Same limits as Submit. */
+EXPORTPREPROCESS Vship_Exception Vship_SSIMU2BatchSubmitHost(Vship_SSIMU2BatchHandler handler,
+                                                             const Vship_SSIMU2Pair* pairs, int npairs);
+/* Shape of SubmitHost's staging ring: chunk bytes (4 KiB .. 1 GiB), slots
+ * (2 .. 64) and CPU copy threads (1 .. 64). Defaults 8 MiB, 4, 4. Only
+ * before the handler's first SubmitHost (Vship_BadHandler after it). */
+EXPORTPREPROCESS Vship_Exception Vship_SSIMU2BatchHostStaging(Vship_SSIMU2BatchHandler handler,
+                                                              uint64_t chunk_bytes, int slots, int copy_threads);
+/* Wait for the oldest outstanding batch; scores[npairs] and, if not NULL,
  * features[npairs*108] (reference feature order) receive the results. */
 EXPORTPREPROCESS Vship_Exception Vship_SSIMU2BatchWait(Vship_SSIMU2BatchHandler handler, double* scores,
                                                        double* features);
-/* Submit + Wait. */
+/* Submit + Wait; Vship_BadHandler while a batch is outstanding. */
 EXPORTPREPROCESS Vship_Exception Vship_SSIMU2BatchCompute(Vship_SSIMU2BatchHandler handler,
                                                           const Vship_SSIMU2Pair* pairs, int npairs, double* scores,
                                                           double* features);

@@ -14,62 +14,11 @@ import pytest
 
 from vship_amd import api
 
+from tests.vs_mock import MOCK, VS_HEADER_DIR, libpath, mock, run_butter, run_ssimu2
+
 ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
-MOCK = os.path.join(ROOT, "tests", "_build", "libvs_mock_host.so")
-VS_HEADER_DIR = os.environ.get("VSHIP_VS_INCLUDE", "/root/reference/include")
 
 needs_mock = pytest.mark.skipif(not os.path.exists(MOCK), reason="mock host not built (python -m vship_amd.build)")
-
-
-def mock():
-    L = ctypes.CDLL(MOCK)
-    L.vsmock_registration.argtypes = [ctypes.c_char_p, ctypes.c_char_p, ctypes.c_int]
-    PP = ctypes.POINTER(ctypes.c_void_p)
-    L.vsmock_ssimu2.argtypes = [ctypes.c_char_p] + [ctypes.c_int] * 5 + [PP, PP] + [ctypes.c_int] * 7 + [
-        ctypes.POINTER(ctypes.c_double), ctypes.POINTER(ctypes.c_int), ctypes.c_char_p, ctypes.c_int]
-    L.vsmock_gpuinfo.argtypes = [ctypes.c_char_p, ctypes.c_int, ctypes.c_char_p, ctypes.c_int]
-    L.vsmock_butter.argtypes = [ctypes.c_char_p] + [ctypes.c_int] * 5 + [PP, PP, ctypes.c_int, ctypes.c_int,
-                                                                          ctypes.c_double] + [ctypes.c_int] * 5 + [
-        ctypes.POINTER(ctypes.c_double), ctypes.POINTER(ctypes.c_float), ctypes.POINTER(ctypes.c_int),
-        ctypes.c_char_p, ctypes.c_int]
-    return L
-
-
-def libpath():
-    return api.lib_path().encode()
-
-
-def run_ssimu2(frames_ref, frames_dist, w, h, stride, w2=None, h2=None, rgbs=(1, 1), num_stream=-1, gpu_id=-1,
-               core_threads=4, nthreads=4):
-    """frames_*: list of 3 (h, stride/4) float32 arrays per frame."""
-    n = len(frames_ref)
-    ptrs_r = (ctypes.c_void_p * (3 * n))(*[p.ctypes.data for f in frames_ref for p in f])
-    ptrs_d = (ctypes.c_void_p * (3 * n))(*[p.ctypes.data for f in frames_dist for p in f])
-    scores = np.full(n, np.nan)
-    matches = ctypes.c_int(0)
-    err = ctypes.create_string_buffer(4096)
-    rc = mock().vsmock_ssimu2(libpath(), w, h, w2 or w, h2 or h, n, ptrs_r, ptrs_d, stride, rgbs[0], rgbs[1],
-                              num_stream, gpu_id, core_threads, nthreads,
-                              scores.ctypes.data_as(ctypes.POINTER(ctypes.c_double)), ctypes.byref(matches), err,
-                              len(err))
-    return rc, scores, matches.value, err.value.decode()
-
-
-def run_butter(frames_ref, frames_dist, w, h, stride, w2=None, h2=None, qnorm=-1, intensity=-1.0, distmap=-1,
-               num_stream=-1, gpu_id=-1, core_threads=4, nthreads=4):
-    n = len(frames_ref)
-    ptrs_r = (ctypes.c_void_p * (3 * n))(*[p.ctypes.data for f in frames_ref for p in f])
-    ptrs_d = (ctypes.c_void_p * (3 * n))(*[p.ctypes.data for f in frames_dist for p in f])
-    norms = np.full((n, 3), np.nan)
-    maps = np.zeros((n, h, w), np.float32)
-    matches = ctypes.c_int(0)
-    err = ctypes.create_string_buffer(4096)
-    rc = mock().vsmock_butter(libpath(), w, h, w2 or w, h2 or h, n, ptrs_r, ptrs_d, stride, qnorm, intensity,
-                              distmap, num_stream, gpu_id, core_threads, nthreads,
-                              norms.ctypes.data_as(ctypes.POINTER(ctypes.c_double)),
-                              maps.ctypes.data_as(ctypes.POINTER(ctypes.c_float)), ctypes.byref(matches), err,
-                              len(err))
-    return rc, norms, maps, matches.value, err.value.decode()
 
 
 def padded_frames(n, w, h, stride_px, seed):
@@ -227,3 +176,35 @@ def test_butteraugli_filter_scores_every_frame():
         want = bl.butteraugli(r, d, 2, 203.0)
         assert np.all(np.abs(norms2[i] - want) <= 1e-4 * want)
 
+
+
+@pytest.mark.gpu
+@needs_mock
+def test_ssimulacra2_filter_1080p_throughput():
+    """48 frames of 1920x1080 float RGB in pageable memory (16 distinct
+    pairs, each requested three times) through the filter: every score equal
+    to the per-frame entry point's, and the frame rate printed for 1, 3 and 8
+    engines on 8 request threads."""
+    import time
+
+    import vship_amd
+
+    vship_amd.gpu_full_check(0)
+    w, h, uniq, reps = 1920, 1080, 16, 3
+    t0 = time.time()
+    from tests.gpu_helpers import rgb_pair
+
+    pairs = [rgb_pair(w, h, api.PF_RGB_F32, 700 + i) for i in range(uniq)]
+    print(f"case: vship.SSIMULACRA2 via the mock host, {uniq * reps} frames {w}x{h} float RGB "
+          f"(generated in {time.time() - t0:.1f}s)", flush=True)
+    ref = [pairs[i % uniq][0] for i in range(uniq * reps)]
+    dist = [pairs[i % uniq][1] for i in range(uniq * reps)]
+    with vship_amd.SSIMU2Handler(w, h) as hd:
+        direct = np.array([hd.compute_float(r, d) for r, d in pairs])
+    for ns in (1, 3, 8):
+        rc, scores, matches, err, ms = run_ssimu2(ref, dist, w, h, w * 4, num_stream=ns, gpu_id=0, core_threads=8,
+                                                  nthreads=8, timed=True)
+        assert rc == 0, err
+        assert matches == uniq * reps
+        print(f"  numStream {ns}, 8 threads: {uniq * reps / ms * 1e3:.0f} pairs/s ({ms:.1f} ms)", flush=True)
+        assert np.array_equal(scores, np.tile(direct, reps))

@@ -12,6 +12,7 @@
 #include <atomic>
 #include <cstdio>
 #include <cstdlib>
+#include <chrono>
 #include <cstring>
 #include <map>
 #include <mutex>
@@ -42,6 +43,7 @@ struct Frame {
     std::vector<uint8_t> own;
     Map props;
     bool output = false;
+    bool shared = false;  // planes belong to the frame this one was copied from
 };
 struct Node {
     VideoInfo vi{};
@@ -100,16 +102,15 @@ void m_freeFrame(const Frame* f) {
         g_output_frames--;
     }
 }
+// As in VapourSynth, a copied frame shares the source's planes until one of
+// them is written (m_getWritePtr copies them then).
 Frame* m_copyFrame(const Frame* f, Core*) {
     Frame* c = new Frame;
     c->w = f->w;
     c->h = f->h;
     c->stride = f->stride;
-    c->own.resize((size_t)f->stride * f->h * 3);
-    for (int p = 0; p < 3; p++) {
-        std::memcpy(c->own.data() + (size_t)p * f->stride * f->h, f->planes[p], (size_t)f->stride * f->h);
-        c->planes[p] = c->own.data() + (size_t)p * f->stride * f->h;
-    }
+    for (int p = 0; p < 3; p++) c->planes[p] = f->planes[p];
+    c->shared = true;
     c->props = f->props;
     c->output = true;
     g_output_frames++;
@@ -127,7 +128,17 @@ Frame* m_newVideoFrame(const VideoFormat* fmt, int w, int h, const Frame* prop_s
     g_output_frames++;
     return c;
 }
-uint8_t* m_getWritePtr(Frame* f, int p) { return const_cast<uint8_t*>(f->planes[p]); }
+uint8_t* m_getWritePtr(Frame* f, int p) {
+    if (f->shared) {
+        f->own.resize((size_t)f->stride * f->h * 3);
+        for (int q = 0; q < 3; q++) {
+            std::memcpy(f->own.data() + (size_t)q * f->stride * f->h, f->planes[q], (size_t)f->stride * f->h);
+            f->planes[q] = f->own.data() + (size_t)q * f->stride * f->h;
+        }
+        f->shared = false;
+    }
+    return const_cast<uint8_t*>(f->planes[p]);
+}
 int m_queryVideoFormat(VideoFormat* f, int family, int sample, int bits, int ssw, int ssh, Core*) {
     f->colorFamily = family;
     f->sampleType = sample;
@@ -357,9 +368,25 @@ int vsmock_registration(const char* libpath, char* out, int len) {
 // the conversion path); w2/h2: size of the distorted clip. numStream/gpu_id
 // < 0: argument absent. Returns 0 and scores[nframes] on success; 1 with the
 // create error in err; 2 with the first frame error in err.
+int vsmock_ssimu2_timed(const char* libpath, int w, int h, int w2, int h2, int nframes, const uint8_t* const* ref,
+                        const uint8_t* const* dist, int stride, int rgbs1, int rgbs2, int numStream, int gpu_id,
+                        int core_threads, int nthreads, double* scores, int* output_matches, char* err, int errlen,
+                        double* loop_ms);
+
 int vsmock_ssimu2(const char* libpath, int w, int h, int w2, int h2, int nframes, const uint8_t* const* ref,
                   const uint8_t* const* dist, int stride, int rgbs1, int rgbs2, int numStream, int gpu_id,
                   int core_threads, int nthreads, double* scores, int* output_matches, char* err, int errlen) {
+    return vsmock_ssimu2_timed(libpath, w, h, w2, h2, nframes, ref, dist, stride, rgbs1, rgbs2, numStream, gpu_id,
+                               core_threads, nthreads, scores, output_matches, err, errlen, nullptr);
+}
+
+// vsmock_ssimu2, and *loop_ms (if not null) receives the wall time of the
+// frame requests alone: from the first request to the last frame returned,
+// without the filter's creation and free.
+int vsmock_ssimu2_timed(const char* libpath, int w, int h, int w2, int h2, int nframes, const uint8_t* const* ref,
+                        const uint8_t* const* dist, int stride, int rgbs1, int rgbs2, int numStream, int gpu_id,
+                        int core_threads, int nthreads, double* scores, int* output_matches, char* err, int errlen,
+                        double* loop_ms) {
     Plugin p;
     std::string e;
     if (!load_plugin(libpath, &p, &e)) {
@@ -411,14 +438,17 @@ int vsmock_ssimu2(const char* libpath, int w, int h, int w2, int h2, int nframes
                 scores[i] = it->second[0].f;
             }
             for (int pl = 0; pl < 3 && ok; pl++)
-                ok = std::memcmp(r->planes[pl], dist[i * 3 + pl], (size_t)stride * h) == 0;
+                ok = r->planes[pl] == dist[i * 3 + pl] ||
+                     std::memcmp(r->planes[pl], dist[i * 3 + pl], (size_t)stride * h) == 0;
             matches += ok ? 1 : 0;
             m_freeFrame(r);
         }
     };
+    const auto t0 = std::chrono::steady_clock::now();
     std::vector<std::thread> th;
     for (int t = 0; t < nthreads; t++) th.emplace_back(work, t);
     for (auto& x : th) x.join();
+    if (loop_ms) *loop_ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
     filt->free(filt->inst, &core, &api);
     if (output_matches) *output_matches = matches;
     if (!first_err.empty()) {

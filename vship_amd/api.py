@@ -174,6 +174,10 @@ def lib():
     L.Vship_SSIMU2BatchFree.argtypes = [Handler]
     L.Vship_SSIMU2BatchSubmit.restype = E
     L.Vship_SSIMU2BatchSubmit.argtypes = [Handler, ctypes.POINTER(Pair), ctypes.c_int]
+    L.Vship_SSIMU2BatchSubmitHost.restype = E
+    L.Vship_SSIMU2BatchSubmitHost.argtypes = [Handler, ctypes.POINTER(Pair), ctypes.c_int]
+    L.Vship_SSIMU2BatchHostStaging.restype = E
+    L.Vship_SSIMU2BatchHostStaging.argtypes = [Handler, ctypes.c_uint64, ctypes.c_int, ctypes.c_int]
     L.Vship_SSIMU2BatchWait.restype = E
     L.Vship_SSIMU2BatchWait.argtypes = [Handler, ctypes.c_void_p, ctypes.c_void_p]
     L.Vship_SSIMU2BatchCompute.restype = E
@@ -438,8 +442,64 @@ class DevicePairs:
         self.buffer.free()
 
 
+class HostPairs:
+    """Frame pairs in host memory with the Vship_SSIMU2Pair descriptors that
+    point at them, for SSIMU2Batch.submit_host. pinned=True packs them into
+    one page-locked allocation (Vship_PinnedMalloc), which submit_host
+    uploads by DMA directly; pinned=False keeps them in pageable numpy memory,
+    which goes through the handler's page-locked staging ring."""
+
+    def __init__(self, pairs, fmt, bits=8, matrix=MATRIX_BT709, chroma_location=CHROMA_LEFT, pinned=False):
+        self.n = len(pairs)
+        self.descs = (Pair * self.n)()
+        self.pinned = None
+        self._keep = []
+        # pageable planes keep their row pitch (rows must be contiguous);
+        # page-locked ones are packed
+        planes_all = [[p if p.ndim == 2 and p.strides[1] == p.itemsize and p.strides[0] > 0 else
+                       np.ascontiguousarray(p) for p in (np.asarray(q) for q in list(ref) + list(dist))]
+                      for ref, dist in pairs]
+        if pinned:
+            total = sum((p.nbytes + 255) // 256 * 256 for planes in planes_all for p in planes)
+            self.pinned = PinnedBuffer(total)
+            off = 0
+            placed = []
+            for planes in planes_all:
+                q = []
+                for p in planes:
+                    v = self.pinned.array[off:off + p.nbytes].view(p.dtype).reshape(p.shape)  # packed rows
+                    v[...] = p
+                    q.append(v)
+                    off += (p.nbytes + 255) // 256 * 256
+                placed.append(q)
+            planes_all = placed
+        else:
+            self._keep = planes_all
+        self.nbytes = sum(p.nbytes for planes in planes_all for p in planes)
+        for i, planes in enumerate(planes_all):
+            d = self.descs[i]
+            for j in range(3):
+                d.ref[j] = planes[j].ctypes.data
+                d.dist[j] = planes[3 + j].ctypes.data
+                d.ref_stride[j] = planes[j].strides[0]
+                d.dist_stride[j] = planes[3 + j].strides[0]
+            d.height, d.width = planes[0].shape
+            d.format = fmt
+            d.bits = bits
+            d.matrix = matrix
+            d.chroma_location = chroma_location
+
+    def free(self):
+        if self.pinned is not None:
+            self.pinned.free()
+            self.pinned = None
+        self._keep = []
+
+
 class SSIMU2Batch:
-    """Batched SSIMULACRA2 over device-resident pairs (Vship_SSIMU2Batch*)."""
+    """Batched SSIMULACRA2 over device-resident pairs (Vship_SSIMU2Batch*);
+    submit_host takes pairs in host memory. Two batches may be outstanding:
+    wait() returns the oldest."""
 
     def __init__(self, max_pairs, max_width, max_height, fp16=False, arena_bytes=0, map_tile_height=0,
                  persistent_grid=0, use_graph=False, profile=False):
@@ -448,24 +508,42 @@ class SSIMU2Batch:
                           map_tile_height=int(map_tile_height), persistent_grid=int(persistent_grid),
                           use_graph=int(bool(use_graph)), profile=int(bool(profile)))
         self.max_pairs = max_pairs
+        self._ns = []
         self._h = Handler()
         _check(lib().Vship_SSIMU2BatchInit(ctypes.byref(self._h), ctypes.byref(cfg)), "Vship_SSIMU2BatchInit")
 
     def submit(self, descs, n=None):
         n = len(descs) if n is None else n
         _check(lib().Vship_SSIMU2BatchSubmit(self._h, descs, n), "Vship_SSIMU2BatchSubmit")
-        self._n = n
+        self._ns.append(n)
+
+    def submit_host(self, descs, n=None):
+        """descs: Vship_SSIMU2Pair descriptors with host plane pointers (HostPairs.descs)."""
+        n = len(descs) if n is None else n
+        _check(lib().Vship_SSIMU2BatchSubmitHost(self._h, descs, n), "Vship_SSIMU2BatchSubmitHost")
+        self._ns.append(n)
+
+    def host_staging(self, chunk_bytes, slots, copy_threads):
+        _check(lib().Vship_SSIMU2BatchHostStaging(self._h, int(chunk_bytes), int(slots), int(copy_threads)),
+               "Vship_SSIMU2BatchHostStaging")
 
     def wait(self, features=False):
-        scores = np.zeros(self._n)
-        feats = np.zeros((self._n, 108)) if features else None
+        n = self._ns[0] if self._ns else 1
+        scores = np.zeros(n)
+        feats = np.zeros((n, 108)) if features else None
         _check(lib().Vship_SSIMU2BatchWait(self._h, scores.ctypes.data, feats.ctypes.data if features else None),
                "Vship_SSIMU2BatchWait")
+        self._ns.pop(0)
         return (scores, feats) if features else scores
 
     def compute(self, descs, n=None, features=False):
-        self.submit(descs, n)
-        return self.wait(features)
+        """Vship_SSIMU2BatchCompute: refused while a batch is outstanding."""
+        n = len(descs) if n is None else n
+        scores = np.zeros(n)
+        feats = np.zeros((n, 108)) if features else None
+        _check(lib().Vship_SSIMU2BatchCompute(self._h, descs, n, scores.ctypes.data,
+                                              feats.ctypes.data if features else None), "Vship_SSIMU2BatchCompute")
+        return (scores, feats) if features else scores
 
     def stats(self):
         s = BatchStats()

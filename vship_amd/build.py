@@ -22,7 +22,7 @@ HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 ARCH = os.environ.get("VSHIP_OFFLOAD_ARCH", "gfx950")
 
 SOURCES = ["ssimu2_kernels.hip", "map_mfma.hip", "engine.hip", "butter.hip", "vship_api.hip", "serve.hip", "tune.hip",
-           "vs_filter.hip", "noalloc.hip"]
+           "vs_filter.hip", "noalloc.hip", "host_stage.hip"]
 HIPFLAGS = [
     f"--offload-arch={ARCH}", "-O3", "-std=c++17", "-fPIC", "-fvisibility=hidden",
     "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",

@@ -6,6 +6,7 @@
 
 #include "butter.hpp"
 #include "engine.hpp"
+#include "host_stage.hpp"
 
 namespace vship {
 
@@ -29,13 +30,12 @@ bool tuned_shape(int w, int h, bool fp16, int* th, int* grid, int* graph);
 void record_shape(const TuneEntry& e);
 
 // One per-frame engine of fixed size, as a Vship_SSIMU2Handler refers to
-// (src/ssimu2/main.hpp:153-191 of the reference): host RGB planes are copied
-// row-packed into a device staging area and scored as a batch of one.
+// (src/ssimu2/main.hpp:153-191 of the reference): host RGB planes are
+// uploaded through the engine's page-locked staging ring (Engine::submit_host)
+// and scored as a batch of one.
 struct FrameHandle {
     int width, height, device;
     std::unique_ptr<Engine> engine;
-    uint8_t* staging = nullptr;
-    size_t staging_bytes = 0;
     std::atomic<bool> busy{false};
 
     FrameHandle(int w, int h, int device);
@@ -46,11 +46,12 @@ struct FrameHandle {
 
 // One Butteraugli engine of fixed size, as a Vship_ButteraugliHandler refers
 // to (ButterComputingImplementation, src/butter/main.hpp:197-240 of the
-// reference): host RGB planes are copied into device staging and scored as a
-// batch of one.
+// reference): host RGB planes are uploaded into device staging through a
+// page-locked staging ring and scored as a batch of one.
 struct ButterHandle {
     int width, height, device;
     std::unique_ptr<bt::Engine> engine;
+    std::unique_ptr<HostStager> stager;
     uint8_t* staging = nullptr;
     size_t staging_bytes = 0;
     float* map = nullptr;  // device map, rows of width floats

@@ -5,6 +5,7 @@
 #include <cmath>
 #include <cstring>
 
+#include "host_stage.hpp"
 #include "ssimu2_kernels.hpp"
 
 namespace vship {
@@ -144,9 +145,16 @@ Engine::Engine(const EngineOptions& opt) : opt_(opt) {
     try {
         HIP_OR_THROW(hipStreamCreateWithFlags(&stream_, hipStreamNonBlocking), E_OutOfVRAM);
         desc_ = desc_layout(opt_.max_pairs);
-        HIP_OR_THROW(hipHostMalloc(reinterpret_cast<void**>(&desc_h_), desc_.bytes, hipHostMallocDefault),
-                     E_OutOfRAM);
-        std::memset(desc_h_, 0, desc_.bytes);
+        const size_t out_bytes = sizeof(double) * 109 * opt_.max_pairs;
+        for (int k = 0; k < kDepth; k++) {
+            HIP_OR_THROW(hipHostMalloc(reinterpret_cast<void**>(&desc_h_[k]), desc_.bytes, hipHostMallocDefault),
+                         E_OutOfRAM);
+            std::memset(desc_h_[k], 0, desc_.bytes);
+            HIP_OR_THROW(hipHostMalloc(reinterpret_cast<void**>(&out_h_[k]), out_bytes, hipHostMallocDefault),
+                         E_OutOfRAM);
+            for (auto& e : ev_[k]) HIP_OR_THROW(hipEventCreate(&e), E_OutOfRAM);
+            HIP_OR_THROW(hipEventCreateWithFlags(&done_[k], hipEventDisableTiming), E_OutOfRAM);
+        }
         HIP_OR_THROW(hipMalloc(reinterpret_cast<void**>(&desc_d_), desc_.bytes), E_OutOfVRAM);
         arena_bytes_ = opt_.arena_bytes / 256 * 256;
         if (arena_bytes_ <= 0) VSHIP_THROW(E_OutOfVRAM);
@@ -155,10 +163,7 @@ Engine::Engine(const EngineOptions& opt) : opt_(opt) {
         if (max_map_tiles_ <= 0) VSHIP_THROW(E_OutOfVRAM);
         HIP_OR_THROW(hipMalloc(reinterpret_cast<void**>(&partials_d_), sizeof(double) * kFeat * max_map_tiles_),
                      E_OutOfVRAM);
-        const size_t out_bytes = sizeof(double) * 109 * opt_.max_pairs;
         HIP_OR_THROW(hipMalloc(reinterpret_cast<void**>(&out_d_), out_bytes), E_OutOfVRAM);
-        HIP_OR_THROW(hipHostMalloc(reinterpret_cast<void**>(&out_h_), out_bytes, hipHostMallocDefault), E_OutOfRAM);
-        for (auto& e : ev_) HIP_OR_THROW(hipEventCreate(&e), E_OutOfRAM);
     } catch (...) {
         release();
         throw;
@@ -169,28 +174,42 @@ Engine::~Engine() { release(); }
 
 void Engine::release() {
     if (stream_) (void)hipStreamSynchronize(stream_);
-    if (graph_exec_) (void)hipGraphExecDestroy(graph_exec_);
-    if (graph_) (void)hipGraphDestroy(graph_);
-    graph_exec_ = nullptr;
-    graph_ = nullptr;
-    for (auto& e : ev_)
-        if (e) {
-            (void)hipEventDestroy(e);
-            e = nullptr;
-        }
-    if (out_h_) (void)hipHostFree(out_h_);
+    if (copy_) (void)hipStreamSynchronize(copy_);
+    stager_.reset();
+    for (int k = 0; k < kDepth; k++) {
+        if (graph_exec_[k]) (void)hipGraphExecDestroy(graph_exec_[k]);
+        if (graph_[k]) (void)hipGraphDestroy(graph_[k]);
+        graph_exec_[k] = nullptr;
+        graph_[k] = nullptr;
+        for (hipEvent_t* e : {&done_[k], &in_ready_[k], &in_free_[k]})
+            if (*e) {
+                (void)hipEventDestroy(*e);
+                *e = nullptr;
+            }
+        for (auto& e : ev_[k])
+            if (e) {
+                (void)hipEventDestroy(e);
+                e = nullptr;
+            }
+        if (out_h_[k]) (void)hipHostFree(out_h_[k]);
+        if (desc_h_[k]) (void)hipHostFree(desc_h_[k]);
+        if (in_d_[k]) (void)hipFree(in_d_[k]);
+        out_h_[k] = nullptr;
+        desc_h_[k] = nullptr;
+        in_d_[k] = nullptr;
+        in_cap_[k] = 0;
+    }
     if (out_d_) (void)hipFree(out_d_);
     if (partials_d_) (void)hipFree(partials_d_);
     if (arena_d_) (void)hipFree(arena_d_);
     if (desc_d_) (void)hipFree(desc_d_);
-    if (desc_h_) (void)hipHostFree(desc_h_);
+    if (copy_) (void)hipStreamDestroy(copy_);
     if (stream_) (void)hipStreamDestroy(stream_);
-    out_h_ = nullptr;
     out_d_ = nullptr;
     partials_d_ = nullptr;
     arena_d_ = nullptr;
     desc_d_ = nullptr;
-    desc_h_ = nullptr;
+    copy_ = nullptr;
     stream_ = nullptr;
 }
 
@@ -329,10 +348,6 @@ PlanInfo plan_batch(const PairInput* in, int n, int max_pairs, bool fp16, int ma
     return P;
 }
 
-void Engine::plan(const PairInput* in, int n) {
-    plan_ = plan_batch(in, n, opt_.max_pairs, opt_.fp16, opt_.map_th, arena_bytes_, max_map_tiles_, desc_h_, desc_);
-}
-
 void launch_batch(const DescView& d, const PlanInfo& P, const LaunchShape& S, void* arena, double* partials,
                   double* scores, double* features, const Taps& taps, hipStream_t st, hipEvent_t* ev) {
     const bool graph = S.graph;
@@ -378,7 +393,7 @@ void launch_batch(const DescView& d, const PlanInfo& P, const LaunchShape& S, vo
     if (ev) HIP_OR_THROW(hipEventRecord(ev[3], st), E_BadDeviceCode);
 }
 
-void Engine::enqueue_body() {
+void Engine::enqueue_body(int k) {
     const bool prof = opt_.profile && !opt_.use_graph;
     LaunchShape S;
     S.fp16 = opt_.fp16;
@@ -387,60 +402,138 @@ void Engine::enqueue_body() {
     S.graph = opt_.use_graph;
     S.max_pairs = opt_.max_pairs;
     S.num_cus = num_cus_;
-    if (prof) HIP_OR_THROW(hipEventRecord(ev_[0], stream_), E_BadDeviceCode);
-    HIP_OR_THROW(hipMemcpyAsync(desc_d_, desc_h_, desc_.bytes, hipMemcpyHostToDevice, stream_), E_BadDeviceCode);
-    if (prof) HIP_OR_THROW(hipEventRecord(ev_[1], stream_), E_BadDeviceCode);
+    hipEvent_t* ev = ev_[k];
+    if (prof) HIP_OR_THROW(hipEventRecord(ev[0], stream_), E_BadDeviceCode);
+    HIP_OR_THROW(hipMemcpyAsync(desc_d_, desc_h_[k], desc_.bytes, hipMemcpyHostToDevice, stream_),
+                 E_BadDeviceCode);
+    if (prof) HIP_OR_THROW(hipEventRecord(ev[1], stream_), E_BadDeviceCode);
     double* scores_d = out_d_;
     double* feats_d = out_d_ + opt_.max_pairs;
     launch_batch(desc_view(desc_d_, desc_), plan_, S, arena_d_, partials_d_, scores_d, feats_d, taps_, stream_,
-                 prof ? ev_ + 2 : nullptr);
+                 prof ? ev + 2 : nullptr);
     const int np = opt_.use_graph ? opt_.max_pairs : plan_.npairs;
-    HIP_OR_THROW(hipMemcpyAsync(out_h_, scores_d, sizeof(double) * np, hipMemcpyDeviceToHost, stream_),
+    HIP_OR_THROW(hipMemcpyAsync(out_h_[k], scores_d, sizeof(double) * np, hipMemcpyDeviceToHost, stream_),
                  E_BadDeviceCode);
-    HIP_OR_THROW(hipMemcpyAsync(out_h_ + opt_.max_pairs, feats_d, sizeof(double) * 108 * np,
+    HIP_OR_THROW(hipMemcpyAsync(out_h_[k] + opt_.max_pairs, feats_d, sizeof(double) * 108 * np,
                                 hipMemcpyDeviceToHost, stream_),
                  E_BadDeviceCode);
-    if (prof) HIP_OR_THROW(hipEventRecord(ev_[6], stream_), E_BadDeviceCode);
+    if (prof) HIP_OR_THROW(hipEventRecord(ev[6], stream_), E_BadDeviceCode);
 }
 
 void Engine::submit(const PairInput* pairs, int n) {
-    if (pending_) VSHIP_THROW(E_BadHandler);
+    if (outstanding() >= kDepth) VSHIP_THROW(E_BadHandler);
     HIP_OR_THROW(hipSetDevice(opt_.device), E_BadDeviceArgument);
-    plan(pairs, n);
+    // slot k was last used by batch head_ - kDepth, which has been waited for,
+    // so its descriptor and result buffers are free
+    const int k = (int)(head_ % kDepth);
+    plan_ = plan_batch(pairs, n, opt_.max_pairs, opt_.fp16, opt_.map_th, arena_bytes_, max_map_tiles_, desc_h_[k],
+                       desc_);
     if (opt_.use_graph) {
-        if (!graph_exec_) {
+        if (!graph_exec_[k]) {
             HIP_OR_THROW(hipStreamBeginCapture(stream_, hipStreamCaptureModeThreadLocal), E_BadDeviceCode);
             try {
-                enqueue_body();
+                enqueue_body(k);
             } catch (...) {
                 hipGraph_t g = nullptr;
                 (void)hipStreamEndCapture(stream_, &g);
                 if (g) (void)hipGraphDestroy(g);
                 throw;
             }
-            HIP_OR_THROW(hipStreamEndCapture(stream_, &graph_), E_BadDeviceCode);
-            HIP_OR_THROW(hipGraphInstantiate(&graph_exec_, graph_, nullptr, nullptr, 0), E_BadDeviceCode);
+            HIP_OR_THROW(hipStreamEndCapture(stream_, &graph_[k]), E_BadDeviceCode);
+            HIP_OR_THROW(hipGraphInstantiate(&graph_exec_[k], graph_[k], nullptr, nullptr, 0), E_BadDeviceCode);
         }
-        HIP_OR_THROW(hipGraphLaunch(graph_exec_, stream_), E_BadDeviceCode);
+        HIP_OR_THROW(hipGraphLaunch(graph_exec_[k], stream_), E_BadDeviceCode);
     } else {
-        enqueue_body();
+        enqueue_body(k);
     }
-    pending_ = true;
+    HIP_OR_THROW(hipEventRecord(done_[k], stream_), E_BadDeviceCode);
+    npairs_[k] = plan_.npairs;
+    head_++;
 }
 
+void Engine::submit_host(const PairInput* pairs, int n) {
+    if (outstanding() >= kDepth) VSHIP_THROW(E_BadHandler);
+    if (!pairs || n < 1 || n > opt_.max_pairs) VSHIP_THROW(E_BadPointer);
+    HIP_OR_THROW(hipSetDevice(opt_.device), E_BadDeviceArgument);
+    // one block per plane: (rows - 1) strides plus one row, 256-byte aligned
+    std::vector<size_t> blk((size_t)n * 6), off((size_t)n * 6);
+    size_t total = 0;
+    for (int i = 0; i < n; i++) {
+        const PairInput& p = pairs[i];
+        if (p.w < 1 || p.h < 1 || p.fmt < 0 || p.fmt > IN_YUV410_U16) VSHIP_THROW(E_BadPointer);
+        const int64_t bps = input_bytes_per_sample(p.fmt);
+        for (int im = 0; im < 2; im++)
+            for (int pl = 0; pl < 3; pl++) {
+                const int64_t row = (int64_t)plane_w(p, pl) * bps;
+                const int64_t st = p.stride[im][pl];
+                if (!p.src[im][pl] || st < row) VSHIP_THROW(E_BadPointer);
+                const size_t j = (size_t)i * 6 + im * 3 + pl;
+                blk[j] = (size_t)st * (plane_h(p, pl) - 1) + (size_t)row;
+                off[j] = total;
+                total += (blk[j] + 255) / 256 * 256;
+            }
+    }
+    if (!copy_) {
+        HIP_OR_THROW(hipStreamCreateWithFlags(&copy_, hipStreamNonBlocking), E_OutOfVRAM);
+        for (int k = 0; k < kDepth; k++) {
+            HIP_OR_THROW(hipEventCreateWithFlags(&in_ready_[k], hipEventDisableTiming), E_OutOfRAM);
+            HIP_OR_THROW(hipEventCreateWithFlags(&in_free_[k], hipEventDisableTiming), E_OutOfRAM);
+        }
+        stager_ = std::make_unique<HostStager>(opt_.device, (size_t)opt_.host_chunk_bytes, opt_.host_slots,
+                                               opt_.host_copy_threads);
+    }
+    const int k = (int)(head_ % kDepth);
+    if (total > in_cap_[k]) {
+        // the batch that last read this buffer has been waited for
+        if (in_d_[k]) (void)hipFree(in_d_[k]);
+        in_d_[k] = nullptr;
+        in_cap_[k] = 0;
+        HIP_OR_THROW(hipMalloc(reinterpret_cast<void**>(&in_d_[k]), total), E_OutOfVRAM);
+        in_cap_[k] = total;
+    }
+    // the upload into buffer k starts after the kernels that read it last
+    HIP_OR_THROW(hipStreamWaitEvent(copy_, in_free_[k], 0), E_BadDeviceCode);
+    std::vector<PairInput> dev(pairs, pairs + n);
+    for (int i = 0; i < n; i++)
+        for (int im = 0; im < 2; im++)
+            for (int pl = 0; pl < 3; pl++) {
+                const size_t j = (size_t)i * 6 + im * 3 + pl;
+                uint8_t* dst = in_d_[k] + off[j];
+                stager_->upload(dst, pairs[i].src[im][pl], blk[j], copy_);
+                dev[i].src[im][pl] = dst;
+            }
+    HIP_OR_THROW(hipEventRecord(in_ready_[k], copy_), E_BadDeviceCode);
+    HIP_OR_THROW(hipStreamWaitEvent(stream_, in_ready_[k], 0), E_BadDeviceCode);
+    submit(dev.data(), n);
+    HIP_OR_THROW(hipEventRecord(in_free_[k], stream_), E_BadDeviceCode);
+}
+
+void Engine::set_host_staging(int64_t chunk_bytes, int slots, int copy_threads) {
+    if (stager_) VSHIP_THROW(E_BadHandler);
+    opt_.host_chunk_bytes = chunk_bytes;
+    opt_.host_slots = slots;
+    opt_.host_copy_threads = copy_threads;
+}
+
+uint64_t Engine::host_pageable_bytes() const { return stager_ ? stager_->pageable_bytes() : 0; }
+uint64_t Engine::host_direct_bytes() const { return stager_ ? stager_->direct_bytes() : 0; }
+
 void Engine::wait(double* scores, double* features) {
-    if (!pending_) VSHIP_THROW(E_BadHandler);
-    pending_ = false;
-    HIP_OR_THROW(hipStreamSynchronize(stream_), E_BadDeviceCode);
-    if (scores) std::memcpy(scores, out_h_, sizeof(double) * plan_.npairs);
-    if (features) std::memcpy(features, out_h_ + opt_.max_pairs, sizeof(double) * 108 * plan_.npairs);
+    if (outstanding() < 1) VSHIP_THROW(E_BadHandler);
+    const int k = (int)(tail_ % kDepth);
+    tail_++;
+    HIP_OR_THROW(hipEventSynchronize(done_[k]), E_BadDeviceCode);
+    const int np = npairs_[k];
+    if (scores) std::memcpy(scores, out_h_[k], sizeof(double) * np);
+    if (features) std::memcpy(features, out_h_[k] + opt_.max_pairs, sizeof(double) * 108 * np);
     if (opt_.profile && !opt_.use_graph) {
-        (void)hipEventElapsedTime(&timings_.prep_ms, ev_[1], ev_[2]);
-        (void)hipEventElapsedTime(&timings_.map32_ms, ev_[2], ev_[3]);
-        (void)hipEventElapsedTime(&timings_.map64_ms, ev_[3], ev_[4]);
-        (void)hipEventElapsedTime(&timings_.map_ms, ev_[2], ev_[4]);
-        (void)hipEventElapsedTime(&timings_.finalize_ms, ev_[4], ev_[5]);
-        (void)hipEventElapsedTime(&timings_.total_ms, ev_[0], ev_[6]);
+        const hipEvent_t* ev = ev_[k];
+        (void)hipEventElapsedTime(&timings_.prep_ms, ev[1], ev[2]);
+        (void)hipEventElapsedTime(&timings_.map32_ms, ev[2], ev[3]);
+        (void)hipEventElapsedTime(&timings_.map64_ms, ev[3], ev[4]);
+        (void)hipEventElapsedTime(&timings_.map_ms, ev[2], ev[4]);
+        (void)hipEventElapsedTime(&timings_.finalize_ms, ev[4], ev[5]);
+        (void)hipEventElapsedTime(&timings_.total_ms, ev[0], ev[6]);
     }
 }
 

@@ -5,6 +5,7 @@
 #include <hip/hip_runtime.h>
 
 #include <cstdint>
+#include <memory>
 #include <stdexcept>
 #include <string>
 #include <vector>
@@ -79,6 +80,11 @@ struct EngineOptions {
     int persistent_grid = 0;   // 0: grid = tile count
     bool use_graph = false;
     bool profile = false;
+    // submit_host: page-locked staging ring (chunk size, slots) and the CPU
+    // threads that copy pageable planes into it
+    int64_t host_chunk_bytes = 8ll << 20;
+    int host_slots = 4;
+    int host_copy_threads = 1;
 };
 
 struct Timings {
@@ -150,6 +156,8 @@ void launch_batch(const DescView& d, const PlanInfo& P, const LaunchShape& S, vo
 
 Taps make_taps();
 
+class HostStager;
+
 class Engine {
 public:
     explicit Engine(const EngineOptions& opt);
@@ -157,10 +165,25 @@ public:
     Engine(const Engine&) = delete;
     Engine& operator=(const Engine&) = delete;
 
-    // Plan and enqueue one batch (asynchronous).
+    // Batches in flight at once: submitted and not yet waited for.
+    static constexpr int kDepth = 2;
+
+    // Plan and enqueue one batch of device-resident pairs (asynchronous).
+    // Up to kDepth batches may be outstanding; a further submit is an error.
     void submit(const PairInput* pairs, int n);
-    // Wait for the last submitted batch and copy results out.
+    // The same for pairs whose plane pointers are host memory. The planes
+    // are uploaded on the engine's copy stream into one of kDepth device
+    // input buffers (each plane as one block of (rows - 1) strides plus a
+    // row, keeping the caller's stride): through the page-locked staging ring
+    // for pageable memory, by one DMA each for page-locked memory. The
+    // kernels wait for the upload, so the upload of batch k+1 runs while
+    // batch k's kernels do. Returns once the caller's memory may be reused.
+    void submit_host(const PairInput* pairs, int n);
+    // Staging-ring shape of submit_host; only before its first call.
+    void set_host_staging(int64_t chunk_bytes, int slots, int copy_threads);
+    // Wait for the oldest outstanding batch and copy its results out.
     void wait(double* scores, double* features);
+    int outstanding() const { return (int)(head_ - tail_); }
 
     const EngineOptions& options() const { return opt_; }
     hipStream_t stream() const { return stream_; }
@@ -172,11 +195,12 @@ public:
     uint64_t last_map_bytes() const { return plan_.map_bytes; }
     uint64_t last_map_bytes_class(int c) const { return plan_.cls_bytes[c]; }
     uint64_t last_prep_bytes() const { return plan_.prep_bytes; }
+    uint64_t host_pageable_bytes() const;
+    uint64_t host_direct_bytes() const;
     int num_cus() const { return num_cus_; }
 
 private:
-    void plan(const PairInput* pairs, int n);
-    void enqueue_body();
+    void enqueue_body(int slot);
     void release();
 
     EngineOptions opt_;
@@ -185,20 +209,29 @@ private:
     Taps taps_{};
 
     DescLayout desc_;
-    uint8_t* desc_h_ = nullptr;  // pinned
+    uint8_t* desc_h_[kDepth] = {};  // pinned, one per outstanding batch
     uint8_t* desc_d_ = nullptr;
     void* arena_d_ = nullptr;
     int64_t arena_bytes_ = 0;
     double* partials_d_ = nullptr;
     int64_t max_map_tiles_ = 0;
     double* out_d_ = nullptr;   // scores [max_pairs] then features [max_pairs*108]
-    double* out_h_ = nullptr;   // pinned
-    hipEvent_t ev_[7] = {};
-    hipGraph_t graph_ = nullptr;
-    hipGraphExec_t graph_exec_ = nullptr;
+    double* out_h_[kDepth] = {};  // pinned, same layout
+    hipEvent_t ev_[kDepth][7] = {};
+    hipEvent_t done_[kDepth] = {};
+    hipGraph_t graph_[kDepth] = {};
+    hipGraphExec_t graph_exec_[kDepth] = {};
+    int npairs_[kDepth] = {};
+
+    // host input (created on the first submit_host)
+    hipStream_t copy_ = nullptr;
+    std::unique_ptr<HostStager> stager_;
+    uint8_t* in_d_[kDepth] = {};
+    size_t in_cap_[kDepth] = {};
+    hipEvent_t in_ready_[kDepth] = {}, in_free_[kDepth] = {};
 
     PlanInfo plan_;  // the last planned batch
-    bool pending_ = false;
+    uint64_t head_ = 0, tail_ = 0;  // batches submitted, waited for
     Timings timings_;
 };
 

@@ -98,12 +98,15 @@ FrameHandle::FrameHandle(int w, int h, int device) : width(w), height(h), device
     o.map_th = default_map_th(w, h);
     o.arena_bytes = pair_bytes(w, h, false);
     o.max_map_tiles = map_tiles(w, h, 32);
+    // one 1080p float plane is 8.3 MB: 4 MiB chunks keep two of them in
+    // flight per plane while the next one is copied in
+    o.host_chunk_bytes = 4ll << 20;
+    o.host_slots = 4;
+    o.host_copy_threads = 1;  // frame-level parallelism comes from the callers' threads
     engine = std::make_unique<Engine>(o);
 }
 
-FrameHandle::~FrameHandle() {
-    if (staging) (void)hipFree(staging);
-}
+FrameHandle::~FrameHandle() = default;
 
 // A size-independent default keeps a pair's result independent of the batch
 // it is in (the tile decomposition, and so the per-tile centring, is fixed):
@@ -117,38 +120,18 @@ double FrameHandle::compute(int fmt, const uint8_t* const p1[3], const uint8_t* 
     const int bps = fmt == IN_RGB_F32 ? 4 : (fmt == IN_RGB_U8 ? 1 : 2);
     const int64_t row = (int64_t)width * bps;
     if (s1 < row || s2 < row) VSHIP_THROW(E_BadPointer);
-    // each plane is copied as one contiguous block of (height - 1) strides
-    // plus one row, keeping the caller's stride: a 1-D copy from pageable
-    // memory runs at copy-engine speed, a 2-D one row by row
-    const size_t blk1 = (size_t)s1 * (height - 1) + row, blk2 = (size_t)s2 * (height - 1) + row;
-    const size_t a1 = (blk1 + 255) / 256 * 256, a2 = (blk2 + 255) / 256 * 256;
-    const size_t need = 3 * (a1 + a2);
-    if (need > staging_bytes) {
-        if (staging) (void)hipFree(staging);
-        staging = nullptr;
-        staging_bytes = 0;
-        HIP_OR_THROW(hipSetDevice(device), E_BadDeviceArgument);
-        HIP_OR_THROW(hipMalloc(reinterpret_cast<void**>(&staging), need), E_OutOfVRAM);
-        staging_bytes = need;
-    }
-    hipStream_t st = engine->stream();
     PairInput in{};
-    for (int im = 0; im < 2; im++) {
-        const uint8_t* const* src = im == 0 ? p1 : p2;
-        const int64_t sp = im == 0 ? s1 : s2;
-        const size_t blk = im == 0 ? blk1 : blk2;
-        for (int pl = 0; pl < 3; pl++) {
-            uint8_t* dst = staging + (im == 0 ? pl * a1 : 3 * a1 + pl * a2);
-            HIP_OR_THROW(hipMemcpyAsync(dst, src[pl], blk, hipMemcpyHostToDevice, st), E_BadPointer);
-            in.src[im][pl] = dst;
-            in.stride[im][pl] = sp;
-        }
+    for (int pl = 0; pl < 3; pl++) {
+        in.src[0][pl] = p1[pl];
+        in.src[1][pl] = p2[pl];
+        in.stride[0][pl] = s1;
+        in.stride[1][pl] = s2;
     }
     in.w = width;
     in.h = height;
     in.fmt = fmt;
     double score = 0.0;
-    engine->submit(&in, 1);
+    engine->submit_host(&in, 1);
     engine->wait(&score, nullptr);
     return score;
 }
@@ -158,6 +141,7 @@ double FrameHandle::compute(int fmt, const uint8_t* const p1[3], const uint8_t* 
 // ---------------------------------------------------------------------------
 ButterHandle::ButterHandle(int w, int h, int qnorm, float intensity, int dev) : width(w), height(h), device(dev) {
     engine = std::make_unique<bt::Engine>(w, h, 1, qnorm, intensity, dev);
+    stager = std::make_unique<HostStager>(dev, (size_t)4 << 20, 4, 1);
     HIP_OR_THROW(hipMalloc(reinterpret_cast<void**>(&map), sizeof(float) * (size_t)w * h), E_OutOfVRAM);
 }
 
@@ -194,7 +178,7 @@ void ButterHandle::compute(int fmt, const uint8_t* const p1[3], const uint8_t* c
         const size_t blk = im == 0 ? blk1 : blk2;
         for (int pl = 0; pl < 3; pl++) {
             uint8_t* dstp = staging + (im == 0 ? pl * a1 : 3 * a1 + pl * a2);
-            HIP_OR_THROW(hipMemcpyAsync(dstp, src[pl], blk, hipMemcpyHostToDevice, st), E_BadPointer);
+            stager->upload(dstp, src[pl], blk, st);
             d.src[im][pl] = dstp;
             d.stride[im][pl] = sp;
         }
@@ -202,12 +186,7 @@ void ButterHandle::compute(int fmt, const uint8_t* const p1[3], const uint8_t* c
     d.map = dst ? map : nullptr;
     d.map_stride = (int64_t)width * 4;
     engine->run(fmt, &d, 1, norms);
-    if (dst) {
-        HIP_OR_THROW(hipMemcpy2DAsync(dst, dst_stride, map, (size_t)width * 4, (size_t)width * 4, height,
-                                      hipMemcpyDeviceToHost, st),
-                     E_BadPointer);
-        HIP_OR_THROW(hipStreamSynchronize(st), E_BadPointer);
-    }
+    if (dst) stager->download_rows(dst, dst_stride, map, (int64_t)width * 4, (size_t)width * 4, height, st);
 }
 
 template <typename T>
@@ -632,6 +611,7 @@ Vship_Exception Vship_SSIMU2BatchInit(Vship_SSIMU2BatchHandler* handler, const V
         o.max_map_tiles = (int64_t)cfg->max_pairs * map_tiles(cfg->max_width, cfg->max_height, 32);
         o.use_graph = cfg->use_graph != 0;  // a captured graph runs a fixed grid (4 x CUs when 0)
         o.profile = cfg->profile != 0;
+        o.host_copy_threads = 4;
         auto b = std::make_shared<BatchHandle>();
         b->engine = std::make_unique<Engine>(o);
         batch_table().set(id, std::move(b));
@@ -658,6 +638,32 @@ Vship_Exception Vship_SSIMU2BatchSubmit(Vship_SSIMU2BatchHandler handler, const 
     });
 }
 
+Vship_Exception Vship_SSIMU2BatchSubmitHost(Vship_SSIMU2BatchHandler handler, const Vship_SSIMU2Pair* pairs,
+                                            int npairs) {
+    std::shared_ptr<BatchHandle> b = batch_table().get(handler.id);
+    if (!b) return Vship_BadHandler;
+    if (!pairs || npairs < 1) return Vship_BadPointer;
+    BusyGuard g(b->busy);
+    if (!g.ok) return Vship_BadHandler;
+    return guarded([&] {
+        std::vector<PairInput> in(npairs);
+        for (int i = 0; i < npairs; i++) in[i] = to_pair_input(pairs[i]);
+        b->engine->submit_host(in.data(), npairs);
+    });
+}
+
+Vship_Exception Vship_SSIMU2BatchHostStaging(Vship_SSIMU2BatchHandler handler, uint64_t chunk_bytes, int slots,
+                                             int copy_threads) {
+    std::shared_ptr<BatchHandle> b = batch_table().get(handler.id);
+    if (!b) return Vship_BadHandler;
+    if (chunk_bytes < 4096 || chunk_bytes > (1ull << 30) || slots < 2 || slots > 64 || copy_threads < 1 ||
+        copy_threads > 64)
+        return Vship_BadPointer;
+    BusyGuard g(b->busy);
+    if (!g.ok) return Vship_BadHandler;
+    return guarded([&] { b->engine->set_host_staging((int64_t)chunk_bytes, slots, copy_threads); });
+}
+
 Vship_Exception Vship_SSIMU2BatchWait(Vship_SSIMU2BatchHandler handler, double* scores, double* features) {
     std::shared_ptr<BatchHandle> b = batch_table().get(handler.id);
     if (!b) return Vship_BadHandler;
@@ -668,6 +674,14 @@ Vship_Exception Vship_SSIMU2BatchWait(Vship_SSIMU2BatchHandler handler, double* 
 
 Vship_Exception Vship_SSIMU2BatchCompute(Vship_SSIMU2BatchHandler handler, const Vship_SSIMU2Pair* pairs,
                                          int npairs, double* scores, double* features) {
+    {
+        // Wait returns the oldest outstanding batch: with one already in
+        // flight, Compute would hand back that batch's results
+        std::shared_ptr<BatchHandle> b = batch_table().get(handler.id);
+        if (!b) return Vship_BadHandler;
+        BusyGuard g(b->busy);
+        if (!g.ok || b->engine->outstanding() != 0) return Vship_BadHandler;
+    }
     const Vship_Exception e = Vship_SSIMU2BatchSubmit(handler, pairs, npairs);
     if (e != Vship_NoError) return e;
     return Vship_SSIMU2BatchWait(handler, scores, features);

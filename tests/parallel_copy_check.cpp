@@ -1,8 +1,10 @@
 // CPU check of vship_amd/csrc/parallel_copy.hpp: the parts of every split
 // tile [0, bytes) exactly, and ParallelCopy reproduces memcpy for sizes
-// around the part boundaries and the single-thread cut-off, for 1..9 threads.
+// around the part boundaries and the single-thread cut-off, for 1..9 threads,
+// and serial copies for random segment lists.
 // Prints "ok" and exits 0 on success.
 #include <cstdio>
+#include <cstring>
 #include <random>
 #include <vector>
 
@@ -50,6 +52,35 @@ int main() {
                     std::printf("overrun: threads %d bytes %zu at %zu\n", threads, bytes, i);
                     return 1;
                 }
+        }
+    }
+    // gathered copies: random segment lists (empty segments included) into
+    // disjoint destinations with gaps that must stay untouched
+    for (int threads = 1; threads <= 9; threads += 2) {
+        vship::ParallelCopy pc(threads);
+        for (int trial = 0; trial < 20; trial++) {
+            std::vector<vship::CopySeg> segs;
+            std::vector<std::pair<size_t, size_t>> ranges;  // (dst offset, bytes)
+            size_t dpos = 0;
+            const int n = 1 + (int)(rng() % 12);
+            for (int i = 0; i < n; i++) {
+                const size_t bytes = trial % 5 == 0 ? rng() % 5000 : rng() % ((size_t)3 << 20);
+                dpos += rng() % 300;  // gap
+                const size_t soff = rng() % (src.size() - bytes);
+                if (dpos + bytes > dst.size()) break;
+                segs.push_back({dst.data() + dpos, src.data() + soff, bytes});
+                ranges.push_back({dpos, soff});
+                dpos += bytes;
+            }
+            std::fill(dst.begin(), dst.end(), 0x5A);
+            pc.copy(segs.data(), (int)segs.size());
+            std::vector<uint8_t> want(dst.size(), 0x5A);
+            for (size_t i = 0; i < segs.size(); i++)
+                std::memcpy(want.data() + ranges[i].first, src.data() + ranges[i].second, segs[i].bytes);
+            if (std::memcmp(want.data(), dst.data(), dst.size()) != 0) {
+                std::printf("segments mismatch: threads %d trial %d\n", threads, trial);
+                return 1;
+            }
         }
     }
     std::printf("ok\n");

@@ -149,3 +149,52 @@ def test_per_frame_handle_pageable_and_pinned_planes_agree():
         assert s_pageable[0] == s_pageable[1] == s_pinned[0] == s_pinned[1]
     finally:
         pin.free()
+
+
+def test_mixed_page_locked_and_pageable_pieces():
+    """One batch whose planes mix page-locked planes of one allocation (some at
+    256-byte multiples, which one DMA carries together, some at 64-byte
+    offsets, which it cannot), pageable planes, and a pair whose reference and
+    distorted planes are the same page-locked memory."""
+    w, h = 720, 405
+    pairs = yuv_pairs(w, h, 8, 4, 67)
+    say(f"case: 5 pairs {w}x{h} YUV420P8, page-locked / pageable / odd-offset / aliased planes in one batch")
+    dp = api.DevicePairs(pairs + [(pairs[0][0], pairs[0][0])], api.PF_YUV420_U8, 8, api.MATRIX_BT709)
+    sizes = [p.nbytes for r, d in pairs[:2] for p in list(r) + list(d)]
+    pin = api.PinnedBuffer(sum(sizes) + 64 * 12 + 4096)
+    pageable = api.HostPairs(pairs, api.PF_YUV420_U8, 8, api.MATRIX_BT709)
+    try:
+        descs = (api.Pair * 5)()
+        for i in range(4):
+            descs[i] = pageable.descs[i]
+        # pairs 0 and 1 in page-locked memory: pair 0's planes at 256-byte
+        # multiples, pair 1's at 64-byte offsets
+        off = 0
+        for i in (0, 1):
+            planes = list(pairs[i][0]) + list(pairs[i][1])
+            ptrs = []
+            for p in planes:
+                step = 256 if i == 0 else 64
+                off = (off + step - 1) // step * step + (64 if i == 1 else 0)
+                v = pin.array[off:off + p.nbytes].view(p.dtype).reshape(p.shape)
+                v[...] = p
+                ptrs.append(v.ctypes.data)
+                off += p.nbytes
+            for j in range(3):
+                descs[i].ref[j] = ptrs[j]
+                descs[i].dist[j] = ptrs[3 + j]
+        # pair 4: pair 0's page-locked reference planes as both images
+        descs[4] = descs[0]
+        for j in range(3):
+            descs[4].dist[j] = descs[0].ref[j]
+        with api.SSIMU2Batch(5, w, h) as b:
+            want = b.compute(dp.descs)
+            for rep in range(2):
+                b.submit_host(descs)
+                got = b.wait()
+                assert np.array_equal(got, want), (rep, got, want)
+        assert want[4] == 100.0
+    finally:
+        pageable.free()
+        pin.free()
+        dp.free()

@@ -455,9 +455,9 @@ void Engine::submit_host(const PairInput* pairs, int n) {
     if (outstanding() >= kDepth) VSHIP_THROW(E_BadHandler);
     if (!pairs || n < 1 || n > opt_.max_pairs) VSHIP_THROW(E_BadPointer);
     HIP_OR_THROW(hipSetDevice(opt_.device), E_BadDeviceArgument);
-    // one block per plane: (rows - 1) strides plus one row, 256-byte aligned
-    std::vector<size_t> blk((size_t)n * 6), off((size_t)n * 6);
-    size_t total = 0;
+    // one block per plane: (rows - 1) strides plus one row
+    std::vector<const uint8_t*> src((size_t)n * 6);
+    std::vector<size_t> blk((size_t)n * 6);
     for (int i = 0; i < n; i++) {
         const PairInput& p = pairs[i];
         if (p.w < 1 || p.h < 1 || p.fmt < 0 || p.fmt > IN_YUV410_U16) VSHIP_THROW(E_BadPointer);
@@ -468,9 +468,8 @@ void Engine::submit_host(const PairInput* pairs, int n) {
                 const int64_t st = p.stride[im][pl];
                 if (!p.src[im][pl] || st < row) VSHIP_THROW(E_BadPointer);
                 const size_t j = (size_t)i * 6 + im * 3 + pl;
+                src[j] = p.src[im][pl];
                 blk[j] = (size_t)st * (plane_h(p, pl) - 1) + (size_t)row;
-                off[j] = total;
-                total += (blk[j] + 255) / 256 * 256;
             }
     }
     if (!copy_) {
@@ -482,26 +481,25 @@ void Engine::submit_host(const PairInput* pairs, int n) {
         stager_ = std::make_unique<HostStager>(opt_.device, (size_t)opt_.host_chunk_bytes, opt_.host_slots,
                                                opt_.host_copy_threads);
     }
+    // device layout: page-locked planes of one allocation keep their host
+    // spacing (one DMA for the lot), pageable ones are packed
+    const UploadPlan U = plan_upload(src.data(), blk.data(), n * 6);
     const int k = (int)(head_ % kDepth);
-    if (total > in_cap_[k]) {
+    if (U.total > in_cap_[k]) {
         // the batch that last read this buffer has been waited for
         if (in_d_[k]) (void)hipFree(in_d_[k]);
         in_d_[k] = nullptr;
         in_cap_[k] = 0;
-        HIP_OR_THROW(hipMalloc(reinterpret_cast<void**>(&in_d_[k]), total), E_OutOfVRAM);
-        in_cap_[k] = total;
+        HIP_OR_THROW(hipMalloc(reinterpret_cast<void**>(&in_d_[k]), U.total), E_OutOfVRAM);
+        in_cap_[k] = U.total;
     }
     // the upload into buffer k starts after the kernels that read it last
     HIP_OR_THROW(hipStreamWaitEvent(copy_, in_free_[k], 0), E_BadDeviceCode);
+    stager_->upload(in_d_[k], U, copy_);
     std::vector<PairInput> dev(pairs, pairs + n);
     for (int i = 0; i < n; i++)
         for (int im = 0; im < 2; im++)
-            for (int pl = 0; pl < 3; pl++) {
-                const size_t j = (size_t)i * 6 + im * 3 + pl;
-                uint8_t* dst = in_d_[k] + off[j];
-                stager_->upload(dst, pairs[i].src[im][pl], blk[j], copy_);
-                dev[i].src[im][pl] = dst;
-            }
+            for (int pl = 0; pl < 3; pl++) dev[i].src[im][pl] = in_d_[k] + U.off[(size_t)i * 6 + im * 3 + pl];
     HIP_OR_THROW(hipEventRecord(in_ready_[k], copy_), E_BadDeviceCode);
     HIP_OR_THROW(hipStreamWaitEvent(stream_, in_ready_[k], 0), E_BadDeviceCode);
     submit(dev.data(), n);

@@ -8,18 +8,61 @@
 
 namespace vship {
 
-bool host_page_locked(const void* p, size_t bytes) {
-    if (!p || bytes == 0) return false;
+HostMem host_memory_kind(const void* p, size_t bytes) {
+    if (!p || bytes == 0) return HostMem::Pageable;
     const uint8_t* b = static_cast<const uint8_t*>(p);
+    bool locked = true;
     for (const uint8_t* q : {b, b + bytes - 1}) {
         hipPointerAttribute_t attr;
         if (hipPointerGetAttributes(&attr, q) != hipSuccess) {
             (void)hipGetLastError();  // pageable memory: the query fails and leaves its error behind
-            return false;
+            locked = false;
+            continue;
         }
-        if (attr.type != hipMemoryTypeHost) return false;
+        if (q == b && attr.type == hipMemoryTypeDevice) return HostMem::Device;
+        if (attr.type != hipMemoryTypeHost) locked = false;
     }
-    return true;
+    return locked ? HostMem::PageLocked : HostMem::Pageable;
+}
+
+UploadPlan plan_upload(const uint8_t* const* src, const size_t* bytes, int n) {
+    constexpr size_t kAlign = 256, kMaxGap = 64 << 10;
+    UploadPlan P;
+    P.off.resize(n);
+    size_t total = 0;
+    for (int i = 0; i < n;) {
+        if (!src[i] || bytes[i] == 0) VSHIP_THROW(E_BadPointer);
+        const HostMem kind = host_memory_kind(src[i], bytes[i]);
+        if (kind == HostMem::Device) VSHIP_THROW(E_BadPointer);
+        UploadPlan::Run r{src[i], bytes[i], (total + kAlign - 1) / kAlign * kAlign, kind == HostMem::PageLocked};
+        P.off[i] = r.off;
+        int j = i + 1;
+        if (r.locked) {
+            hipDeviceptr_t base = nullptr;
+            size_t size = 0;
+            if (hipMemGetAddressRange(&base, &size, (hipDeviceptr_t)src[i]) != hipSuccess) {
+                (void)hipGetLastError();
+                size = 0;
+            }
+            const uint8_t* lo = static_cast<const uint8_t*>(base);
+            const uint8_t* hi = lo + size;
+            if (size == 0 || r.src < lo || r.src + r.bytes > hi) hi = lo = nullptr;
+            for (; lo && j < n; j++) {
+                const uint8_t* s = src[j];
+                const uint8_t* prev_end = r.src + r.bytes;
+                if (!s || bytes[j] == 0 || s < prev_end || (size_t)(s - prev_end) > kMaxGap ||
+                    (size_t)(s - r.src) % kAlign != 0 || s + bytes[j] > hi)
+                    break;
+                P.off[j] = r.off + (size_t)(s - r.src);
+                r.bytes = (size_t)(s + bytes[j] - r.src);
+            }
+        }
+        P.runs.push_back(r);
+        total = r.off + r.bytes;
+        i = j;
+    }
+    P.total = total;
+    return P;
 }
 
 HostStager::HostStager(int device, size_t chunk_bytes, int slots, int threads) {
@@ -59,27 +102,55 @@ void HostStager::release() {
 void HostStager::upload(void* dst, const void* src, size_t bytes, hipStream_t st) {
     if (bytes == 0) return;
     if (!dst || !src) VSHIP_THROW(E_BadPointer);
-    if (host_page_locked(src, bytes)) {
-        HIP_OR_THROW(hipMemcpyAsync(dst, src, bytes, hipMemcpyHostToDevice, st), E_BadPointer);
-        direct_bytes_ += bytes;
-        return;
-    }
-    uint8_t* d = static_cast<uint8_t*>(dst);
     const uint8_t* s = static_cast<const uint8_t*>(src);
-    for (size_t off = 0; off < bytes; off += chunk_) {
-        const size_t n = std::min(chunk_, bytes - off);
-        const int k = next_;
-        next_ = (next_ + 1) % (int)ring_.size();
-        // the slot's previous DMA has to have read it before it is refilled
-        HIP_OR_THROW(hipEventSynchronize(ev_[k]), E_BadDeviceCode);
+    const UploadPlan P = plan_upload(&s, &bytes, 1);
+    upload(static_cast<uint8_t*>(dst), P, st);
+}
+
+void HostStager::upload(uint8_t* dst_base, const UploadPlan& plan, hipStream_t st) {
+    if (!dst_base) VSHIP_THROW(E_BadPointer);
+    // the open chunk: slot k holds device bytes [c0, c0 + fill)
+    int k = -1;
+    size_t c0 = 0, fill = 0;
+    std::vector<CopySeg> segs;
+    auto flush = [&] {
+        if (k < 0) return;
         if (pool_)
-            pool_->copy(ring_[k], s + off, n);
+            pool_->copy(segs.data(), (int)segs.size());
         else
-            std::memcpy(ring_[k], s + off, n);
-        HIP_OR_THROW(hipMemcpyAsync(d + off, ring_[k], n, hipMemcpyHostToDevice, st), E_BadPointer);
+            copy_seg_range(segs.data(), (int)segs.size(), 0, SIZE_MAX);
+        HIP_OR_THROW(hipMemcpyAsync(dst_base + c0, ring_[k], fill, hipMemcpyHostToDevice, st), E_BadPointer);
         HIP_OR_THROW(hipEventRecord(ev_[k], st), E_BadDeviceCode);
+        k = -1;
+        segs.clear();
+    };
+    for (const UploadPlan::Run& r : plan.runs) {
+        if (r.locked) {
+            flush();
+            HIP_OR_THROW(hipMemcpyAsync(dst_base + r.off, r.src, r.bytes, hipMemcpyHostToDevice, st), E_BadPointer);
+            direct_bytes_ += r.bytes;
+            continue;
+        }
+        for (size_t done = 0; done < r.bytes;) {
+            const size_t dpos = r.off + done;
+            if (k >= 0 && dpos >= c0 + chunk_) flush();
+            if (k < 0) {
+                k = next_;
+                next_ = (next_ + 1) % (int)ring_.size();
+                // the slot's previous DMA has to have read it before it is refilled
+                HIP_OR_THROW(hipEventSynchronize(ev_[k]), E_BadDeviceCode);
+                c0 = dpos;
+                fill = 0;
+            }
+            const size_t n = std::min(r.bytes - done, c0 + chunk_ - dpos);
+            segs.push_back({ring_[k] + (dpos - c0), r.src + done, n});
+            fill = dpos - c0 + n;
+            done += n;
+            if (fill == chunk_) flush();
+        }
+        pageable_bytes_ += r.bytes;
     }
-    pageable_bytes_ += bytes;
+    flush();
 }
 
 void HostStager::download_rows(void* dst, int64_t dst_stride, const void* src, int64_t src_stride,
@@ -89,7 +160,9 @@ void HostStager::download_rows(void* dst, int64_t dst_stride, const void* src, i
         VSHIP_THROW(E_BadPointer);
     uint8_t* d = static_cast<uint8_t*>(dst);
     const uint8_t* s = static_cast<const uint8_t*>(src);
-    if (host_page_locked(d, (size_t)dst_stride * (rows - 1) + row_bytes)) {
+    const HostMem kind = host_memory_kind(d, (size_t)dst_stride * (rows - 1) + row_bytes);
+    if (kind == HostMem::Device) VSHIP_THROW(E_BadPointer);
+    if (kind == HostMem::PageLocked) {
         HIP_OR_THROW(hipMemcpy2DAsync(d, dst_stride, s, src_stride, row_bytes, rows, hipMemcpyDeviceToHost, st),
                      E_BadPointer);
         HIP_OR_THROW(hipStreamSynchronize(st), E_BadPointer);

@@ -22,9 +22,34 @@
 
 namespace vship {
 
-// True when [p, p + bytes) starts and ends in page-locked host memory.
-// Clears the runtime's last-error state that a query on pageable memory sets.
-bool host_page_locked(const void* p, size_t bytes);
+enum class HostMem { Pageable, PageLocked, Device };
+
+// What [p, p + bytes) is: page-locked when it starts and ends in page-locked
+// host memory, Device when it starts in device memory (an error for an
+// upload), pageable otherwise. Clears the runtime's last-error state that a
+// query on pageable memory sets.
+HostMem host_memory_kind(const void* p, size_t bytes);
+
+// The device layout of a list of host pieces, and the runs one upload moves.
+struct UploadPlan {
+    struct Run {
+        const uint8_t* src;
+        size_t bytes;
+        size_t off;   // device offset
+        bool locked;  // page-locked: one DMA straight from src
+    };
+    std::vector<size_t> off;  // device offset of each piece
+    std::vector<Run> runs;    // in device order
+    size_t total = 0;         // device bytes
+};
+
+// Lays out pieces (src[i], bytes[i] > 0) in one device buffer, in order, each
+// at a 256-byte-aligned offset. A page-locked piece that follows a page-locked
+// piece of the same allocation, at a host distance that is a multiple of 256
+// and with a gap of at most 64 KiB, keeps that host distance on the device,
+// so that one DMA carries both (and the gap). Throws E_BadPointer for a piece
+// in device memory.
+UploadPlan plan_upload(const uint8_t* const* src, const size_t* bytes, int n);
 
 class HostStager {
 public:
@@ -38,6 +63,12 @@ public:
     // page-locked source, after the last chunk has been copied into the ring
     // otherwise. The DMA itself completes in stream order.
     void upload(void* dst, const void* src, size_t bytes, hipStream_t st);
+    // Uploads the runs of `plan` to dst_base + their offsets, in stream order
+    // on `st`: page-locked runs by one DMA each, pageable runs packed in
+    // device order into the ring's chunks (a chunk carries as many runs as
+    // fit, copied by the CPU threads, then one DMA). Returns once the callers'
+    // memory may be reused.
+    void upload(uint8_t* dst_base, const UploadPlan& plan, hipStream_t st);
     // Copies `rows` rows of `row_bytes` bytes from device `src` (pitch
     // `src_stride`) to host `dst` (pitch `dst_stride`) after the work already
     // on `st`, and returns when they have arrived. A page-locked destination

@@ -1,6 +1,7 @@
 // A fork-join group of worker threads for large memcpy calls (no HIP
-// dependency): copy() splits the bytes into equal parts cut at 4 KiB
-// boundaries, and the calling thread copies the first part itself.
+// dependency): copy() splits the bytes of one range, or of a list of
+// segments, into equal parts cut at 4 KiB boundaries, and the calling thread
+// copies the first part itself.
 #pragma once
 #include <algorithm>
 #include <condition_variable>
@@ -19,6 +20,24 @@ inline void copy_part_range(size_t bytes, int idx, int parts, size_t* begin, siz
     const size_t per = ((bytes + parts - 1) / parts + 4095) / 4096 * 4096;
     *begin = std::min(bytes, per * idx);
     *end = std::min(bytes, *begin + per);
+}
+
+// One contiguous piece of a gathered copy.
+struct CopySeg {
+    uint8_t* dst;
+    const uint8_t* src;
+    size_t bytes;
+};
+
+// Copies the bytes [b, e) of the concatenation of `segs`.
+inline void copy_seg_range(const CopySeg* segs, int n, size_t b, size_t e) {
+    size_t s0 = 0;
+    for (int i = 0; i < n && s0 < e; i++) {
+        const size_t s1 = s0 + segs[i].bytes;
+        const size_t lo = std::max(b, s0), hi = std::min(e, s1);
+        if (hi > lo) std::memcpy(segs[i].dst + (lo - s0), segs[i].src + (lo - s0), hi - lo);
+        s0 = s1;
+    }
 }
 
 class ParallelCopy {
@@ -40,30 +59,38 @@ public:
     int threads() const { return nthreads_; }
 
     void copy(void* dst, const void* src, size_t bytes) {
+        const CopySeg s{static_cast<uint8_t*>(dst), static_cast<const uint8_t*>(src), bytes};
+        copy(&s, 1);
+    }
+
+    // Copies every segment; the threads split the segments' total bytes.
+    void copy(const CopySeg* segs, int n) {
+        size_t total = 0;
+        for (int i = 0; i < n; i++) total += segs[i].bytes;
         // below ~1 MiB per extra thread the wake-ups cost more than they save
-        if (workers_.empty() || bytes < ((size_t)1 << 20) * workers_.size()) {
-            std::memcpy(dst, src, bytes);
+        if (workers_.empty() || total < ((size_t)1 << 20) * workers_.size()) {
+            copy_seg_range(segs, n, 0, total);
             return;
         }
         {
             std::lock_guard<std::mutex> lk(mu_);
-            dst_ = static_cast<uint8_t*>(dst);
-            src_ = static_cast<const uint8_t*>(src);
-            bytes_ = bytes;
+            segs_ = segs;
+            nseg_ = n;
+            total_ = total;
             remaining_ = (int)workers_.size();
             gen_++;
         }
         go_.notify_all();
-        part(static_cast<uint8_t*>(dst), static_cast<const uint8_t*>(src), bytes, 0);
+        part(segs, n, total, 0);
         std::unique_lock<std::mutex> lk(mu_);
         done_.wait(lk, [&] { return remaining_ == 0; });
     }
 
 private:
-    void part(uint8_t* d, const uint8_t* s, size_t bytes, int idx) const {
+    void part(const CopySeg* segs, int n, size_t total, int idx) const {
         size_t b, e;
-        copy_part_range(bytes, idx, nthreads_, &b, &e);
-        if (e > b) std::memcpy(d + b, s + b, e - b);
+        copy_part_range(total, idx, nthreads_, &b, &e);
+        if (e > b) copy_seg_range(segs, n, b, e);
     }
 
     // Each worker takes every generation: copy() starts the next one only
@@ -71,19 +98,19 @@ private:
     void worker(int idx) {
         uint64_t seen = 0;
         for (;;) {
-            uint8_t* d;
-            const uint8_t* s;
-            size_t n;
+            const CopySeg* segs;
+            int n;
+            size_t total;
             {
                 std::unique_lock<std::mutex> lk(mu_);
                 go_.wait(lk, [&] { return stop_ || gen_ != seen; });
                 if (stop_) return;
                 seen = gen_;
-                d = dst_;
-                s = src_;
-                n = bytes_;
+                segs = segs_;
+                n = nseg_;
+                total = total_;
             }
-            part(d, s, n, idx);
+            part(segs, n, total, idx);
             {
                 std::lock_guard<std::mutex> lk(mu_);
                 if (--remaining_ == 0) done_.notify_one();
@@ -98,9 +125,9 @@ private:
     uint64_t gen_ = 0;
     int remaining_ = 0;
     bool stop_ = false;
-    uint8_t* dst_ = nullptr;
-    const uint8_t* src_ = nullptr;
-    size_t bytes_ = 0;
+    const CopySeg* segs_ = nullptr;
+    int nseg_ = 0;
+    size_t total_ = 0;
 };
 
 }  // namespace vship

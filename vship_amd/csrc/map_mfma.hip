@@ -20,19 +20,24 @@
 //     last 8); the six products above 2^-24 are accumulated (hh + hm + mh +
 //     hl + mm + lh). FP32 accuracy, FP32 range.
 //   SplitF16x2 (8- to 16-bit RGB and YUV input): operands are scaled by a
-//     power of two (a, b: 2^11, (a - b)^2: 2^4, ab: 2^6, taps: 2^14) and
-//     split into two fp16 parts h + l (22 significant bits); the products
-//     hh + hl + lh are accumulated, each exact (the lo x lo product, below
-//     the split's own error, is dropped). The taps are the values the fp16 pair
-//     represents exactly (within 2^-22 of the FP32 taps), and the
-//     renormalisation sums use the same values. The scales keep every value
-//     inside the fp16 range for any input of these formats: their R'G'B' lies
-//     in [-0.97, 1.98] (limited-range YUV with samples clamped to the
-//     declared depth), so XYB lies in [-11.7, 12.5] x [-0.15, 1.58] x
-//     [-1.18, 2.28], |a|, |b| <= 24.2, |a - b| <= 48.4 and |ab| <= 586. A CPU
-//     emulation of this arithmetic on real frames puts the score within
-//     1e-5 of the FP64 maps (typically 1e-6), at about half the splitting
-//     work of the exact split and two thirds of its matrix products.
+//     power of two (a, b and a - b: 2^2, so (a - b)^2 and ab: 2^4; taps:
+//     2^14) and split into two fp16 parts h + l; the products hh + hl + lh
+//     are accumulated, each exact (the lo x lo product, below the split's own
+//     error, is dropped). A value whose low part is a normal fp16 number keeps
+//     22 significant bits; a smaller one keeps an absolute error of at most
+//     2^-25 at its scale (under 1e-8 in XYB units, far below the SSIM
+//     constants 0.0009 these moments meet). The scales are the smallest
+//     powers of two that need no multiply of their own (a and b come from
+//     one FMA each, (a - b)^2 and ab are products of the scaled factors) and
+//     they keep every value inside the fp16 range for any input of these
+//     formats: their R'G'B' lies in [-0.97, 1.98] (limited-range YUV with
+//     samples clamped to the declared depth), so XYB lies in [-11.7, 12.5] x
+//     [-0.15, 1.58] x [-1.18, 2.28], |a|, |b| <= 24.2, |a - b| <= 48.4 and
+//     |ab| <= 586, i.e. at most 37,500 at scale 2^4. The taps are the values
+//     the fp16 pair represents exactly (within 2^-22 of the FP32 taps), and
+//     the renormalisation sums use the same values. On the GPU the scores of
+//     real frames stay within 3e-5 of the FP64 oracle, at about half the
+//     splitting work of the exact split and two thirds of its matrix products.
 //
 // Tile: 64 x 64 output pixels of one scale; each of the four waves owns a
 // 16-column strip (80 input rows x 32 input columns) and needs nothing from
@@ -136,9 +141,9 @@ __device__ __forceinline__ f32x4 mfma_f16(u32x4 a, u32x4 b, f32x4 c) {
 
 struct SplitF16x2 {
     static constexpr int kParts = 2;
-    static constexpr float kScaleAB = 2048.f;        // a, b: 2^11
-    static constexpr float kScaleDD = 4.f;           // (a - b)^2 as (4 (a - b))^2: 2^4
-    static constexpr float kScaleQ = 1.f / 65536.f;  // ab from the scaled a, b: 2^22 / 2^16 = 2^6
+    static constexpr float kScaleAB = 4.f;  // a, b: 2^2
+    static constexpr float kScaleDD = 4.f;  // (a - b)^2 as (4 (a - b))^2: 2^4
+    static constexpr float kScaleQ = 1.f;   // ab from the scaled a, b: 2^4
     static constexpr float kTapScale = 16384.f;      // taps: 2^14
     struct Op {
         u32x4 p[2];  // h, l
@@ -275,11 +280,11 @@ __device__ __forceinline__ const ST* at(const ST* base, uint32_t byte_off) {
 // Shared memory of one workgroup, for a split into NP parts.
 template <int NP>
 struct MxLds {
-    // ring[wave][slot][moment][part][column][8 words = 16 rows of 16-bit parts];
-    // the two 4-word halves of a column are swapped for columns with swz = 4,
-    // so that the 8- and 16-lane groups of one LDS access (columns j .. j+7 or
-    // j+15 at a stride of 8 words) fall on distinct banks
-    uint32_t ring[4][2][4][NP][16][8];
+    // ring[wave][slot][moment][part][row group g][column j][2 words = rows
+    // 4g .. 4g+3 of 16-bit parts]: the 16 lanes of one row group write 32
+    // consecutive words, and a column-pass operand is two 8-byte reads 16
+    // slots apart, so that no access of 16 lanes meets a bank twice
+    uint32_t ring[4][2][4][NP][4][16][2];
     double red[4][kFeat];
     float rowf[4][64];  // per wave: renormalisation factor of the tile's 64 output rows
     float cum[18];      // cumulative taps, as the split represents them
@@ -287,7 +292,7 @@ struct MxLds {
 
 // Geometry of one tile, the same for every channel.
 struct TileGeo {
-    int W, H, x0, y0, xo, g, j, wave, swz;
+    int W, H, x0, y0, xo, g, j, wave;
     int xl;     // column of this lane's 8-element input load
     int cmask;  // bit e: input column xi + e lies in the image
     int64_t pitch, pe;
@@ -331,7 +336,7 @@ template <typename SP, typename ST, bool EDGE>
 __device__ __forceinline__ void mx_tile(MxLds<SP::kParts>& L, const TileGeo& T, const typename SP::Op& G,
                                         const ST* base1, const ST* base2, const float* cum, int lane) {
     constexpr int NP = SP::kParts;
-    const int W = T.W, H = T.H, g = T.g, j = T.j, wave = T.wave, swz = T.swz;
+    const int W = T.W, H = T.H, g = T.g, j = T.j, wave = T.wave;
     const int64_t pitch = T.pitch;
     // moments relative to a and b: (a - b)^2 and ab carry kScaleDD^2 and
     // kScaleAB^2 kScaleQ where a and b carry kScaleAB
@@ -420,20 +425,24 @@ __device__ __forceinline__ void mx_tile(MxLds<SP::kParts>& L, const TileGeo& T, 
                 SP::template split_scaled<4>(rv, kmid, w);
 #pragma unroll
                 for (int p = 0; p < NP; p++)
-                    *reinterpret_cast<uint2*>(&L.ring[wave][slot][m][p][j][(2 * g) ^ swz]) = make_uint2(w[p][0], w[p][1]);
+                    *reinterpret_cast<uint2*>(&L.ring[wave][slot][m][p][g][j][0]) = make_uint2(w[p][0], w[p][1]);
             }
             wave_lds_sync();
             if (mb == 0) continue;
 
             // ---- column pass of output rows y0 + 16 ob .. +15 from ring blocks ob, ob + 1
             const int ob = mb - 1;
-            const int rs = (ob + (g >> 1)) & 1, rw = (4 * (g & 1)) ^ swz;
+            const int rs = (ob + (g >> 1)) & 1, gg = 2 * (g & 1);
             f32x4 o[4];
 #pragma unroll
             for (int m = 0; m < 4; m++) {
                 typename SP::Op B;
 #pragma unroll
-                for (int p = 0; p < NP; p++) B.p[p] = *reinterpret_cast<const u32x4*>(&L.ring[wave][rs][m][p][j][rw]);
+                for (int p = 0; p < NP; p++) {
+                    const uint2 lo = *reinterpret_cast<const uint2*>(&L.ring[wave][rs][m][p][gg][j][0]);
+                    const uint2 hi = *reinterpret_cast<const uint2*>(&L.ring[wave][rs][m][p][gg + 1][j][0]);
+                    B.p[p] = u32x4{lo.x, lo.y, hi.x, hi.y};
+                }
                 o[m] = SP::mma(G, B, z);
             }
             wave_lds_sync();
@@ -499,13 +508,18 @@ __device__ __forceinline__ void mx_tile(MxLds<SP::kParts>& L, const TileGeo& T, 
                 acc[5] = fmaf(t2, t2, acc[5]);
             }
         }
-        // wave sums of this channel in FP64, fixed order
+        // wave sums of this channel in FP64, fixed order; the six chains of
+        // lane exchanges are independent and interleave
+        double x[6];
 #pragma unroll
-        for (int k = 0; k < 6; k++) {
-            double x = (double)acc[k];
+        for (int k = 0; k < 6; k++) x[k] = (double)acc[k];
 #pragma unroll
-            for (int off = 32; off > 0; off >>= 1) x += __shfl_xor(x, off, 64);
-            if (lane == 0) L.red[wave][c * 6 + k] = x;
+        for (int off = 32; off > 0; off >>= 1)
+#pragma unroll
+            for (int k = 0; k < 6; k++) x[k] += __shfl_xor(x[k], off, 64);
+        if (lane == 0) {
+#pragma unroll
+            for (int k = 0; k < 6; k++) L.red[wave][c * 6 + k] = x[k];
         }
     }
 }
@@ -594,7 +608,6 @@ __global__ __launch_bounds__(256, SP::kParts == 2 ? 4 : 2) void map_mx_kernel(co
         T.g = g;
         T.j = j;
         T.wave = wave;
-        T.swz = (((j >> 2) ^ (j >> 3)) & 1) << 2;
         const int xs0 = T.x0 + 16 * wave;  // first output column of this wave's strip
         T.xo = xs0 + j;                    // this lane's output column
         // this lane's eight input columns xi .. xi + 7 (xi a multiple of 8): a

@@ -113,7 +113,9 @@ def test_mfma_extreme_yuv_codes_stay_in_range():
         say(f"  oracle {s_o:.6f} mfma {s_m[k]:.6f} ({s_m[k] - s_o:+.1e}) valu {s_v[k]:.6f}")
         assert np.isfinite(s_m[k])
         assert abs(s_m[k] - s_o) <= SCORE_TOL
-        assert abs(s_m[k] - s_v[k]) <= 1e-4
+        # the two GPU kernels agree to 1e-4 plus a few FP32 ulps of the
+        # score (these noise frames score near -1163, where one ulp is 1.2e-4)
+        assert abs(s_m[k] - s_v[k]) <= 1e-4 + 4 * 2.0 ** -23 * abs(s_o)
 
 
 def test_mfma_yuv10_codes_above_depth_are_clamped():

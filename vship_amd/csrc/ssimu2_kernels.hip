@@ -102,26 +102,27 @@ __device__ __forceinline__ void store_planes(uint8_t* arena, const PairDev& P, i
     stf_nt(reinterpret_cast<T*>(b2 + o), B);
 }
 
-// Four consecutive scale-0 pixels x .. x+3 (x a multiple of 4) of plane c of
-// row y in one 16-byte (FP16 planes: 8-byte) store. The planes are read back
-// only by the map kernels, after this kernel has written far more than the
-// caches hold, so the stores are non-temporal (streaming): measured at
+// Four consecutive pixels x .. x+3 (x a multiple of 4) of plane c of row y of
+// scale s in one 16-byte (FP16 planes: 8-byte) store. The planes are read
+// back only by the map kernels, after this kernel has written far more than
+// the caches hold, so the stores are non-temporal (streaming): measured at
 // 1920x1080, 240 pairs, 5.36 -> 4.40 ms for the whole kernel.
 template <typename T>
-__device__ __forceinline__ void store_row4(uint8_t* arena, const PairDev& P, int img, int c, int x, int y, float4 v);
+__device__ __forceinline__ void store_row4(uint8_t* arena, const PairDev& P, int img, int s, int c, int x, int y,
+                                           float4 v);
 template <>
-__device__ __forceinline__ void store_row4<float>(uint8_t* arena, const PairDev& P, int img, int c, int x, int y,
-                                                  float4 v) {
-    uint8_t* b = arena + P.arena_off + img * P.img_bytes + P.scale_off[0] + c * P.plane_elems[0] * 4;
-    const uint32_t o = ((uint32_t)y * (uint32_t)P.pitch[0] + (uint32_t)x) * 4u;
+__device__ __forceinline__ void store_row4<float>(uint8_t* arena, const PairDev& P, int img, int s, int c, int x,
+                                                  int y, float4 v) {
+    uint8_t* b = arena + P.arena_off + img * P.img_bytes + P.scale_off[s] + c * P.plane_elems[s] * 4;
+    const uint32_t o = ((uint32_t)y * (uint32_t)P.pitch[s] + (uint32_t)x) * 4u;
     typedef float f4 __attribute__((ext_vector_type(4)));
     __builtin_nontemporal_store((f4){v.x, v.y, v.z, v.w}, reinterpret_cast<f4*>(b + o));
 }
 template <>
-__device__ __forceinline__ void store_row4<__half>(uint8_t* arena, const PairDev& P, int img, int c, int x, int y,
-                                                   float4 v) {
-    uint8_t* b = arena + P.arena_off + img * P.img_bytes + P.scale_off[0] + c * P.plane_elems[0] * 2;
-    const uint32_t o = ((uint32_t)y * (uint32_t)P.pitch[0] + (uint32_t)x) * 2u;
+__device__ __forceinline__ void store_row4<__half>(uint8_t* arena, const PairDev& P, int img, int s, int c, int x,
+                                                   int y, float4 v) {
+    uint8_t* b = arena + P.arena_off + img * P.img_bytes + P.scale_off[s] + c * P.plane_elems[s] * 2;
+    const uint32_t o = ((uint32_t)y * (uint32_t)P.pitch[s] + (uint32_t)x) * 2u;
     const __half2 a = __floats2half2_rn(v.x, v.y), d = __floats2half2_rn(v.z, v.w);
     typedef uint32_t u2 __attribute__((ext_vector_type(2)));
     __builtin_nontemporal_store((u2){*reinterpret_cast<const uint32_t*>(&a), *reinterpret_cast<const uint32_t*>(&d)},
@@ -221,10 +222,21 @@ __device__ __forceinline__ void prep_quads(PrepLds& L, const PairDev& P, uint8_t
 
     const int nv1x = min(32, P.sw[1] - tx * 32), nv1y = min(32, P.sh[1] - ty * 32);
     const bool full = x0 + kPrepTile <= W && y0 + kPrepTile <= H;
-#pragma unroll 1
+    // thread t takes the quads of column t % 32 in rows 4 (t / 32) .. +3 (lanes
+    // 2k and 2k+1 still hold horizontally adjacent quads); the staged 4:2:0
+    // path keeps the horizontal chroma results of the eight staged rows these
+    // four quads read in registers, each row computed once
+    const int qx = tid & 31, qy0 = 4 * (tid >> 5);
+    float che[2][8], cho[2][8];
+#pragma unroll
     for (int i = 0; i < 4; i++) {
-        const int q = tid + 256 * i;
-        const int qy = q >> 5, qx = q & 31;
+        const int qy = qy0 + i;
+        if constexpr (yuv && !GEN) {
+#pragma unroll
+            for (int r = (i == 0 ? 0 : i + 4); r <= i + 4; r++)
+#pragma unroll
+                for (int pl = 0; pl < 2; pl++) chroma420_hrow(L.ch[pl][qy0 + r], qx, che[pl][r], cho[pl][r]);
+        }
         const int xa = x0 + 2 * qx, ya = y0 + 2 * qy;
         if (xa >= W || ya >= H) continue;  // no pixel of this quad is in the image
         const bool hx = xa + 1 < W, hy = ya + 1 < H;
@@ -259,8 +271,7 @@ __device__ __forceinline__ void prep_quads(PrepLds& L, const PairDev& P, uint8_t
                 }
             } else {
 #pragma unroll
-                for (int pl = 0; pl < 2; pl++)
-                    chroma420_quad(L.ch[pl], qx, qy, hx, hy, cmul, cadd, cbcr[pl]);
+                for (int pl = 0; pl < 2; pl++) chroma420_vquad(&che[pl][i], &cho[pl][i], hx, hy, cmul, cadd, cbcr[pl]);
             }
 #pragma unroll
             for (int k = 0; k < 4; k++) {
@@ -292,7 +303,7 @@ __device__ __forceinline__ void prep_quads(PrepLds& L, const PairDev& P, uint8_t
                 const float s0 = odd ? v[c][0] : v[c][1], s1 = odd ? v[c][2] : v[c][3];
                 const float r0 = __shfl_xor(s0, 1, 64), r1 = __shfl_xor(s1, 1, 64);
                 const float4 o = odd ? make_float4(r0, r1, v[c][1], v[c][3]) : make_float4(v[c][0], v[c][2], r0, r1);
-                store_row4<ST>(arena, P, img, c, sx, sy, o);
+                store_row4<ST>(arena, P, img, 0, c, sx, sy, o);
             }
         } else {
             store_xyb<ST>(arena, P, img, 0, xa, ya, rgb[0][0], rgb[0][1], rgb[0][2]);
@@ -300,21 +311,40 @@ __device__ __forceinline__ void prep_quads(PrepLds& L, const PairDev& P, uint8_t
             if (hx) store_xyb<ST>(arena, P, img, 0, xb, ya, rgb[2][0], rgb[2][1], rgb[2][2]);
             if (hx && hy) store_xyb<ST>(arena, P, img, 0, xb, yb, rgb[3][0], rgb[3][1], rgb[3][2]);
         }
-        // scale-1 child (src/ssimu2/downsample.hpp:3-19 summation order)
+        // scale-1 child (src/ssimu2/downsample.hpp:3-19 summation order),
+        // into LDS: store_scale1 writes its XYB in whole-row vector stores
         if (qx < nv1x && qy < nv1y) {
-            float v[3];
 #pragma unroll
             for (int p = 0; p < 3; p++) {
                 float a = rgb[0][p];
                 a += rgb[1][p];
                 a += rgb[2][p];
                 a += rgb[3][p];
-                v[p] = a * 0.25f;
+                L.l1[p][qy][qx] = a * 0.25f;
             }
-            store_xyb<ST>(arena, P, img, 1, tx * 32 + qx, ty * 32 + qy, v[0], v[1], v[2]);
-#pragma unroll
-            for (int p = 0; p < 3; p++) L.l1[p][qy][qx] = v[p];
         }
+    }
+}
+
+// XYB of the tile's 32x32 scale-1 pixels from their linear RGB in LDS: thread
+// t takes pixels 4 (t % 8) .. +3 of row t / 8, one 16-byte store per plane for
+// four whole pixels (a wave writes eight 128-byte rows per plane), pixel by
+// pixel where the row ends inside the four.
+template <typename ST>
+__device__ __forceinline__ void store_scale1(const PrepLds& L, const PairDev& P, uint8_t* arena, int img, int tx,
+                                             int ty, int nv1x, int nv1y) {
+    const int r = threadIdx.x >> 3, c4 = (threadIdx.x & 7) * 4;
+    if (r >= nv1y || c4 >= nv1x) return;
+    const int x = tx * 32 + c4, y = ty * 32 + r;
+    if (c4 + 4 <= nv1x) {
+        float v[3][4];
+#pragma unroll
+        for (int k = 0; k < 4; k++) to_xyb(L.l1[0][r][c4 + k], L.l1[1][r][c4 + k], L.l1[2][r][c4 + k], v[0][k], v[1][k], v[2][k]);
+#pragma unroll
+        for (int c = 0; c < 3; c++) store_row4<ST>(arena, P, img, 1, c, x, y, make_float4(v[c][0], v[c][1], v[c][2], v[c][3]));
+    } else {
+        for (int k = 0; k < nv1x - c4; k++)
+            store_xyb<ST>(arena, P, img, 1, x + k, y, L.l1[0][r][c4 + k], L.l1[1][r][c4 + k], L.l1[2][r][c4 + k]);
     }
 }
 
@@ -366,6 +396,7 @@ __global__ __launch_bounds__(256) void prep_kernel(const PairDev* __restrict__ p
         }
         __syncthreads();
         const int nv1x = min(32, P.sw[1] - tx * 32), nv1y = min(32, P.sh[1] - ty * 32);
+        store_scale1<ST>(L, P, arena, img, tx, ty, nv1x, nv1y);
 
         // scales 2..5 from LDS
         pyr_level<16, 33, 17, ST>(&L.l1[0][0][0], &L.l2[0][0][0], arena, P, img, 2, tx, ty, nv1x, nv1y);

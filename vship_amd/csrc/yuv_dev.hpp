@@ -112,19 +112,21 @@ __device__ __forceinline__ void stage_chroma420(float (*ch)[kCh420][kCh420 + 1],
 // (2 qx + {0, 1}, 2 qy + {0, 1}) of the staged tile, in the order (xa, ya),
 // (xa, yb), (xb, ya), (xb, yb); hx / hy: the quad has a second column / row
 // (else it repeats the first). Horizontally the even column takes the
-// co-sited sample and the odd one Catmull-Rom at 1/2, (-1, 9, 9, -1) / 16;
-// vertically (centre-sited) the even row is Catmull-Rom at 3/4 from rows
-// 0..3 and the odd row at 1/4 from rows 1..4, in integer weights of total 128
-// that cmul takes out.
-__device__ __forceinline__ void chroma420_quad(const float (*ch)[kCh420 + 1], int qx, int qy, bool hx, bool hy,
-                                               float cmul, float cadd, float out[4]) {
-    float he[5], ho[5];
-#pragma unroll
-    for (int r = 0; r < 5; r++) {
-        const float* row = ch[qy + r];
-        he[r] = row[qx + 1];
-        ho[r] = (9.0f * (row[qx + 1] + row[qx + 2]) - (row[qx] + row[qx + 3])) * (1.0f / 16.0f);
-    }
+// co-sited sample and the odd one Catmull-Rom at 1/2, (-1, 9, 9, -1) / 16
+// (chroma420_hrow, once per staged row: vertically adjacent quads share four
+// of their five rows); vertically (centre-sited) the even row is Catmull-Rom
+// at 3/4 from rows 0..3 and the odd row at 1/4 from rows 1..4, in integer
+// weights of total 128 that cmul takes out.
+// Horizontal step of one staged row for the quad column qx: the even column's
+// co-sited sample and the odd column's Catmull-Rom value at 1/2.
+__device__ __forceinline__ void chroma420_hrow(const float* row, int qx, float& he, float& ho) {
+    he = row[qx + 1];
+    ho = (9.0f * (row[qx + 1] + row[qx + 2]) - (row[qx] + row[qx + 3])) * (1.0f / 16.0f);
+}
+
+// Vertical step from the horizontal results of staged rows qy .. qy + 4.
+__device__ __forceinline__ void chroma420_vquad(const float* he, const float* ho, bool hx, bool hy, float cmul,
+                                                float cadd, float out[4]) {
     const float ve_e = -3.0f * he[0] + 29.0f * he[1] + 111.0f * he[2] - 9.0f * he[3];
     const float vo_e = -9.0f * he[1] + 111.0f * he[2] + 29.0f * he[3] - 3.0f * he[4];
     const float ve_o = -3.0f * ho[0] + 29.0f * ho[1] + 111.0f * ho[2] - 9.0f * ho[3];
@@ -133,6 +135,15 @@ __device__ __forceinline__ void chroma420_quad(const float (*ch)[kCh420 + 1], in
     out[1] = fmaf(hy ? vo_e : ve_e, cmul, cadd);
     out[2] = fmaf(hx ? ve_o : ve_e, cmul, cadd);
     out[3] = fmaf(hx ? (hy ? vo_o : ve_o) : (hy ? vo_e : ve_e), cmul, cadd);
+}
+
+// Both steps for one quad, the five rows filtered here.
+__device__ __forceinline__ void chroma420_quad(const float (*ch)[kCh420 + 1], int qx, int qy, bool hx, bool hy,
+                                               float cmul, float cadd, float out[4]) {
+    float he[5], ho[5];
+#pragma unroll
+    for (int r = 0; r < 5; r++) chroma420_hrow(ch[qy + r], qx, he[r], ho[r]);
+    chroma420_vquad(he, ho, hx, hy, cmul, cadd, out);
 }
 
 }  // namespace vship

@@ -406,6 +406,17 @@ __device__ __forceinline__ void mx_tile(MxLds<SP::kParts>& L, const TileGeo& T, 
     }
     const float k1i = inv_total * kOut;
     const int xc = EDGE ? min(T.xo, W - 1) : T.xo;
+    // Work that can only add zeros is skipped on edge tiles: a wave whose 16
+    // output columns all lie right of the image, and the row blocks below the
+    // last 16-row output block that holds an image row (the last tile row of
+    // the coarse scales holds 4 to 28 of its 64 rows). Skipping an addition
+    // of exact zeros leaves every sum bit for bit as it was.
+    if (EDGE && T.x0 + 16 * wave >= W) {
+        if (lane == 63)
+            for (int k = 0; k < kFeat; k++) L.red[wave][k] = 0.0;
+        return;
+    }
+    const int nmb = EDGE ? min(5, (min(H - T.y0, kMxTile) + 15) / 16 + 1) : 5;
 
 #pragma unroll 1
     for (int c = 0; c < 3; c++) {
@@ -423,7 +434,7 @@ __device__ __forceinline__ void mx_tile(MxLds<SP::kParts>& L, const TileGeo& T, 
         float acc[6] = {0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
 
 #pragma unroll 1
-        for (int mb = 0; mb < 5; mb++) {
+        for (int mb = 0; mb < nmb; mb++) {
             // ---- row pass of input rows y0 - 8 + 16 mb .. +15; this lane: row j, columns xi .. xi + 7
             const int yr = T.y0 - 8 + 16 * mb + j;
             typename SP::Op A[4];

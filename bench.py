@@ -142,7 +142,8 @@ TRAFFIC_FILE = os.path.join(ROOT, "profiles", "hbm_traffic.json")
 VALU_FILE = os.path.join(ROOT, "profiles", "valu_issue.json")
 # MI355X vector ALUs: 256 CUs x 4 SIMDs x 16 lanes at the 2.4 GHz peak engine
 # clock; a wave64 VALU instruction is 64 lane-ops (4 cycles of its SIMD)
-VALU_PEAK_TOPS = 256 * 4 * 16 * 2.4e9 / 1e12
+VALU_SIMDS, VALU_CLOCK_HZ = 256 * 4, 2.4e9
+VALU_PEAK_TOPS = VALU_SIMDS * 16 * VALU_CLOCK_HZ / 1e12
 
 
 def _record(path, config, kernel):
@@ -172,10 +173,16 @@ def valu_position(config, kernel, ms):
         return None
     lane_ops = (rec["valu_insts"] - rec["mfma_insts"]) * 64
     achieved = lane_ops / (ms * 1e-3) / 1e12
-    return {"achieved": round(achieved, 2), "peak": round(VALU_PEAK_TOPS, 2), "unit": "T VALU lane-ops/s",
-            "frac": round(achieved / VALU_PEAK_TOPS, 4), "lane_ops_per_launch": int(lane_ops),
-            "transcendental_share": round(rec["trans_insts"] / max(1, rec["valu_insts"] - rec["mfma_insts"]), 4),
-            "issue_of_active_cycles": rec.get("valu_issue_of_active_cycles")}
+    out = {"achieved": round(achieved, 2), "peak": round(VALU_PEAK_TOPS, 2), "unit": "T VALU lane-ops/s",
+           "frac": round(achieved / VALU_PEAK_TOPS, 4), "lane_ops_per_launch": int(lane_ops),
+           "transcendental_share": round(rec["trans_insts"] / max(1, rec["valu_insts"] - rec["mfma_insts"]), 4),
+           "issue_of_active_cycles": rec.get("valu_issue_of_active_cycles")}
+    if "valu_quad_cycles" in rec:
+        # quad-cycles the VALU instructions occupy (half-rate ones twice), x 4
+        # cycles, over every SIMD's cycles at the peak clock for the live time
+        out["issue_cycles_frac"] = round(rec["valu_quad_cycles"] * 4 / (VALU_SIMDS * VALU_CLOCK_HZ * ms * 1e-3), 4)
+        out["issue_cycles_of_active_cycles"] = rec.get("valu_cycles_of_active_cycles")
+    return out
 
 
 def map_kernel_name(st, fp16):

@@ -1,7 +1,7 @@
-"""The matrix-core map kernel (map_tile_height = 1: 64x64 tiles, both blur
-passes as 16x16x32 MFMA products of operands split into 16-bit parts: a
-scaled two-way fp16 split for the bounded 8- to 16-bit RGB and YUV formats,
-an exact three-way bf16 split for float RGB) against the FP64 oracle
+"""The matrix-core map kernel (map_tile_height = 1: tiles of 64 columns by
+128 rows, both blur passes as 16x16x32 MFMA products of operands split into
+16-bit parts: a scaled two-way fp16 split for the bounded 8- to 16-bit RGB and
+YUV formats, an exact three-way bf16 split for float RGB) against the FP64 oracle
 (|dscore| <= 1e-3, features as in the parity tests) and against the
 vector-ALU kernel (the same FP32 maps to rounding: |dscore| <= 1e-4)."""
 import numpy as np
@@ -31,8 +31,9 @@ def gpu_ready():
 @pytest.mark.parametrize("fmt", [api.PF_RGB_U8, api.PF_RGB_U16, api.PF_RGB_F16, api.PF_RGB_F32])
 def test_mfma_sizes_and_formats(fmt):
     # sizes with FP32-class segments (>= 32768 px) that are multiples of the
-    # 64x64 tile, odd, thin, and wider than tall
-    for i, (w, h) in enumerate([(256, 256), (300, 211), (193, 177), (1031, 33), (64, 600), (517, 389)]):
+    # 64 x 128 tile, odd, thin, wider than tall, and with interior tiles above
+    # a last tile row of 18 rows
+    for i, (w, h) in enumerate([(256, 256), (300, 211), (193, 177), (1031, 33), (64, 600), (517, 389), (401, 530)]):
         ref, dist = rgb_pair(w, h, fmt, 300 + i)
         say(f"case: mfma map, {FMT_NAMES[fmt]} {w}x{h}")
         s_o, f_o = oracle_rgb(ref, dist, fmt)

@@ -39,10 +39,10 @@
 //     real frames stay within 3e-5 of the FP64 oracle, at about half the
 //     splitting work of the exact split and two thirds of its matrix products.
 //
-// Tile: 64 x 64 output pixels of one scale; each of the four waves owns a
-// 16-column strip (80 input rows x 32 input columns) and needs nothing from
-// the other waves until the tile sums. A wave streams five 16-row blocks
-// through the row pass; the row-pass results go, split, to a two-block LDS
+// Tile: 64 columns x 128 rows of output pixels of one scale; each of the four
+// waves owns a 16-column strip (144 input rows x 32 input columns) and needs
+// nothing from the other waves until the tile sums. A wave streams nine 16-row
+// blocks through the row pass; the row-pass results go, split, to a two-block LDS
 // ring of the wave, from which the column pass of output block ob reads
 // blocks ob and ob + 1. Each output pixel's maps are formed in the registers
 // the column pass leaves them in.
@@ -337,7 +337,7 @@ struct MxLds {
     // slots apart, so that no access of 16 lanes meets a bank twice
     uint32_t ring[4][2][4][NP][4][16][2];
     double red[4][kFeat];
-    float rowf[4][64];  // per wave: renormalisation factor of the tile's 64 output rows
+    float rowf[4][kMxTileH];  // per wave: renormalisation factor of the tile's output rows
     float cum[18];      // cumulative taps, as the split represents them
 };
 
@@ -369,7 +369,7 @@ __device__ __forceinline__ typename SP::Op tap_operand(int g, int j) {
 
 // The three channels of one tile; leaves the 18 wave sums in red[wave].
 //
-// EDGE is false for tiles whose 80 x 80 input region lies inside the image:
+// EDGE is false for tiles whose 80 x 144 input region lies inside the image:
 // no element is masked, every output pixel is in the image and every
 // renormalisation total is the full sum of the taps, so the factors fold into
 // constants. Edge tiles clamp their loads into the image, give out-of-image
@@ -400,8 +400,11 @@ __device__ __forceinline__ void mx_tile(MxLds<SP::kParts>& L, const TileGeo& T, 
     const float kmid =
         (EDGE ? __builtin_amdgcn_rcpf(tap_total(cum, T.xo, W)) : inv_total) * (1.f / SP::kTapScale);
     if (EDGE) {
-        // per-wave table of k1 for the tile's 64 output rows (lane l: row y0 + l)
-        L.rowf[wave][lane] = __builtin_amdgcn_rcpf(tap_total(cum, min(T.y0 + lane, H - 1), H)) * kOut;
+        // per-wave table of k1 for the tile's output rows (lane l: rows y0 + l + 64 q)
+#pragma unroll
+        for (int q = 0; q < kMxTileH / 64; q++)
+            L.rowf[wave][lane + 64 * q] =
+                __builtin_amdgcn_rcpf(tap_total(cum, min(T.y0 + lane + 64 * q, H - 1), H)) * kOut;
         wave_lds_sync();
     }
     const float k1i = inv_total * kOut;
@@ -409,20 +412,27 @@ __device__ __forceinline__ void mx_tile(MxLds<SP::kParts>& L, const TileGeo& T, 
     // Work that can only add zeros is skipped on edge tiles: a wave whose 16
     // output columns all lie right of the image, and the row blocks below the
     // last 16-row output block that holds an image row (the last tile row of
-    // the coarse scales holds 4 to 28 of its 64 rows). Skipping an addition
-    // of exact zeros leaves every sum bit for bit as it was.
+    // a scale often holds far fewer than 128 rows). Skipping an addition of
+    // exact zeros leaves every sum bit for bit as it was.
     if (EDGE && T.x0 + 16 * wave >= W) {
         if (lane == 63)
             for (int k = 0; k < kFeat; k++) L.red[wave][k] = 0.0;
         return;
     }
-    const int nmb = EDGE ? min(5, (min(H - T.y0, kMxTile) + 15) / 16 + 1) : 5;
+    constexpr int kMb = kMxTileH / 16 + 1;
+    const int nmb = EDGE ? min(kMb, (min(H - T.y0, kMxTileH) + 15) / 16 + 1) : kMb;
+    // The masks and selects of an edge tile are skipped, by a wave-uniform
+    // branch, in the blocks that do not need them: when this wave's 32 input
+    // columns (xin) or 16 output columns (xout) lie in the image, and so do
+    // the block's rows, every mask bit is set and every select keeps its value.
+    const int xs0 = T.x0 + 16 * wave;
+    const bool xin = xs0 >= 8 && xs0 + 24 <= W, xout = xs0 + 16 <= W;
 
 #pragma unroll 1
     for (int c = 0; c < 3; c++) {
         const ST* p1 = base1 + c * T.pe;
         const ST* p2 = base2 + c * T.pe;
-        const int cxp = min(T.x0 + kMxTile / 2, W - 1), cyp = min(T.y0 + kMxTile / 2, H - 1);
+        const int cxp = min(T.x0 + kMxTile / 2, W - 1), cyp = min(T.y0 + kMxTileH / 2, H - 1);
         // each image centred on its own value at the tile centre: a = x1 - cc,
         // b = x2 - cc2; the difference moment is then (a - b)^2, which stays
         // near 0 where the images differ by a locally constant offset, so
@@ -443,7 +453,7 @@ __device__ __forceinline__ void mx_tile(MxLds<SP::kParts>& L, const TileGeo& T, 
                 const uint32_t ro = plane_off<ST>(EDGE ? min(max(yr, 0), H - 1) : yr, pitch, T.xl);
                 Load8<ST>::run(at(p1, ro), u1);
                 Load8<ST>::run(at(p2, ro), u2);
-                if (EDGE) {
+                if (EDGE && !(xin && T.y0 - 8 + 16 * mb >= 0 && T.y0 + 8 + 16 * mb <= H)) {
                     // one vector load per image for every lane: the row is
                     // clamped into the image and an element outside it takes
                     // the centre values cc and cc2, so that all four of its
@@ -555,7 +565,7 @@ __device__ __forceinline__ void mx_tile(MxLds<SP::kParts>& L, const TileGeo& T, 
                 const float ev = ((fb - fa) * den) * rr;
                 float art = fmaxf(ev, 0.f), det = fmaxf(-ev, 0.f);
                 d = fmaxf(d, 0.f);
-                if (EDGE) {
+                if (EDGE && !(xout && T.y0 + 16 * ob + 16 <= H)) {
                     const bool ok = T.xo < W && yo + r < H;
                     d = ok ? d : 0.f;
                     art = ok ? art : 0.f;
@@ -583,8 +593,8 @@ __device__ __forceinline__ void mx_tile(MxLds<SP::kParts>& L, const TileGeo& T, 
 
 }  // namespace
 
-// Occupancy: the two-part split fits 4 workgroups per CU (33 KB of LDS, at
-// most 128 VGPRs), the three-part split 3 (49 KB, about 135 VGPRs).
+// Occupancy: the two-part split fits 4 workgroups per CU (35 KB of LDS, at
+// most 128 VGPRs), the three-part split 3 (52 KB of LDS).
 template <typename ST, typename SP>
 __global__ __launch_bounds__(256, SP::kParts == 2 ? 4 : 2) void map_mx_kernel(const PairDev* __restrict__ pairs,
                                                         const int32_t* __restrict__ order_seg,
@@ -661,7 +671,7 @@ __global__ __launch_bounds__(256, SP::kParts == 2 ? 4 : 2) void map_mx_kernel(co
         const int local = t - order_start[lo];
         const int ty = local / ntx, tx = local - ty * ntx;
         T.x0 = tx * TS;
-        T.y0 = ty * TS;
+        T.y0 = ty * kMxTileH;
         T.g = g;
         T.j = j;
         T.wave = wave;
@@ -677,8 +687,8 @@ __global__ __launch_bounds__(256, SP::kParts == 2 ? 4 : 2) void map_mx_kernel(co
         T.cmask = gin ? (int)((1u << min(T.W - xi, 8)) - 1u) : 0;
         const ST* base1 = reinterpret_cast<const ST*>(arena + P.arena_off + P.scale_off[s]);
         const ST* base2 = reinterpret_cast<const ST*>(arena + P.arena_off + P.img_bytes + P.scale_off[s]);
-        // the 80 x 80 input region in the image: no masks, constant renormalisation
-        const bool interior = T.x0 >= 8 && T.x0 + kMxTile + 8 <= T.W && T.y0 >= 8 && T.y0 + kMxTile + 8 <= T.H;
+        // the 80 x 144 input region in the image: no masks, constant renormalisation
+        const bool interior = T.x0 >= 8 && T.x0 + kMxTile + 8 <= T.W && T.y0 >= 8 && T.y0 + kMxTileH + 8 <= T.H;
         if (interior) mx_tile<SP, ST, false>(L, T, G, base1, base2, L.cum, lane);
         else mx_tile<SP, ST, true>(L, T, G, base1, base2, L.cum, lane);
 

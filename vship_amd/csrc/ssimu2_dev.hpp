@@ -121,11 +121,14 @@ constexpr int64_t kSmallSegment = 32768;
 constexpr int64_t kSmallSegmentMx = 4096;
 constexpr int kSmallTH = 48;
 // Tile height value that selects the matrix-core map kernel for the FP32
-// class; its tiles are kMxTile x kMxTile.
+// class; its tiles are kMxTile columns by kMxTileH rows (a wave streams down
+// its 16-column strip of the tile, so a taller tile re-reads a smaller share
+// of halo rows: 144 input rows for 128 output rows).
 constexpr int kMapMfma = 1;
 constexpr int kMxTile = 64;
+constexpr int kMxTileH = 128;
 __host__ __device__ constexpr int map_tile_w(int th) { return th == kMapMfma ? kMxTile : kMapTW; }
-__host__ __device__ constexpr int map_tile_h(int th) { return th == kMapMfma ? kMxTile : th; }
+__host__ __device__ constexpr int map_tile_h(int th) { return th == kMapMfma ? kMxTileH : th; }
 
 // The 17-tap Gaussian (sigma 1.5, radius 8) of src/ssimu2/gaussianblur.hpp:7-18
 // and src/util/preprocessor.hpp:89-92, normalised to sum 1 and rounded to

@@ -191,7 +191,7 @@ def map_kernel_name(st, fp16):
     st_name = "__half+float" if fp16 else "float"
     if st["map_tile_height"] == 1:
         kernel = f"map_mx_kernel<{st_name}, SplitF16x2>"
-        return kernel, f"{kernel} (matrix cores, 64x64 tiles)"
+        return kernel, f"{kernel} (matrix cores, tiles of 64 columns x 128 rows)"
     kernel = f"map_kernel<{st['map_tile_height']}, {st_name}, float>"
     return kernel, kernel
 

@@ -7,6 +7,15 @@
 
 namespace vship {
 
+// threadIdx.x read anew where it is used: the index arithmetic derived from it
+// is recomputed per use instead of being hoisted out of a loop and held (or
+// spilled) for the whole kernel.
+__device__ __forceinline__ int tid_in_loop() {
+    int t = threadIdx.x;
+    asm volatile("" : "+v"(t));
+    return t;
+}
+
 // Global-memory view of a device pointer that the compiler cannot place (one
 // read from a descriptor), so that loads through it are global loads rather
 // than flat ones, which also wait on LDS traffic.

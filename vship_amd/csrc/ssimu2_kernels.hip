@@ -74,10 +74,11 @@ __device__ __forceinline__ void to_xyb(float r, float g, float b, float& X, floa
 struct PrepLds {
     float l1[3][32][33];  // linear RGB of scales 1..4
     // the staged chroma is read only while scales 0 and 1 are formed, the
-    // coarser levels only after: they share their memory (23 KB in all, so
-    // that six workgroups fit in one CU's LDS)
+    // coarser levels only after: they share their memory, and the chroma rows
+    // carry no pad word, so that the struct is 23,040 bytes (45 blocks of
+    // 512) and seven workgroups fit in one CU's LDS
     union {
-        float ch[2][36][37];  // staged chroma neighbourhood of the tile (clamped indices)
+        float ch[2][36][36];  // staged chroma neighbourhood of the tile (clamped indices)
         struct {
             float l2[3][16][17];
             float l3[3][8][9];
@@ -155,7 +156,7 @@ __device__ __forceinline__ void pyr_level(const float* par, float* chi, uint8_t*
                                           int tx, int ty, int npx, int npy) {
     const int x0 = tx * TS, y0 = ty * TS;
     const int nvx = min(TS, P.sw[s] - x0), nvy = min(TS, P.sh[s] - y0);
-    for (int idx = threadIdx.x; idx < TS * TS; idx += blockDim.x) {
+    for (int idx = tid_in_loop(); idx < TS * TS; idx += blockDim.x) {
         const int cy = idx / TS, cx = idx - cy * TS;
         if (cx >= nvx || cy >= nvy) continue;
         const int px0 = 2 * cx, py0 = 2 * cy;
@@ -199,7 +200,7 @@ template <typename ST, int FMT, bool GEN = false>
 __device__ __forceinline__ void prep_quads(PrepLds& L, const PairDev& P, uint8_t* arena, int img, int tx, int ty) {
     constexpr bool yuv = FMT >= IN_YUV420_U8;
     constexpr bool wide = FMT == IN_YUV420_U16;
-    const int tid = threadIdx.x;
+    const int tid = tid_in_loop();
     const int W = P.w, H = P.h;
     const int x0 = tx * kPrepTile, y0 = ty * kPrepTile;
     const uint8_t* const* src = P.src[img];
@@ -340,7 +341,8 @@ __device__ __forceinline__ void prep_quads(PrepLds& L, const PairDev& P, uint8_t
 template <typename ST>
 __device__ __forceinline__ void store_scale1(const PrepLds& L, const PairDev& P, uint8_t* arena, int img, int tx,
                                              int ty, int nv1x, int nv1y) {
-    const int r = threadIdx.x >> 3, c4 = (threadIdx.x & 7) * 4;
+    const int tid = tid_in_loop();
+    const int r = tid >> 3, c4 = (tid & 7) * 4;
     if (r >= nv1y || c4 >= nv1x) return;
     const int x = tx * 32 + c4, y = ty * 32 + r;
     if (c4 + 4 <= nv1x) {
@@ -359,12 +361,13 @@ __device__ __forceinline__ void store_scale1(const PrepLds& L, const PairDev& P,
 // come from prep_quads, instantiated per input format; scales 2..5 go through
 // LDS. Every XYB plane is written exactly once. GEN = false takes the RGB and
 // staged 4:2:0 pairs, GEN = true the pairs of the generic chroma path; each
-// skips the other's tiles. Both are held to six waves per SIMD (at most 80
-// VGPRs; the staged path spills four registers): at 1920x1080 the kernel,
-// bound by its stores, takes 4.24 ms at six waves per SIMD against 4.38 ms at
-// five.
+// skips the other's tiles. Both are held to seven waves per SIMD (at most 72
+// VGPRs, no spills: the thread-index arithmetic is recomputed per use rather
+// than held across the tile loop, see tid_in_loop): at 1920x1080 the kernel,
+// bound by its stores, takes 3.95 ms at seven waves per SIMD, 4.24 ms at six
+// and 4.38 ms at five.
 template <typename ST, bool GEN>
-__global__ __launch_bounds__(256, 6) void prep_kernel(const PairDev* __restrict__ pairs,
+__global__ __launch_bounds__(256, 7) void prep_kernel(const PairDev* __restrict__ pairs,
                                                    const int32_t* __restrict__ prep_start,
                                                    const BatchHdr* __restrict__ hdr, uint8_t* __restrict__ arena) {
     __shared__ PrepLds L;

@@ -88,13 +88,15 @@ __device__ __forceinline__ float chroma_at(const uint8_t* plane, int64_t stride,
 // chroma neighbourhood of a 64 x 64 luma tile at (x0, y0), origin
 // (x0 / 2 - 1, y0 / 2 - 2), indices clamped to the plane. Thread t < 252 owns
 // column t % 36 and rows t / 36 + 7 k of both planes, so a wave reads whole
-// row runs and the clamped column is computed once.
+// row runs and the clamped column is computed once. PITCH: floats per staged
+// row (36, or 37 with a pad word).
 constexpr int kCh420 = 36;
 
-__device__ __forceinline__ void stage_chroma420(float (*ch)[kCh420][kCh420 + 1], const uint8_t* const* src,
+template <int PITCH>
+__device__ __forceinline__ void stage_chroma420(float (*ch)[kCh420][PITCH], const uint8_t* const* src,
                                                 const int64_t* stride, int W, int H, int x0, int y0, bool wide,
                                                 int maxcode) {
-    const int tid = threadIdx.x;
+    const int tid = tid_in_loop();
     const int cw = (W + 1) >> 1, chh = (H + 1) >> 1;
     const int cx0 = x0 / 2 - 1, cy0 = y0 / 2 - 2;
     if (tid < 252) {

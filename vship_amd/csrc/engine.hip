@@ -376,10 +376,15 @@ void launch_batch(const DescView& d, const PlanInfo& P, const LaunchShape& S, vo
                 const int g = map_group(cls, half, exact);
                 if (half && !S.fp16) continue;
                 if (!graph && P.ntiles_grp[g] == 0) continue;
-                const int grid = grid_for(P.ntiles_grp[g], cls == 0 ? fixed : std::max(1, fixed / 4));
+                // an eager launch of fewer matrix-core tiles than two per
+                // workgroup slot gives each channel of a tile its own workgroup
+                const int csplit =
+                    (cls == 0 && mfma && !graph && S.persistent_grid == 0 && P.ntiles_grp[g] < 8 * S.num_cus) ? 3
+                                                                                                             : 1;
+                const int grid = grid_for(P.ntiles_grp[g] * csplit, cls == 0 ? fixed : std::max(1, fixed / 4));
                 if (cls == 0 && mfma)
                     HIP_OR_THROW(launch_map_mx(half, exact, grid, d.pairs, d.order_seg, d.order_start, d.hdr, arena,
-                                               partials, st),
+                                               partials, csplit, st),
                                  E_BadDeviceCode);
                 else
                     HIP_OR_THROW(launch_map(half, cls, cls ? kSmallTH : S.map_th, grid, d.pairs, d.order_seg,

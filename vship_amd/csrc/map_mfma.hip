@@ -367,7 +367,8 @@ __device__ __forceinline__ typename SP::Op tap_operand(int g, int j) {
     return G;
 }
 
-// The three channels of one tile; leaves the 18 wave sums in red[wave].
+// Channels c_lo .. c_hi - 1 of one tile; leaves their wave sums in red[wave]
+// (6 per channel, at 6 c).
 //
 // EDGE is false for tiles whose 80 x 144 input region lies inside the image:
 // no element is masked, every output pixel is in the image and every
@@ -385,7 +386,8 @@ __device__ __forceinline__ typename SP::Op tap_operand(int g, int j) {
 // and b.
 template <typename SP, typename ST, bool EDGE>
 __device__ __forceinline__ void mx_tile(MxLds<SP::kParts>& L, const TileGeo& T, const typename SP::Op& G,
-                                        const ST* base1, const ST* base2, const float* cum, int lane) {
+                                        const ST* base1, const ST* base2, const float* cum, int lane, int c_lo,
+                                        int c_hi) {
     constexpr int NP = SP::kParts;
     const int W = T.W, H = T.H, g = T.g, j = T.j, wave = T.wave;
     const int64_t pitch = T.pitch;
@@ -429,7 +431,7 @@ __device__ __forceinline__ void mx_tile(MxLds<SP::kParts>& L, const TileGeo& T, 
     const bool xin = xs0 >= 8 && xs0 + 24 <= W, xout = xs0 + 16 <= W;
 
 #pragma unroll 1
-    for (int c = 0; c < 3; c++) {
+    for (int c = c_lo; c < c_hi; c++) {
         const ST* p1 = base1 + c * T.pe;
         const ST* p2 = base2 + c * T.pe;
         const int cxp = min(T.x0 + kMxTile / 2, W - 1), cyp = min(T.y0 + kMxTileH / 2, H - 1);
@@ -595,13 +597,18 @@ __device__ __forceinline__ void mx_tile(MxLds<SP::kParts>& L, const TileGeo& T, 
 
 // Occupancy: the two-part split fits 4 workgroups per CU (35 KB of LDS, at
 // most 128 VGPRs), the three-part split 3 (52 KB of LDS).
+//
+// csplit: 1, a workgroup takes the three channels of a tile; 3, one channel
+// (work item w: tile w / 3, channel w % 3), for batches with too few tiles to
+// fill the GPU. A channel's sums do not depend on the others, so both give
+// the same partials bit for bit.
 template <typename ST, typename SP>
 __global__ __launch_bounds__(256, SP::kParts == 2 ? 4 : 2) void map_mx_kernel(const PairDev* __restrict__ pairs,
                                                         const int32_t* __restrict__ order_seg,
                                                         const int32_t* __restrict__ order_start,
                                                         const BatchHdr* __restrict__ hdr,
                                                         const uint8_t* __restrict__ arena,
-                                                        double* __restrict__ partials) {
+                                                        double* __restrict__ partials, int csplit) {
     constexpr int TS = kMxTile;
     constexpr int grp = map_group(0, sizeof(ST) == 2, SP::kParts == 3);
     __shared__ MxLds<SP::kParts> L;
@@ -627,32 +634,35 @@ __global__ __launch_bounds__(256, SP::kParts == 2 ? 4 : 2) void map_mx_kernel(co
     }
     const int seg_hi = seg_lo + hdr->nseg[grp] - 1;
     const int t_end = t_begin + hdr->tiles[grp];
-    // A grid of one workgroup per tile (or more) numbers its workgroups with
-    // xcd_block, so that the workgroups of one XCD hold one contiguous range
-    // of tiles. A smaller, persistent grid takes its tiles from the per-XCD
-    // dynamic queue.
+    // A grid of one workgroup per work item (or more) numbers its workgroups
+    // with xcd_block, so that the workgroups of one XCD hold one contiguous
+    // range of tiles. A smaller, persistent grid takes its work items from the
+    // per-XCD dynamic queue.
     const int nwg = gridDim.x;
     const int ntl = t_end - t_begin;
-    const bool persistent = nwg < ntl;
+    const int nwork = ntl * csplit;
+    const bool persistent = nwg < nwork;
     int32_t* queue = const_cast<int32_t*>(hdr->queue[grp]);
     __shared__ int s_tile;
-    // (with more workgroups than tiles, tile floor(v n / nwg) of virtual
-    // workgroup v, so that every XCD still gets tiles)
+    // (with more workgroups than work items, item floor(v n / nwg) of virtual
+    // workgroup v, so that every XCD still gets work)
     const int vb = xcd_block(blockIdx.x, nwg);
-    int next = t_begin + (int)(((int64_t)vb * ntl) / nwg);
-    const int t_hi = t_begin + (int)(((int64_t)(vb + 1) * ntl) / nwg);
+    int next = (int)(((int64_t)vb * nwork) / nwg);
+    const int w_hi = (int)(((int64_t)(vb + 1) * nwork) / nwg);
     for (;;) {
-        int t;
+        int w;
         if (persistent) {
-            if (tid == 0) s_tile = queue_next(queue, ntl);
+            if (tid == 0) s_tile = queue_next(queue, nwork);
             __syncthreads();
-            t = s_tile;  // read by every thread before the tile's closing barrier
-            if (t < 0) break;
-            t += t_begin;
+            w = s_tile;  // read by every thread before the tile's closing barrier
+            if (w < 0) break;
         } else {
-            if (next >= t_hi) break;
-            t = next++;
+            if (next >= w_hi) break;
+            w = next++;
         }
+        const int tq = w / csplit, part = w - tq * csplit;
+        const int t = t_begin + tq;
+        const int c_lo = csplit == 1 ? 0 : part, c_hi = csplit == 1 ? 3 : part + 1;
         int lo = seg_lo, hi = seg_hi;
         while (lo < hi) {
             const int mid = (lo + hi + 1) >> 1;
@@ -689,11 +699,11 @@ __global__ __launch_bounds__(256, SP::kParts == 2 ? 4 : 2) void map_mx_kernel(co
         const ST* base2 = reinterpret_cast<const ST*>(arena + P.arena_off + P.img_bytes + P.scale_off[s]);
         // the 80 x 144 input region in the image: no masks, constant renormalisation
         const bool interior = T.x0 >= 8 && T.x0 + kMxTile + 8 <= T.W && T.y0 >= 8 && T.y0 + kMxTileH + 8 <= T.H;
-        if (interior) mx_tile<SP, ST, false>(L, T, G, base1, base2, L.cum, lane);
-        else mx_tile<SP, ST, true>(L, T, G, base1, base2, L.cum, lane);
+        if (interior) mx_tile<SP, ST, false>(L, T, G, base1, base2, L.cum, lane, c_lo, c_hi);
+        else mx_tile<SP, ST, true>(L, T, G, base1, base2, L.cum, lane, c_lo, c_hi);
 
         __syncthreads();
-        if (tid < kFeat) {
+        if (tid >= 6 * c_lo && tid < 6 * c_hi) {
             const double sum = ((L.red[0][tid] + L.red[1][tid]) + L.red[2][tid]) + L.red[3][tid];
             partials[(int64_t)t * kFeat + tid] = sum;
         }
@@ -703,16 +713,17 @@ __global__ __launch_bounds__(256, SP::kParts == 2 ? 4 : 2) void map_mx_kernel(co
 
 hipError_t launch_map_mx(bool half, bool exact, int grid, const PairDev* pairs, const int32_t* order_seg,
                          const int32_t* order_start, const BatchHdr* hdr, const void* arena, double* partials,
-                         hipStream_t st) {
+                         int csplit, hipStream_t st) {
+    if (csplit != 1 && csplit != 3) return hipErrorInvalidValue;
     const uint8_t* a = reinterpret_cast<const uint8_t*>(arena);
     // float RGB input may hold any value: the exact split; every other format
     // gives bounded XYB values: the scaled fp16 split
     if (exact) {
-        if (half) map_mx_kernel<__half, SplitBf16x3><<<grid, 256, 0, st>>>(pairs, order_seg, order_start, hdr, a, partials);
-        else map_mx_kernel<float, SplitBf16x3><<<grid, 256, 0, st>>>(pairs, order_seg, order_start, hdr, a, partials);
+        if (half) map_mx_kernel<__half, SplitBf16x3><<<grid, 256, 0, st>>>(pairs, order_seg, order_start, hdr, a, partials, csplit);
+        else map_mx_kernel<float, SplitBf16x3><<<grid, 256, 0, st>>>(pairs, order_seg, order_start, hdr, a, partials, csplit);
     } else {
-        if (half) map_mx_kernel<__half, SplitF16x2><<<grid, 256, 0, st>>>(pairs, order_seg, order_start, hdr, a, partials);
-        else map_mx_kernel<float, SplitF16x2><<<grid, 256, 0, st>>>(pairs, order_seg, order_start, hdr, a, partials);
+        if (half) map_mx_kernel<__half, SplitF16x2><<<grid, 256, 0, st>>>(pairs, order_seg, order_start, hdr, a, partials, csplit);
+        else map_mx_kernel<float, SplitF16x2><<<grid, 256, 0, st>>>(pairs, order_seg, order_start, hdr, a, partials, csplit);
     }
     return hipGetLastError();
 }

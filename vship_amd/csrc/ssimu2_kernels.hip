@@ -196,8 +196,8 @@ __device__ __forceinline__ float rgb_sample(const uint8_t* plane, int64_t stride
 // IN_YUV420_U8 or IN_YUV420_U16 (the container), and GEN selects the generic
 // chroma path (any layout and location) over the LDS-staged 4:2:0 one
 // (left-located chroma).
-template <typename ST, int FMT, bool GEN = false>
-__device__ __forceinline__ void prep_quads(PrepLds& L, const PairDev& P, uint8_t* arena, int img, int tx, int ty) {
+template <typename ST, int FMT, bool GEN, bool FULL>
+__device__ __forceinline__ void prep_quads_body(PrepLds& L, const PairDev& P, uint8_t* arena, int img, int tx, int ty) {
     constexpr bool yuv = FMT >= IN_YUV420_U8;
     constexpr bool wide = FMT == IN_YUV420_U16;
     const int tid = tid_in_loop();
@@ -229,7 +229,7 @@ __device__ __forceinline__ void prep_quads(PrepLds& L, const PairDev& P, uint8_t
     const float rcr = 2.0f * (1.0f - kr), bcb = 2.0f * (1.0f - kb);
 
     const int nv1x = min(32, P.sw[1] - tx * 32), nv1y = min(32, P.sh[1] - ty * 32);
-    const bool full = x0 + kPrepTile <= W && y0 + kPrepTile <= H;
+    constexpr bool full = FULL;
     // thread t takes the quads of column t % 32 in rows 4 (t / 32) .. +3 (lanes
     // 2k and 2k+1 still hold horizontally adjacent quads); the staged 4:2:0
     // path keeps the horizontal chroma results of the eight staged rows these
@@ -246,8 +246,8 @@ __device__ __forceinline__ void prep_quads(PrepLds& L, const PairDev& P, uint8_t
                 for (int pl = 0; pl < 2; pl++) chroma420_hrow(L.ch[pl][qy0 + r], qx, che[pl][r], cho[pl][r]);
         }
         const int xa = x0 + 2 * qx, ya = y0 + 2 * qy;
-        if (xa >= W || ya >= H) continue;  // no pixel of this quad is in the image
-        const bool hx = xa + 1 < W, hy = ya + 1 < H;
+        if (!FULL && (xa >= W || ya >= H)) continue;  // no pixel of this quad is in the image
+        const bool hx = FULL || xa + 1 < W, hy = FULL || ya + 1 < H;
         const int xb = hx ? xa + 1 : xa, yb = hy ? ya + 1 : ya;  // clamped second column/row
         float rgb[4][3];  // quad pixels: (xa,ya), (xa,yb), (xb,ya), (xb,yb)
         if constexpr (!yuv) {
@@ -321,7 +321,7 @@ __device__ __forceinline__ void prep_quads(PrepLds& L, const PairDev& P, uint8_t
         }
         // scale-1 child (src/ssimu2/downsample.hpp:3-19 summation order),
         // into LDS: store_scale1 writes its XYB in whole-row vector stores
-        if (qx < nv1x && qy < nv1y) {
+        if (FULL || (qx < nv1x && qy < nv1y)) {
 #pragma unroll
             for (int p = 0; p < 3; p++) {
                 float a = rgb[0][p];
@@ -332,6 +332,16 @@ __device__ __forceinline__ void prep_quads(PrepLds& L, const PairDev& P, uint8_t
             }
         }
     }
+}
+
+// A tile wholly inside the image takes an instantiation in which every quad
+// is complete (no clamps, no selects of the chroma filter outputs).
+template <typename ST, int FMT, bool GEN = false>
+__device__ __forceinline__ void prep_quads(PrepLds& L, const PairDev& P, uint8_t* arena, int img, int tx, int ty) {
+    if ((tx + 1) * kPrepTile <= P.w && (ty + 1) * kPrepTile <= P.h)
+        prep_quads_body<ST, FMT, GEN, true>(L, P, arena, img, tx, ty);
+    else
+        prep_quads_body<ST, FMT, GEN, false>(L, P, arena, img, tx, ty);
 }
 
 // XYB of the tile's 32x32 scale-1 pixels from their linear RGB in LDS: thread

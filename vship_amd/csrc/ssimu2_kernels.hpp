@@ -17,10 +17,12 @@ hipError_t launch_map(bool half, int cls, int th, int grid, const PairDev* pairs
                       const int32_t* order_start, const BatchHdr* hdr, const void* arena, double* partials,
                       const Taps& taps, hipStream_t st);
 // Map tiles of the FP32 class on the matrix cores (map_mfma.hip); exact: the
-// groups of float-input pairs (three-way bf16 split), else the scaled fp16 split.
+// groups of float-input pairs (three-way bf16 split), else the scaled fp16
+// split. csplit: 1 (a workgroup per tile) or 3 (a workgroup per tile and
+// channel; the same partials).
 hipError_t launch_map_mx(bool half, bool exact, int grid, const PairDev* pairs, const int32_t* order_seg,
                          const int32_t* order_start, const BatchHdr* hdr, const void* arena, double* partials,
-                         hipStream_t st);
+                         int csplit, hipStream_t st);
 hipError_t launch_finalize(int grid, const PairDev* pairs, const int32_t* seg_start, const int32_t* seg_count,
                            const BatchHdr* hdr, const double* partials, double* scores, double* features,
                            hipStream_t st);

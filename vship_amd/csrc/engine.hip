@@ -416,10 +416,10 @@ void Engine::enqueue_body(int k) {
     double* feats_d = out_d_ + opt_.max_pairs;
     launch_batch(desc_view(desc_d_, desc_), plan_, S, arena_d_, partials_d_, scores_d, feats_d, taps_, stream_,
                  prof ? ev + 2 : nullptr);
+    // scores [0, max_pairs) and features [max_pairs, ...) are one block on
+    // both sides: one copy brings back the scores and the batch's features
     const int np = opt_.use_graph ? opt_.max_pairs : plan_.npairs;
-    HIP_OR_THROW(hipMemcpyAsync(out_h_[k], scores_d, sizeof(double) * np, hipMemcpyDeviceToHost, stream_),
-                 E_BadDeviceCode);
-    HIP_OR_THROW(hipMemcpyAsync(out_h_[k] + opt_.max_pairs, feats_d, sizeof(double) * 108 * np,
+    HIP_OR_THROW(hipMemcpyAsync(out_h_[k], scores_d, sizeof(double) * (opt_.max_pairs + 108 * (size_t)np),
                                 hipMemcpyDeviceToHost, stream_),
                  E_BadDeviceCode);
     if (prof) HIP_OR_THROW(hipEventRecord(ev[6], stream_), E_BadDeviceCode);

@@ -133,3 +133,30 @@ def test_mfma_yuv10_codes_above_depth_are_clamped():
     say(f"  wild {s[0]:.6f} clamped {s[1]:.6f} oracle {s_o:.6f}")
     assert s[0] == s[1]
     assert abs(s[0] - s_o) <= SCORE_TOL
+
+
+def test_mfma_channel_split_matches_whole_tiles():
+    # a pair alone (379 tiles: one workgroup per tile and channel) against the
+    # same pair in a batch of six (2,274 tiles: one workgroup per tile)
+    clip = synth.video_clip(1920, 1080, 8, 6, 12)
+    pairs = [clip.frame(f) for f in range(6)]
+    clip.close()
+    dp = api.DevicePairs(pairs, api.PF_YUV420_U8, 8, api.MATRIX_BT709)
+    say("case: mfma map, 6 pairs 1920x1080 YUV420P8, each alone vs the batch of six")
+    try:
+        one = (api.Pair * 1)()
+        alone = []
+        with api.SSIMU2Batch(1, 1920, 1080) as b:
+            for i in range(6):
+                one[0] = dp.descs[i]
+                alone.append(b.compute(one, 1, features=True))
+        with api.SSIMU2Batch(6, 1920, 1080) as b:
+            s, f = b.compute(dp.descs, features=True)
+        for i in range(6):
+            assert s[i] == alone[i][0][0], (i, s[i], alone[i][0][0])
+            assert np.array_equal(f[i], alone[i][1][0]), i
+        s_o, _ = ol.score_yuv420(pairs[0][0], pairs[0][1], 8, ol.SO_MAT_BT709)
+        say(f"  scores {np.round(s, 4)}; pair 0 oracle {s_o:.6f}")
+        assert abs(s[0] - s_o) <= SCORE_TOL
+    finally:
+        dp.free()

@@ -91,9 +91,9 @@ typedef struct {
     int max_height;
     int fp16_planes;       /* 1: store XYB scales 0 and 1 in FP16, scales 2-5 in FP32 (FP32 math, FP64 sums) */
     uint64_t arena_bytes;  /* 0: max_pairs * Vship_SSIMU2PairFootprint(max_width, max_height, fp16_planes) */
-    int map_tile_height;   /* 0: tuned shape if recorded, else VSHIP_MAP_MFMA; VSHIP_MAP_MFMA: 64x64 tiles
-                              with the blur on the matrix cores; 48, 64, 112: 32-column tiles of that many
-                              rows on the vector ALUs */
+    int map_tile_height;   /* 0: tuned shape if recorded, else VSHIP_MAP_MFMA; VSHIP_MAP_MFMA: tiles of 64
+                              columns x 128 rows with the blur on the matrix cores; 48, 64, 112: 32-column
+                              tiles of that many rows on the vector ALUs */
     int persistent_grid;   /* 0: tuned shape if recorded, else one workgroup per tile; or a fixed grid */
     int use_graph;         /* 1: capture the batch launch into a hipGraph once and replay it */
     int profile;           /* 1: time each kernel with HIP events (eager launches only) */

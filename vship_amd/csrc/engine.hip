@@ -376,11 +376,15 @@ void launch_batch(const DescView& d, const PlanInfo& P, const LaunchShape& S, vo
                 const int g = map_group(cls, half, exact);
                 if (half && !S.fp16) continue;
                 if (!graph && P.ntiles_grp[g] == 0) continue;
-                // an eager launch of fewer matrix-core tiles than two per
-                // workgroup slot gives each channel of a tile its own workgroup
-                const int csplit =
-                    (cls == 0 && mfma && !graph && S.persistent_grid == 0 && P.ntiles_grp[g] < 8 * S.num_cus) ? 3
-                                                                                                             : 1;
+                // an eager launch of few tiles gives each channel of a tile its
+                // own workgroup (the same partials): fewer than two tiles per
+                // workgroup slot on the matrix-core kernel (4 per CU), and for
+                // the vector kernels (as few as 1 workgroup per CU in the FP64
+                // class) when the split workgroups fit in two rounds, as for
+                // one pair per call
+                const bool small = (cls == 0 && mfma) ? P.ntiles_grp[g] < 8 * S.num_cus
+                                                      : 3 * P.ntiles_grp[g] <= 2 * S.num_cus;
+                const int csplit = (!graph && S.persistent_grid == 0 && small) ? 3 : 1;
                 const int grid = grid_for(P.ntiles_grp[g] * csplit, cls == 0 ? fixed : std::max(1, fixed / 4));
                 if (cls == 0 && mfma)
                     HIP_OR_THROW(launch_map_mx(half, exact, grid, d.pairs, d.order_seg, d.order_start, d.hdr, arena,
@@ -388,7 +392,7 @@ void launch_batch(const DescView& d, const PlanInfo& P, const LaunchShape& S, vo
                                  E_BadDeviceCode);
                 else
                     HIP_OR_THROW(launch_map(half, cls, cls ? kSmallTH : S.map_th, grid, d.pairs, d.order_seg,
-                                            d.order_start, d.hdr, arena, partials, taps, st),
+                                            d.order_start, d.hdr, arena, partials, taps, csplit, st),
                                  E_BadDeviceCode);
             }
         if (ev) HIP_OR_THROW(hipEventRecord(ev[1 + cls], st), E_BadDeviceCode);

@@ -502,7 +502,7 @@ __global__ __launch_bounds__(256, (sizeof(AT) == 4 && TH <= 64) ? 2 : 1) void ma
                                                   const int32_t* __restrict__ order_seg,
                                                   const int32_t* __restrict__ order_start,
                                                   const BatchHdr* __restrict__ hdr, const uint8_t* __restrict__ arena,
-                                                  double* __restrict__ partials, const Taps tp) {
+                                                  double* __restrict__ partials, const Taps tp, int csplit) {
     using C = MapCfg<TH, AT>;
     constexpr int TW = C::TW, HR = C::HR, R = C::R, LDW = C::LDW, OUT = C::OUT, PASS = C::PASS;
     constexpr int NIN = OUT + 16;   // samples a row-pass thread reads
@@ -538,28 +538,35 @@ __global__ __launch_bounds__(256, (sizeof(AT) == 4 && TH <= 64) ? 2 : 1) void ma
     // neighbouring tiles, whose halos overlap, then run on the same XCD and
     // share its L2. A smaller, persistent grid takes its tiles from the
     // per-XCD dynamic queue, which keeps the same ranges.
+    // csplit 3: one workgroup per tile and channel (work item w: tile w / 3,
+    // channel w % 3), for launches with too few tiles to fill the GPU; a
+    // channel's sums do not depend on the others, so the partials are the
+    // same bit for bit as with csplit 1 (a workgroup per tile)
     const int nwg = gridDim.x;
     const int ntl = t_end - t_begin;
-    const bool persistent = nwg < ntl;
+    const int nwork = ntl * csplit;
+    const bool persistent = nwg < nwork;
     int32_t* queue = const_cast<int32_t*>(hdr->queue[grp]);
     __shared__ int s_tile;
-    // (with more workgroups than tiles, tile floor(v n / nwg) of virtual
-    // workgroup v, so that every XCD still gets tiles)
+    // (with more workgroups than work items, item floor(v n / nwg) of virtual
+    // workgroup v, so that every XCD still gets work)
     const int vb = xcd_block(blockIdx.x, nwg);
-    int next = t_begin + (int)(((int64_t)vb * ntl) / nwg);
-    const int t_hi = t_begin + (int)(((int64_t)(vb + 1) * ntl) / nwg);
+    int next = (int)(((int64_t)vb * nwork) / nwg);
+    const int w_hi = (int)(((int64_t)(vb + 1) * nwork) / nwg);
     for (;;) {
-        int t;
+        int w;
         if (persistent) {
-            if (tid == 0) s_tile = queue_next(queue, ntl);
+            if (tid == 0) s_tile = queue_next(queue, nwork);
             __syncthreads();
-            t = s_tile;  // read by every thread before the tile's closing barrier
-            if (t < 0) break;
-            t += t_begin;
+            w = s_tile;  // read by every thread before the tile's closing barrier
+            if (w < 0) break;
         } else {
-            if (next >= t_hi) break;
-            t = next++;
+            if (next >= w_hi) break;
+            w = next++;
         }
+        const int tq = w / csplit, part = w - tq * csplit;
+        const int t = t_begin + tq;
+        const int c_lo = csplit == 1 ? 0 : part, c_hi = csplit == 1 ? 3 : part + 1;
         int lo = seg_lo, hi = seg_hi;
         while (lo < hi) {
             const int mid = (lo + hi + 1) >> 1;
@@ -596,7 +603,7 @@ __global__ __launch_bounds__(256, (sizeof(AT) == 4 && TH <= 64) ? 2 : 1) void ma
         const int xs = x0 + j0 - 8;
 
 #pragma unroll 1
-        for (int c = 0; c < 3; c++) {
+        for (int c = c_lo; c < c_hi; c++) {
             const ST* p1 = base1 + c * pe;
             const ST* p2 = base2 + c * pe;
             const int cxp = min(x0 + TW / 2, W - 1), cyp = min(y0 + TH / 2, H - 1);
@@ -732,7 +739,7 @@ __global__ __launch_bounds__(256, (sizeof(AT) == 4 && TH <= 64) ? 2 : 1) void ma
         }
 
         // ---- tile sums: the four waves in a fixed order
-        if (tid < kFeat) {
+        if (tid >= 6 * c_lo && tid < 6 * c_hi) {
             const double sum = ((red[0][tid] + red[1][tid]) + red[2][tid]) + red[3][tid];
             partials[(int64_t)t * kFeat + tid] = sum;
         }
@@ -833,26 +840,28 @@ hipError_t launch_prep(bool fp16, bool generic, int grid, const PairDev* pairs, 
 
 template <int TH, typename ST, typename AT>
 static void launch_map_t(int grid, const PairDev* pairs, const int32_t* order_seg, const int32_t* order_start,
-                         const BatchHdr* hdr, const void* arena, double* partials, const Taps& taps, hipStream_t st) {
+                         const BatchHdr* hdr, const void* arena, double* partials, const Taps& taps, int csplit,
+                         hipStream_t st) {
     map_kernel<TH, ST, AT><<<grid, 256, 0, st>>>(pairs, order_seg, order_start, hdr,
-                                                 reinterpret_cast<const uint8_t*>(arena), partials, taps);
+                                                 reinterpret_cast<const uint8_t*>(arena), partials, taps, csplit);
 }
 
 hipError_t launch_map(bool half, int cls, int th, int grid, const PairDev* pairs, const int32_t* order_seg,
                       const int32_t* order_start, const BatchHdr* hdr, const void* arena, double* partials,
-                      const Taps& taps, hipStream_t st) {
+                      const Taps& taps, int csplit, hipStream_t st) {
+    if (csplit != 1 && csplit != 3) return hipErrorInvalidValue;
     if (cls == 1) {
-        if (half) launch_map_t<kSmallTH, __half, double>(grid, pairs, order_seg, order_start, hdr, arena, partials, taps, st);
-        else launch_map_t<kSmallTH, float, double>(grid, pairs, order_seg, order_start, hdr, arena, partials, taps, st);
+        if (half) launch_map_t<kSmallTH, __half, double>(grid, pairs, order_seg, order_start, hdr, arena, partials, taps, csplit, st);
+        else launch_map_t<kSmallTH, float, double>(grid, pairs, order_seg, order_start, hdr, arena, partials, taps, csplit, st);
     } else if (th == 112) {
-        if (half) launch_map_t<112, __half, float>(grid, pairs, order_seg, order_start, hdr, arena, partials, taps, st);
-        else launch_map_t<112, float, float>(grid, pairs, order_seg, order_start, hdr, arena, partials, taps, st);
+        if (half) launch_map_t<112, __half, float>(grid, pairs, order_seg, order_start, hdr, arena, partials, taps, csplit, st);
+        else launch_map_t<112, float, float>(grid, pairs, order_seg, order_start, hdr, arena, partials, taps, csplit, st);
     } else if (th == 64) {
-        if (half) launch_map_t<64, __half, float>(grid, pairs, order_seg, order_start, hdr, arena, partials, taps, st);
-        else launch_map_t<64, float, float>(grid, pairs, order_seg, order_start, hdr, arena, partials, taps, st);
+        if (half) launch_map_t<64, __half, float>(grid, pairs, order_seg, order_start, hdr, arena, partials, taps, csplit, st);
+        else launch_map_t<64, float, float>(grid, pairs, order_seg, order_start, hdr, arena, partials, taps, csplit, st);
     } else {
-        if (half) launch_map_t<48, __half, float>(grid, pairs, order_seg, order_start, hdr, arena, partials, taps, st);
-        else launch_map_t<48, float, float>(grid, pairs, order_seg, order_start, hdr, arena, partials, taps, st);
+        if (half) launch_map_t<48, __half, float>(grid, pairs, order_seg, order_start, hdr, arena, partials, taps, csplit, st);
+        else launch_map_t<48, float, float>(grid, pairs, order_seg, order_start, hdr, arena, partials, taps, csplit, st);
     }
     return hipGetLastError();
 }

@@ -15,7 +15,7 @@ hipError_t launch_prep(bool fp16, bool generic, int grid, const PairDev* pairs, 
 // planes are FP16.
 hipError_t launch_map(bool half, int cls, int th, int grid, const PairDev* pairs, const int32_t* order_seg,
                       const int32_t* order_start, const BatchHdr* hdr, const void* arena, double* partials,
-                      const Taps& taps, hipStream_t st);
+                      const Taps& taps, int csplit, hipStream_t st);
 // Map tiles of the FP32 class on the matrix cores (map_mfma.hip); exact: the
 // groups of float-input pairs (three-way bf16 split), else the scaled fp16
 // split. csplit: 1 (a workgroup per tile) or 3 (a workgroup per tile and

@@ -801,8 +801,20 @@ __global__ __launch_bounds__(256) void finalize_kernel(const PairDev* __restrict
             const int t0 = seg_start[p * kScales + s], t1 = t0 + seg_count[p * kScales + s];
             if (tid < 14 * kFeat) {
                 const int k = tid % kFeat, g = tid / kFeat;
+                // tiles t0 + g, t0 + g + 14, ... summed in that order; the
+                // loads go eight at a time, so that one wait covers eight
+                // tiles instead of one
+                const int n = t0 + g < t1 ? (t1 - t0 - g + 13) / 14 : 0;
+                const double* src = partials + (int64_t)(t0 + g) * kFeat + k;
                 double a = 0.0;
-                for (int t = t0 + g; t < t1; t += 14) a += partials[(int64_t)t * kFeat + k];
+                for (int i = 0; i < n; i += 8) {
+                    double v[8];
+#pragma unroll
+                    for (int u = 0; u < 8; u++) v[u] = i + u < n ? src[(int64_t)(i + u) * 14 * kFeat] : 0.0;
+#pragma unroll
+                    for (int u = 0; u < 8; u++)
+                        if (i + u < n) a += v[u];
+                }
                 part[g][k] = a;
             }
             __syncthreads();

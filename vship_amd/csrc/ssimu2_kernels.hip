@@ -373,9 +373,10 @@ __device__ __forceinline__ void store_scale1(const PrepLds& L, const PairDev& P,
 // staged 4:2:0 pairs, GEN = true the pairs of the generic chroma path; each
 // skips the other's tiles. Both are held to seven waves per SIMD (at most 72
 // VGPRs, no spills: the thread-index arithmetic is recomputed per use rather
-// than held across the tile loop, see tid_in_loop): at 1920x1080 the kernel,
-// bound by its stores, takes 3.95 ms at seven waves per SIMD, 4.24 ms at six
-// and 4.38 ms at five.
+// than held across the tile loop, see tid_in_loop), which overlaps one
+// tile's stores with another's arithmetic: at 1920x1080 (240 pairs) the
+// kernel took 3.95 ms at seven waves per SIMD, 4.24 ms at six and 4.38 ms at
+// five, before the full-tile loop took it to 3.78 ms.
 template <typename ST, bool GEN>
 __global__ __launch_bounds__(256, 7) void prep_kernel(const PairDev* __restrict__ pairs,
                                                    const int32_t* __restrict__ prep_start,
